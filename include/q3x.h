@@ -1,0 +1,168 @@
+/*
+ * q3x.h -- C ABI of the MI355X-native H|psi> + block-Davidson library (libq3x.so).
+ *
+ * The library replaces the plane-wave Hamiltonian-application and Davidson path of
+ * camsai/q3.  Each entry point names the reference interface it stands in for
+ * (paths relative to the camsai/q3 source tree).
+ *
+ * Conventions (the reference's, kept so the library drops in):
+ *   - complex numbers are interleaved (re, im) doubles, i.e. std::complex<double>;
+ *   - band blocks are column-major, one band per column, leading dimension npw
+ *     (src/layer_3_representation/quantum_system/kohn_sham/planewave/init.hpp:120-138);
+ *   - projector sets are npw x nproj, one projector after another
+ *     (src/layer_3_representation/pseudopotential/planewave/kleinman_bylander_projector_set.hpp:20-23);
+ *   - energies are in Rydberg; lengths in bohr; lattice vectors in units of alat;
+ *   - every call returns a q3x_status; on failure q3x_last_error() holds a message
+ *     (thread-local).  No C++ exception crosses the ABI.
+ *   - handles are not thread-safe; one context per host thread (the reference's
+ *     solver is not thread-safe either: src/bindings/q3_core_api.hpp:12-14).
+ */
+#ifndef Q3X_H
+#define Q3X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int q3x_status;
+enum {
+    Q3X_OK = 0,
+    Q3X_ERR_ARGUMENT = 1,
+    Q3X_ERR_DEVICE = 2,
+    Q3X_ERR_STATE = 3,    /* e.g. apply before prepare_kpoint (serial.hpp:487-492 logic_error) */
+    Q3X_ERR_NUMERIC = 4,  /* eigensolver / Cholesky breakdown that recovery could not fix */
+    Q3X_ERR_INTERNAL = 5
+};
+
+/* precision of the H-apply / Davidson products */
+enum {
+    Q3X_FP64 = 0,  /* everything complex128 */
+    Q3X_MIXED = 1  /* FFT, V multiply and projector GEMMs in complex64 with fp32 accumulation;
+                      Cholesky, Rayleigh-Ritz and final residual sweep in fp64 */
+};
+
+typedef struct q3x_context_s* q3x_context;
+typedef struct q3x_basis_s* q3x_basis;
+typedef struct q3x_kpoint_s* q3x_kpoint;
+typedef struct q3x_hamiltonian_s* q3x_hamiltonian;
+
+const char* q3x_last_error(void);
+const char* q3x_version(void);
+
+/* ---- device context (stands in for GPUDeviceBackend,
+ *      src/layer_0_core/execution/gpu/device/init.hpp; explicit hipMalloc, no managed memory) */
+q3x_status q3x_context_create(int device, q3x_context* out);
+q3x_status q3x_context_destroy(q3x_context ctx);
+q3x_status q3x_context_synchronize(q3x_context ctx);
+
+/* ---- plane-wave basis (PlanewaveBasis::generate,
+ *      src/layer_3_representation/basis/planewave/init.hpp:60-109, with
+ *      PlanewaveFFTPair::initialize .../internal/fft/pair.hpp:24-100 and
+ *      GVectorsGrid::generate .../internal/g_vectors_grid.hpp:23-165).
+ *      fft_factors replaces spglib's compute_fft_factors (grid_sizer.hpp:83-94): the caller
+ *      passes the divisibility factors.  ecutrho_ry <= 0 means 4 x ecutwfc (init.hpp:70-72).
+ *      Host-only: needs no device. */
+q3x_status q3x_basis_create(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
+                            double ecutrho_ry, int use_exact_wavefunction_grid,
+                            const int fft_factors[3], q3x_basis* out);
+q3x_status q3x_basis_info(q3x_basis b, int rho_dims[3], int wave_dims[3], int* number_of_g_vectors,
+                          double* cell_volume, double* two_pi_over_alat_squared);
+/* miller: ng x 3 ints; g_cart: ng x 3 (2pi/alat units); g2: ng (2pi/alat)^2 units; any may be NULL */
+q3x_status q3x_basis_gvectors(q3x_basis b, int* miller, double* g_cart, double* g2);
+q3x_status q3x_basis_destroy(q3x_basis b);
+
+/* ---- k-point plane-wave list (PlaneWaveSelector::select,
+ *      .../internal/kpoint/plane_wave_selector.hpp:41-109; FFT map of
+ *      compute_g_to_fft_index_mapping, hamiltonian_applicator/planewave_potential_utils.hpp:141-164).
+ *      Host-only. */
+q3x_status q3x_kpoint_create(q3x_basis b, const double k_crystal[3], q3x_kpoint* out, int* npw);
+/* g_index: npw ints into the basis G list; kinetic: |k+G|^2 in Ry; fft_map: wave-grid offsets */
+q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* fft_map);
+q3x_status q3x_kpoint_destroy(q3x_kpoint k);
+
+/* ---- Kleinman-Bylander projector set at one k-point (compute_kleinman_bylander_projectors,
+ *      src/layer_3_representation/pseudopotential/planewave/internal/projector.hpp:42-199).
+ *      Species arrays are concatenated over species: nmesh[s] radial points (r, rab),
+ *      nbeta[s] projectors with angular momentum l and cutoff index, radial values r*beta(r)
+ *      (UPF convention), nbeta[s] x nmesh[s] per species.  atom_pos in crystal coordinates.
+ *      beta_out: npw x nproj complex (caller-sized; *nproj returns the count).  Host-only. */
+q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, const double* atom_pos,
+                             const int* atom_species, int nspecies, const int* nmesh,
+                             const double* r_all, const double* rab_all, const int* nbeta,
+                             const int* l_all, const int* cutoff_all, const double* beta_r_all,
+                             double* beta_out, int* nproj);
+
+/* ---- Hamiltonian applicator (PlanewaveHamiltonianApplicator,
+ *      src/layer_4_computation/diagonalizer/planewave/internal/hamiltonian_applicator/init.hpp,
+ *      serial semantics of .../serial.hpp:64-155) */
+q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian* out);
+q3x_status q3x_hamiltonian_destroy(q3x_hamiltonian h);
+/* update_potential (serial.hpp:64-78): V(r) on the wave grid, Ry, C order (i1,i2,i3), n points.
+ * The library keeps V/N, as the reference does. */
+q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v_r, long n);
+/* prepare_kpoint (serial.hpp:84-103) */
+q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k);
+/* projector set of the prepared k-point + per-atom D blocks (serial.hpp:587-628):
+ * beta npw x nproj complex; block_sizes per atom; d_blocks real, column-major blocks packed in
+ * atom order (Ry).  |D| <= 1e-15 is dropped as the reference does (tolerances.hpp:99). */
+q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta, int nproj,
+                                          int natoms, const int* block_sizes,
+                                          const double* d_blocks);
+/* apply_batch (serial.hpp:110-155): host buffers psi / hpsi, npw x nvec complex. */
+q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* hpsi, int nvec,
+                                 int precision);
+/* same on device buffers (complex128, ld npw) already resident in HBM; stream-ordered on the
+ * context stream, returns after queueing (call q3x_context_synchronize to wait). */
+q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi,
+                                        int nvec, int precision);
+/* device scratch helpers for callers that keep psi resident */
+q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out);
+q3x_status q3x_device_free(q3x_context ctx, void* p);
+q3x_status q3x_copy_to_device(q3x_context ctx, void* dst, const void* src, unsigned long bytes);
+q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigned long bytes);
+
+/* ---- per-kernel timing on the context stream (HIP events around each launch; the
+ *      reference's ProfilingCounters / StageTimer, src/layer_0_core/profiling/counters.hpp).
+ *      Kernel ids: 0 zsticks_inverse, 1 plane_sandwich, 2 zsticks_forward, 3 kb_overlap_gemm,
+ *      4 kb_dblock, 5 kb_update_gemm. */
+q3x_status q3x_profile_enable(q3x_context ctx, int enable);
+/* synchronizes the stream; fills up to n totals (ms) and launch counts; reset != 0 zeroes them */
+q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* count, int reset);
+const char* q3x_profile_kernel_name(int id);
+
+/* ---- block Davidson (PlanewaveDavidsonDiagonalizer::solve,
+ *      src/layer_4_computation/diagonalizer/planewave/davidson.hpp:324-841, with the subspace
+ *      manager of .../internal/subspace_manager/serial.hpp) */
+typedef struct {
+    double tolerance_ry;      /* set_tolerance; eigenvalue-change test (davidson.hpp:607-629) */
+    int max_iterations;       /* set_maximum_iterations (default 20, davidson.hpp:1156) */
+    int basis_factor;         /* set_basis_expansion_factor (default 2, davidson.hpp:864) */
+    int n_occupied;           /* number_of_occupied_bands_per_spin */
+    int precision;            /* Q3X_FP64 or Q3X_MIXED */
+    int seed_from_psi;        /* 1: psi_inout seeds the basis (davidson.hpp:1000-1045);
+                                 0: kinetic-sorted unit vectors (davidson.hpp:1047-1101) */
+    int final_fp64_sweep;     /* MIXED only: finish with fp64 Rayleigh-Ritz + residual sweep */
+} q3x_davidson_options;
+
+typedef struct {
+    int iterations;
+    int unconverged;          /* number_of_unconverged_bands (davidson.hpp:270-276) */
+    int h_applications;
+    int fp64_refinements;     /* fp64 sweeps run after a mixed solve */
+    double max_residual_norm; /* max_n ||H psi_n - e_n psi_n|| after the solve (fp64) */
+    /* stage times in ms, named like DavidsonTimingAccumulator
+     * (src/layer_0_core/profiling/counters.hpp:36-52):
+     * 0 seed, 1 orthogonalize, 2 hamiltonian_application, 3 subspace_update,
+     * 4 diagonalize, 5 collapse, 6 residual, 7 total */
+    double stage_ms[8];
+} q3x_davidson_report;
+
+void q3x_davidson_default_options(q3x_davidson_options* opt);
+/* psi_inout: host, npw x nb complex; eig_out: nb (Ry, ascending). */
+q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
+                              double* psi_inout, double* eig_out, q3x_davidson_report* report);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* Q3X_H */

@@ -1,0 +1,233 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/q3_oracle.cpp).
+
+Test infrastructure only: tests/, ``__graft_entry__.smoke()`` and bench.py's
+cpu_baseline leg call this module as the checker.  The product library never
+imports it.  Builds with ``make -C oracle`` (``build()`` in __graft_entry__ does it).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libq3oracle.so")
+_lib = None
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_good_fft_order.restype = _i
+        L.orc_good_fft_order.argtypes = [_i, _i]
+        L.orc_crystal.argtypes = [_p, _d, _p, _p, _p]
+        L.orc_grid_dims.argtypes = [_p, _d, _d, _p, _p]
+        L.orc_basis_create.restype = _p
+        L.orc_basis_create.argtypes = [_p, _d, _d, _d, _i, _p]
+        L.orc_basis_destroy.argtypes = [_p]
+        L.orc_basis_info.argtypes = [_p, _p, _p, _p]
+        L.orc_basis_gvectors.argtypes = [_p, _p, _p, _p]
+        L.orc_kpoint.restype = _i
+        L.orc_kpoint.argtypes = [_p, _p, _p, _p, _p, _i]
+        L.orc_fft3d.argtypes = [_p, _p, _i]
+        L.orc_hamiltonian_apply.argtypes = [_p, _i, _p, _p, _p, _i, _p, _i, _p, _p, _i, _p, _p]
+        L.orc_beta_table_size.restype = _i
+        L.orc_beta_table_size.argtypes = [_d]
+        L.orc_beta_table.argtypes = [_i, _p, _p, _i, _i, _p, _i, _p]
+        L.orc_sph_bessel.restype = _d
+        L.orc_sph_bessel.argtypes = [_i, _d]
+        L.orc_ylm.argtypes = [_i, _i, _p, _p]
+        L.orc_kb_projectors.restype = _i
+        L.orc_kb_projectors.argtypes = [_p, _p, _i, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
+        L.orc_zhegv.restype = _i
+        L.orc_zhegv.argtypes = [_i, _p, _p, _p, _p]
+        L.orc_davidson.restype = _i
+        L.orc_davidson.argtypes = [_p, _i, _p, _p, _p, _i, _p, _i, _p, _p, _i, _i, _p, _i, _p, _d,
+                                   _i, _i, _p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def good_fft_order(n: int, factor: int = 1) -> int:
+    return lib().orc_good_fft_order(n, factor)
+
+
+def crystal(lat, alat):
+    lat = _c(lat, np.float64)
+    recip = np.zeros(9)
+    vol = np.zeros(1)
+    tpa2 = np.zeros(1)
+    lib().orc_crystal(_ptr(lat), alat, _ptr(recip), _ptr(vol), _ptr(tpa2))
+    return recip.reshape(3, 3), float(vol[0]), float(tpa2[0])
+
+
+def grid_dims(lat, alat, ecut, factors=(1, 1, 1)):
+    lat = _c(lat, np.float64)
+    f = _c(factors, np.int32)
+    dims = np.zeros(3, np.int32)
+    lib().orc_grid_dims(_ptr(lat), alat, ecut, _ptr(f), _ptr(dims))
+    return tuple(int(x) for x in dims)
+
+
+class Basis:
+    """G sphere and FFT grids (PlanewaveBasis::generate)."""
+
+    def __init__(self, lat, alat, ecutwfc, ecutrho=0.0, use_exact=True, factors=(1, 1, 1)):
+        self._lat = _c(lat, np.float64)
+        f = _c(factors, np.int32)
+        self.h = lib().orc_basis_create(_ptr(self._lat), alat, ecutwfc, ecutrho, int(use_exact), _ptr(f))
+        rho = np.zeros(3, np.int32)
+        wave = np.zeros(3, np.int32)
+        ng = np.zeros(1, np.int32)
+        lib().orc_basis_info(self.h, _ptr(rho), _ptr(wave), _ptr(ng))
+        self.rho_dims = tuple(int(x) for x in rho)
+        self.wave_dims = tuple(int(x) for x in wave)
+        self.ng = int(ng[0])
+
+    def gvectors(self):
+        m = np.zeros((self.ng, 3), np.int32)
+        g = np.zeros((self.ng, 3))
+        g2 = np.zeros(self.ng)
+        lib().orc_basis_gvectors(self.h, _ptr(m), _ptr(g), _ptr(g2))
+        return m, g, g2
+
+    def kpoint(self, k):
+        k = _c(k, np.float64)
+        cap = self.ng
+        gidx = np.zeros(cap, np.int32)
+        ek = np.zeros(cap)
+        fm = np.zeros(cap, np.int32)
+        n = lib().orc_kpoint(self.h, _ptr(k), _ptr(gidx), _ptr(ek), _ptr(fm), cap)
+        return gidx[:n].copy(), ek[:n].copy(), fm[:n].copy()
+
+    def kb_projectors(self, k, gidx, atom_pos, atom_species, species):
+        """species: list of dicts with r, rab, l (list), cutoff (list), beta_r (nbeta x nmesh)."""
+        k = _c(k, np.float64)
+        gidx = _c(gidx, np.int32)
+        pos = _c(atom_pos, np.float64)
+        asp = _c(atom_species, np.int32)
+        nmesh = _c([len(s["r"]) for s in species], np.int32)
+        r_all = _c(np.concatenate([s["r"] for s in species]), np.float64)
+        rab_all = _c(np.concatenate([s["rab"] for s in species]), np.float64)
+        nbeta = _c([len(s["l"]) for s in species], np.int32)
+        l_all = _c(np.concatenate([s["l"] for s in species]), np.int32)
+        cut_all = _c(np.concatenate([s["cutoff"] for s in species]), np.int32)
+        br_all = _c(np.concatenate([np.asarray(s["beta_r"]).ravel() for s in species]), np.float64)
+        nproj = sum(sum(2 * l + 1 for l in species[s]["l"]) for s in asp)
+        out = np.zeros((nproj, len(gidx)), np.complex128)
+        n = lib().orc_kb_projectors(self.h, _ptr(k), len(gidx), _ptr(gidx), len(asp), _ptr(pos), _ptr(asp),
+                                    len(species), _ptr(nmesh), _ptr(r_all), _ptr(rab_all), _ptr(nbeta),
+                                    _ptr(l_all), _ptr(cut_all), _ptr(br_all), _ptr(out))
+        assert n == nproj
+        return out.T.copy(order="F")  # npw x nproj, column-major
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_basis_destroy(self.h)
+            self.h = None
+
+
+def fft3d(data: np.ndarray, sign: int) -> np.ndarray:
+    a = _c(data, np.complex128).copy()
+    dims = _c(a.shape, np.int32)
+    lib().orc_fft3d(_ptr(dims), _ptr(a), sign)
+    return a
+
+
+def beta_table(r, rab, l, cutoff, beta_r, ecutrho):
+    nq = lib().orc_beta_table_size(ecutrho)
+    out = np.zeros(nq)
+    r = _c(r, np.float64)
+    rab = _c(rab, np.float64)
+    br = _c(beta_r, np.float64)
+    lib().orc_beta_table(len(r), _ptr(r), _ptr(rab), l, cutoff, _ptr(br), nq, _ptr(out))
+    return out
+
+
+def sph_bessel(l, x):
+    return lib().orc_sph_bessel(l, x)
+
+
+def ylm(lmax, g):
+    g = _c(g, np.float64)
+    n = g.shape[0]
+    out = np.zeros((n, (lmax + 1) ** 2))
+    lib().orc_ylm(lmax, n, _ptr(g), _ptr(out))
+    return out
+
+
+def _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks):
+    dims = _c(dims, np.int32)
+    fm = _c(fftmap, np.int32)
+    ek = _c(ekin, np.float64)
+    Vr = _c(V, np.float64).ravel()
+    if beta is None or beta.size == 0:
+        b = np.zeros(1, np.complex128)
+        nproj = 0
+    else:
+        b = np.asfortranarray(beta, dtype=np.complex128)
+        nproj = beta.shape[1]
+    bs = _c(block_sizes if block_sizes is not None else [], np.int32)
+    db = _c(dblocks if dblocks is not None else [0.0], np.float64)
+    return dims, fm, ek, Vr, b, nproj, bs, db
+
+
+def hamiltonian_apply(dims, fftmap, ekin, V, beta, block_sizes, dblocks, psi):
+    """psi: npw x nvec (column-major); returns H psi of the same shape."""
+    dims, fm, ek, Vr, b, nproj, bs, db = _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks)
+    psi = np.asfortranarray(psi, dtype=np.complex128)
+    npw, nvec = psi.shape
+    out = np.zeros_like(psi, order="F")
+    lib().orc_hamiltonian_apply(_ptr(dims), npw, _ptr(fm), _ptr(ek), _ptr(Vr), nproj, _ptr(b), len(bs),
+                                _ptr(bs), _ptr(db), nvec, _ptr(psi), _ptr(out))
+    return out
+
+
+def zhegv(H, S):
+    H = np.asfortranarray(H, dtype=np.complex128)
+    S = np.asfortranarray(S, dtype=np.complex128)
+    n = H.shape[0]
+    w = np.zeros(n)
+    X = np.zeros((n, n), np.complex128, order="F")
+    info = lib().orc_zhegv(n, _ptr(H), _ptr(S), _ptr(w), _ptr(X))
+    return info, w, X
+
+
+def davidson(dims, fftmap, ekin, V, beta, block_sizes, dblocks, nb, nocc, psi0=None, tol=1e-8,
+             max_iter=20, factor=2):
+    """Returns (eigenvalues, psi (npw x nb), report dict)."""
+    dims, fm, ek, Vr, b, nproj, bs, db = _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks)
+    npw = len(ek)
+    if psi0 is None:
+        psi = np.zeros((npw, nb), np.complex128, order="F")
+        seed = 0
+    else:
+        psi = np.asfortranarray(psi0, dtype=np.complex128).copy(order="F")
+        seed = 1
+    eig = np.zeros(nb)
+    rep = np.zeros(3, np.int32)
+    lib().orc_davidson(_ptr(dims), npw, _ptr(fm), _ptr(ek), _ptr(Vr), nproj, _ptr(b), len(bs), _ptr(bs),
+                       _ptr(db), nb, nocc, _ptr(psi), seed, _ptr(eig), tol, max_iter, factor, _ptr(rep))
+    return eig, psi, {"iterations": int(rep[0]), "unconverged": int(rep[1]), "h_applications": int(rep[2])}

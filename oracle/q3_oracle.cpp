@@ -1,0 +1,1329 @@
+// q3_oracle.cpp -- FP64 CPU restatement of camsai/q3's H|psi> + Davidson path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: tests/, the smoke check
+// and bench.py's cpu_baseline leg load it as a checker.  The product library
+// (q3_amd/csrc) never links, loads or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the camsai/q3 source tree).  Floating-point operation order follows the
+// reference where it fixes bit-exact results (lattice inverse, |G|^2, the sort
+// comparators); the file is compiled with -ffp-contract=off and no -march, so
+// no FMA is formed, as in an x86-64 build of the reference.
+//
+// Parity status per function (how each is pinned; details in the tests):
+//   orc_good_fft_order / orc_grid_dims   reference known answers (test_grid_sizer.cpp)
+//   orc_basis_* (G sphere, sort)          brute-force counts + sortedness + G=0 first;
+//                                         tie order = libstdc++ std::sort on the reference's
+//                                         input sequence (only a compiled reference run
+//                                         could pin it further)
+//   orc_kpoint (selector, maps)           brute force + sortedness + Gamma-first
+//   orc_fft3d                             numpy.fft (FFTW sign/normalisation convention)
+//   orc_hamiltonian_apply                 dense brute-force H on tiny cells, closed forms
+//   orc_kb_projectors                     Gaussian closed form of the Bessel transform,
+//                                         scipy Y_lm / j_l
+//   orc_zhegv                             scipy.linalg.eigh on random pencils
+//   orc_davidson                          dense eigh of the brute-force H (cfg1-sized)
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using cplx = std::complex<double>;
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;      // constants.hpp:24
+constexpr double kTwoPi = 2.0 * kPi;                // constants.hpp:25
+constexpr double kFourPi = 4.0 * kPi;               // constants.hpp:26
+constexpr double kEcutTol = 1e-8;                   // tolerances.hpp:129
+constexpr double kZeroVec = 1e-10;                  // tolerances.hpp:22
+constexpr double kNormThreshold = 1e-14;            // tolerances.hpp:39
+constexpr double kPrecondFloor = 1e-4;              // tolerances.hpp:50
+constexpr double kEmptyFactor = 5.0;                // tolerances.hpp:65
+constexpr double kEmptyFloor = 1e-5;                // tolerances.hpp:66
+constexpr double kDThreshold = 1e-15;               // tolerances.hpp:99
+constexpr double kBesselZero = 1e-8;                // tolerances.hpp:158
+constexpr double kQSpacing = 0.01;                  // tolerances.hpp:109
+
+struct V3 {
+    double x, y, z;
+    double& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    // vector_3d.hpp:59-61
+    double dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+    double norm2() const { return dot(*this); }
+};
+struct M3 {
+    V3 r[3];
+};
+
+// matrix_3x3.hpp:51-55
+double det3(const M3& m) {
+    const V3* rows = m.r;
+    return rows[0].x * (rows[1].y * rows[2].z - rows[1].z * rows[2].y) -
+           rows[0].y * (rows[1].x * rows[2].z - rows[1].z * rows[2].x) +
+           rows[0].z * (rows[1].x * rows[2].y - rows[1].y * rows[2].x);
+}
+
+// matrix_3x3.hpp:58-71
+M3 inv3(const M3& m) {
+    const V3* rows = m.r;
+    double d = det3(m);
+    M3 o;
+    o.r[0] = {(rows[1].y * rows[2].z - rows[1].z * rows[2].y) / d,
+              (rows[0].z * rows[2].y - rows[0].y * rows[2].z) / d,
+              (rows[0].y * rows[1].z - rows[0].z * rows[1].y) / d};
+    o.r[1] = {(rows[1].z * rows[2].x - rows[1].x * rows[2].z) / d,
+              (rows[0].x * rows[2].z - rows[0].z * rows[2].x) / d,
+              (rows[0].z * rows[1].x - rows[0].x * rows[1].z) / d};
+    o.r[2] = {(rows[1].x * rows[2].y - rows[1].y * rows[2].x) / d,
+              (rows[0].y * rows[2].x - rows[0].x * rows[2].y) / d,
+              (rows[0].x * rows[1].y - rows[0].y * rows[1].x) / d};
+    return o;
+}
+
+M3 transpose3(const M3& m) {
+    M3 t;
+    t.r[0] = {m.r[0].x, m.r[1].x, m.r[2].x};
+    t.r[1] = {m.r[0].y, m.r[1].y, m.r[2].y};
+    t.r[2] = {m.r[0].z, m.r[1].z, m.r[2].z};
+    return t;
+}
+
+// crystal.hpp:156-177
+struct Crystal {
+    double alat = 0, tpa = 0, tpa2 = 0, volume = 0;
+    M3 lat{}, recip{};
+    void init(const double* lat9, double a) {
+        alat = a;
+        for (int i = 0; i < 3; ++i) lat.r[i] = {lat9[3 * i], lat9[3 * i + 1], lat9[3 * i + 2]};
+        tpa = kTwoPi / alat;
+        tpa2 = tpa * tpa;
+        double vf = det3(lat);
+        volume = std::abs(vf) * alat * alat * alat;
+        recip = transpose3(inv3(lat));
+    }
+};
+
+// grid_sizer.hpp:173-192
+int good_fft_order(int n, int factor) {
+    int c = n;
+    while (true) {
+        if (factor > 1 && c % factor != 0) {
+            c++;
+            continue;
+        }
+        int rem = c;
+        while (rem % 2 == 0) rem /= 2;
+        while (rem % 3 == 0) rem /= 3;
+        while (rem % 5 == 0) rem /= 5;
+        if (rem == 1) return c;
+        c++;
+    }
+}
+
+// grid_sizer.hpp:101-160
+void grid_dimensions(double ecut, double tpa2, const M3& lat, const M3& recip, const int* fac,
+                     int* dims) {
+    double g2cut = ecut / tpa2;
+    double sq = std::sqrt(g2cut);
+    int est[3];
+    for (int d = 0; d < 3; ++d) est[d] = static_cast<int>(std::floor(sq * std::sqrt(lat.r[d].norm2()))) + 1;
+    int mx[3] = {0, 0, 0};
+    for (int i3 = -est[2]; i3 <= est[2]; i3++)
+        for (int i2 = -est[1]; i2 <= est[1]; i2++)
+            for (int i1 = -est[0]; i1 <= est[0]; i1++) {
+                V3 g;
+                for (int d = 0; d < 3; d++)
+                    g[d] = i1 * recip.r[0][d] + i2 * recip.r[1][d] + i3 * recip.r[2][d];
+                double g2 = g.norm2();
+                if (g2 < g2cut + kEcutTol) {
+                    mx[0] = std::max(mx[0], std::abs(i1));
+                    mx[1] = std::max(mx[1], std::abs(i2));
+                    mx[2] = std::max(mx[2], std::abs(i3));
+                }
+            }
+    for (int d = 0; d < 3; ++d) dims[d] = good_fft_order(2 * mx[d] + 1, fac ? fac[d] : 1);
+}
+
+struct GVec {
+    int m[3];
+    V3 g;
+    double g2;
+};
+
+struct Basis {
+    Crystal c;
+    double ecutwfc = 0, ecutrho = 0;
+    int rho[3] = {0, 0, 0}, wave[3] = {0, 0, 0};
+    std::vector<GVec> gv;
+};
+
+// g_vectors_grid.hpp:23-104 (serial, full sphere)
+void generate_gvectors(Basis& b) {
+    const M3& R = b.c.recip;
+    double g2cut = b.ecutrho / b.c.tpa2;
+    int mx1 = b.rho[0] / 2, mx2 = b.rho[1] / 2, mx3 = b.rho[2] / 2;
+    b.gv.clear();
+    for (int i1 = -mx1; i1 <= mx1; i1++)
+        for (int i2 = -mx2; i2 <= mx2; i2++)
+            for (int i3 = -mx3; i3 <= mx3; i3++) {
+                V3 g;
+                for (int d = 0; d < 3; d++) g[d] = i1 * R.r[0][d] + i2 * R.r[1][d] + i3 * R.r[2][d];
+                double g2 = g.norm2();
+                if (g2 <= g2cut + kEcutTol) b.gv.push_back({{i1, i2, i3}, g, g2});
+            }
+    std::sort(b.gv.begin(), b.gv.end(),
+              [](const GVec& a, const GVec& c) { return a.g2 < c.g2; });
+}
+
+struct KPoint {
+    std::vector<int> gidx;
+    std::vector<double> ekin;
+    std::vector<int> fftmap;
+};
+
+// plane_wave_selector.hpp:41-109 and planewave_potential_utils.hpp:141-164
+void select_kpoint(const Basis& b, const double* kc, KPoint& kp) {
+    const M3& R = b.c.recip;
+    V3 k;
+    for (int d = 0; d < 3; d++) k[d] = kc[0] * R.r[0][d] + kc[1] * R.r[1][d] + kc[2] * R.r[2][d];
+    double wcut = b.ecutwfc / b.c.tpa2;
+    std::vector<int> idx;
+    std::vector<double> ek;
+    for (int g = 0; g < static_cast<int>(b.gv.size()); g++) {
+        V3 kg{k.x + b.gv[g].g.x, k.y + b.gv[g].g.y, k.z + b.gv[g].g.z};
+        double k2 = kg.norm2();
+        if (k2 <= wcut + kEcutTol) {
+            idx.push_back(g);
+            ek.push_back(k2 * b.c.tpa2);
+        }
+    }
+    std::vector<int> order(idx.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&ek](int a, int c) { return ek[a] < ek[c]; });
+    kp.gidx.resize(idx.size());
+    kp.ekin.resize(idx.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        kp.gidx[i] = idx[order[i]];
+        kp.ekin[i] = ek[order[i]];
+    }
+    const int* d = b.wave;
+    kp.fftmap.resize(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        const GVec& gvec = b.gv[kp.gidx[i]];
+        int m1 = gvec.m[0], m2 = gvec.m[1], m3 = gvec.m[2];
+        if (std::abs(m1) <= d[0] / 2 && std::abs(m2) <= d[1] / 2 && std::abs(m3) <= d[2] / 2) {
+            // fft_grid_serial.hpp:301-306
+            int w1 = m1 >= 0 ? m1 : m1 + d[0];
+            int w2 = m2 >= 0 ? m2 : m2 + d[1];
+            int w3 = m3 >= 0 ? m3 : m3 + d[2];
+            kp.fftmap[i] = (w1 * d[1] + w2) * d[2] + w3;
+        } else {
+            kp.fftmap[i] = -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-radix (2,3,4,5) complex FFT, FFTW sign convention: sign=-1 forward
+// (e^{-i}), sign=+1 backward (e^{+i}), unnormalised.  Stands in for FFTW
+// (fft_grid_serial.hpp:209-223), which the reference links and this image lacks.
+// ---------------------------------------------------------------------------
+struct Fft1d {
+    int n = 0;
+    std::vector<int> radix;
+    std::vector<cplx> tw;  // tw[k] = exp(-2 pi i k / n)
+    void init(int len) {
+        n = len;
+        radix.clear();
+        int r = n;
+        while (r % 4 == 0) { radix.push_back(4); r /= 4; }
+        while (r % 2 == 0) { radix.push_back(2); r /= 2; }
+        while (r % 3 == 0) { radix.push_back(3); r /= 3; }
+        while (r % 5 == 0) { radix.push_back(5); r /= 5; }
+        if (r != 1) throw std::runtime_error("fft length not 2,3,5-smooth");
+        tw.resize(n);
+        for (int k = 0; k < n; k++) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * k / n;
+            tw[k] = cplx(static_cast<double>(std::cos(a)), static_cast<double>(std::sin(a)));
+        }
+    }
+    // Stockham autosort, out of place through scratch.
+    void run(cplx* data, cplx* scratch, int sign) const {
+        cplx* src = data;
+        cplx* dst = scratch;
+        int ns = 1;
+        for (int R : radix) {
+            int m = n / R;
+            for (int j = 0; j < m; j++) {
+                int jn = j % ns;
+                cplx v[5];
+                for (int r = 0; r < R; r++) {
+                    int e = (r * jn * (n / (ns * R))) % n;
+                    cplx w = tw[e];
+                    if (sign > 0) w = std::conj(w);
+                    v[r] = src[j + r * m] * w;
+                }
+                cplx o[5];
+                for (int q = 0; q < R; q++) {
+                    cplx s(0, 0);
+                    for (int r = 0; r < R; r++) {
+                        int e = ((q * r) % R) * (n / R);
+                        cplx w = tw[e];
+                        if (sign > 0) w = std::conj(w);
+                        s += v[r] * w;
+                    }
+                    o[q] = s;
+                }
+                int base = (j / ns) * ns * R + jn;
+                for (int q = 0; q < R; q++) dst[base + q * ns] = o[q];
+            }
+            std::swap(src, dst);
+            ns *= R;
+        }
+        if (src != data) std::memcpy(data, src, sizeof(cplx) * n);
+    }
+};
+
+struct Fft3d {
+    int d[3];
+    Fft1d f[3];
+    void init(const int* dims) {
+        for (int i = 0; i < 3; i++) {
+            d[i] = dims[i];
+            f[i].init(dims[i]);
+        }
+    }
+    // In place, C order (i1,i2,i3) with i3 fastest.
+    void run(cplx* a, int sign) const {
+        int n1 = d[0], n2 = d[1], n3 = d[2];
+        int mx = std::max(n1, std::max(n2, n3));
+        std::vector<cplx> line(mx), scr(mx);
+        for (int i = 0; i < n1 * n2; i++) f[2].run(a + static_cast<size_t>(i) * n3, scr.data(), sign);
+        for (int i1 = 0; i1 < n1; i1++)
+            for (int i3 = 0; i3 < n3; i3++) {
+                for (int i2 = 0; i2 < n2; i2++) line[i2] = a[(static_cast<size_t>(i1) * n2 + i2) * n3 + i3];
+                f[1].run(line.data(), scr.data(), sign);
+                for (int i2 = 0; i2 < n2; i2++) a[(static_cast<size_t>(i1) * n2 + i2) * n3 + i3] = line[i2];
+            }
+        for (int i2 = 0; i2 < n2; i2++)
+            for (int i3 = 0; i3 < n3; i3++) {
+                for (int i1 = 0; i1 < n1; i1++) line[i1] = a[(static_cast<size_t>(i1) * n2 + i2) * n3 + i3];
+                f[0].run(line.data(), scr.data(), sign);
+                for (int i1 = 0; i1 < n1; i1++) a[(static_cast<size_t>(i1) * n2 + i2) * n3 + i3] = line[i1];
+            }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Dense FP64 linear algebra (stands in for BLAS/LAPACK: zgemm, zpotrf, ztrsm,
+// zhegv), column-major like Fortran BLAS.
+// ---------------------------------------------------------------------------
+
+// C(m x n) = alpha * op(A) * B + beta * C ; op = 'N' or 'C'.
+void zgemm(char opa, int m, int n, int k, cplx alpha, const cplx* A, int lda, const cplx* B, int ldb,
+           cplx beta, cplx* C, int ldc) {
+#pragma omp parallel for schedule(static) if (static_cast<long long>(m) * n * k > 200000)
+    for (int j = 0; j < n; j++) {
+        std::vector<cplx> acc(m, cplx(0, 0));
+        if (opa == 'N') {
+            for (int p = 0; p < k; p++) {
+                cplx b = B[p + static_cast<size_t>(j) * ldb];
+                const cplx* a = A + static_cast<size_t>(p) * lda;
+                for (int i = 0; i < m; i++) acc[i] += a[i] * b;
+            }
+        } else {
+            for (int i = 0; i < m; i++) {
+                const cplx* a = A + static_cast<size_t>(i) * lda;
+                const cplx* bb = B + static_cast<size_t>(j) * ldb;
+                cplx s(0, 0);
+                for (int p = 0; p < k; p++) s += std::conj(a[p]) * bb[p];
+                acc[i] = s;
+            }
+        }
+        for (int i = 0; i < m; i++) {
+            cplx& c = C[i + static_cast<size_t>(j) * ldc];
+            c = (beta == cplx(0, 0) ? cplx(0, 0) : beta * c) + alpha * acc[i];
+        }
+    }
+}
+
+// zpotrf 'U': A = U^H U, U stored in the upper triangle. Returns 0 or j+1 on failure.
+int zpotrf_upper(int n, cplx* A, int lda) {
+    for (int j = 0; j < n; j++) {
+        double d = A[j + static_cast<size_t>(j) * lda].real();
+        for (int k = 0; k < j; k++) d -= std::norm(A[k + static_cast<size_t>(j) * lda]);
+        if (!(d > 0.0)) return j + 1;
+        d = std::sqrt(d);
+        A[j + static_cast<size_t>(j) * lda] = cplx(d, 0);
+        for (int i = j + 1; i < n; i++) {
+            cplx s = A[j + static_cast<size_t>(i) * lda];
+            for (int k = 0; k < j; k++)
+                s -= std::conj(A[k + static_cast<size_t>(j) * lda]) * A[k + static_cast<size_t>(i) * lda];
+            A[j + static_cast<size_t>(i) * lda] = s / d;
+        }
+    }
+    return 0;
+}
+
+// ztrsm 'R','U','N','N' with alpha=1: B(m x n) <- B * U^{-1}.
+void ztrsm_right_upper(int m, int n, const cplx* U, int ldu, cplx* B, int ldb) {
+#pragma omp parallel for schedule(static) if (static_cast<long long>(m) * n * n > 200000)
+    for (int i = 0; i < m; i++) {
+        for (int j = 0; j < n; j++) {
+            cplx s = B[i + static_cast<size_t>(j) * ldb];
+            for (int k = 0; k < j; k++)
+                s -= B[i + static_cast<size_t>(k) * ldb] * U[k + static_cast<size_t>(j) * ldu];
+            B[i + static_cast<size_t>(j) * ldb] = s / U[j + static_cast<size_t>(j) * ldu];
+        }
+    }
+}
+
+// Symmetric tridiagonal QL with implicit shifts (EISPACK tql2 lineage), eigenvalues
+// ascending, eigenvectors accumulated into Z (n x n, column-major).
+int tridiag_ql(int n, double* d, double* e, double* Z) {
+    for (int i = 1; i < n; i++) e[i - 1] = e[i];
+    if (n > 0) e[n - 1] = 0.0;
+    double f = 0.0, tst1 = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        tst1 = std::max(tst1, std::abs(d[l]) + std::abs(e[l]));
+        int m = l;
+        while (m < n) {
+            if (std::abs(e[m]) <= 2.220446049250313e-16 * tst1) break;
+            m++;
+        }
+        if (m == n) m = n - 1;
+        if (m > l) {
+            do {
+                if (++iter > 60) return l + 1;
+                double g = d[l];
+                double p = (d[l + 1] - g) / (2.0 * e[l]);
+                double r = std::hypot(p, 1.0);
+                if (p < 0) r = -r;
+                d[l] = e[l] / (p + r);
+                d[l + 1] = e[l] * (p + r);
+                double dl1 = d[l + 1];
+                double h = g - d[l];
+                for (int i = l + 2; i < n; i++) d[i] -= h;
+                f += h;
+                p = d[m];
+                double c = 1.0, c2 = 1.0, c3 = 1.0, el1 = e[l + 1], s = 0.0, s2 = 0.0;
+                for (int i = m - 1; i >= l; i--) {
+                    c3 = c2;
+                    c2 = c;
+                    s2 = s;
+                    g = c * e[i];
+                    h = c * p;
+                    r = std::hypot(p, e[i]);
+                    e[i + 1] = s * r;
+                    s = e[i] / r;
+                    c = p / r;
+                    p = c * d[i] - s * g;
+                    d[i + 1] = h + s * (c * g + s * d[i]);
+                    double* zi = Z + static_cast<size_t>(i) * n;
+                    double* zi1 = Z + static_cast<size_t>(i + 1) * n;
+                    for (int k = 0; k < n; k++) {
+                        h = zi1[k];
+                        zi1[k] = s * zi[k] + c * h;
+                        zi[k] = c * zi[k] - s * h;
+                    }
+                }
+                p = -s * s2 * c3 * el1 * e[l] / dl1;
+                e[l] = s * p;
+                d[l] = c * p;
+            } while (std::abs(e[l]) > 2.220446049250313e-16 * tst1);
+        }
+        d[l] = d[l] + f;
+        e[l] = 0.0;
+    }
+    // sort ascending
+    for (int i = 0; i < n - 1; i++) {
+        int k = i;
+        double p = d[i];
+        for (int j = i + 1; j < n; j++)
+            if (d[j] < p) {
+                k = j;
+                p = d[j];
+            }
+        if (k != i) {
+            d[k] = d[i];
+            d[i] = p;
+            for (int q = 0; q < n; q++) std::swap(Z[q + static_cast<size_t>(i) * n], Z[q + static_cast<size_t>(k) * n]);
+        }
+    }
+    return 0;
+}
+
+// Hermitian eigenproblem A x = w x (A full Hermitian, column-major, overwritten).
+// Householder reduction to real tridiagonal form, implicit QL, back transform.
+int zheev(int n, cplx* A, int lda, double* w, cplx* X, int ldx) {
+    if (n == 0) return 0;
+    std::vector<cplx> a(static_cast<size_t>(n) * n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) a[i + static_cast<size_t>(j) * n] = A[i + static_cast<size_t>(j) * lda];
+    // Householder vectors stored in columns below the sub-diagonal of a.
+    std::vector<cplx> tau(n, cplx(0, 0));
+    std::vector<cplx> p(n), v(n);
+    for (int k = 0; k < n - 2; k++) {
+        // x = a[k+1:n, k]
+        int len = n - k - 1;
+        cplx* x = &a[(k + 1) + static_cast<size_t>(k) * n];
+        double xn2 = 0;
+        for (int i = 0; i < len; i++) xn2 += std::norm(x[i]);
+        double xnorm = std::sqrt(xn2);
+        if (xnorm == 0.0) {
+            tau[k] = 0;
+            continue;
+        }
+        cplx alpha = x[0];
+        double aabs = std::abs(alpha);
+        cplx phase = aabs == 0.0 ? cplx(1, 0) : alpha / aabs;
+        cplx beta = -phase * xnorm;  // new sub-diagonal element
+        // v = x - beta e1, normalised so v0 = 1; H = I - tau v v^H
+        cplx v0 = alpha - beta;
+        v[0] = 1.0;
+        for (int i = 1; i < len; i++) v[i] = x[i] / v0;
+        double vn2 = 0;
+        for (int i = 0; i < len; i++) vn2 += std::norm(v[i]);
+        cplx t = 2.0 / vn2;
+        tau[k] = t;
+        // Apply H A H to trailing block B = a[k+1:, k+1:]
+        // p = tau * B v ; K = tau/2 * (v^H p) ; q = p - K v ; B -= v q^H + q v^H
+        cplx* B = &a[(k + 1) + static_cast<size_t>(k + 1) * n];
+        for (int i = 0; i < len; i++) p[i] = 0;
+        for (int j = 0; j < len; j++) {
+            cplx vj = v[j];
+            const cplx* col = B + static_cast<size_t>(j) * n;
+            for (int i = 0; i < len; i++) p[i] += col[i] * vj;
+        }
+        for (int i = 0; i < len; i++) p[i] *= t;
+        cplx vhp(0, 0);
+        for (int i = 0; i < len; i++) vhp += std::conj(v[i]) * p[i];
+        cplx K = 0.5 * t * vhp;  // t is real
+        for (int i = 0; i < len; i++) p[i] -= K * v[i];
+        for (int j = 0; j < len; j++) {
+            cplx* col = B + static_cast<size_t>(j) * n;
+            cplx cvj = std::conj(v[j]), cpj = std::conj(p[j]);
+            for (int i = 0; i < len; i++) col[i] -= v[i] * cpj + p[i] * cvj;
+        }
+        x[0] = beta;
+        for (int i = 1; i < len; i++) x[i] = v[i];  // store v tail
+        // upper part mirrors lower
+        a[k + static_cast<size_t>(k + 1) * n] = std::conj(beta);
+    }
+    // tridiagonal: diag real, off-diagonal complex e_k = a[k+1,k]
+    std::vector<double> d(n), e(n, 0.0);
+    std::vector<cplx> ph(n, cplx(1, 0));
+    for (int i = 0; i < n; i++) d[i] = a[i + static_cast<size_t>(i) * n].real();
+    for (int i = 1; i < n; i++) {
+        cplx off = a[i + static_cast<size_t>(i - 1) * n];
+        double r = std::abs(off);
+        e[i] = r;
+        // D^H T D real: ph[i] = ph[i-1] * off / |off|
+        ph[i] = r == 0.0 ? ph[i - 1] : ph[i - 1] * (off / r);
+    }
+    std::vector<double> Z(static_cast<size_t>(n) * n, 0.0);
+    for (int i = 0; i < n; i++) Z[i + static_cast<size_t>(i) * n] = 1.0;
+    int info = tridiag_ql(n, d.data(), e.data(), Z.data());
+    if (info) return info;
+    // X = Q * diag(ph) * Z, with Q = H_0 H_1 ... H_{n-3}
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) X[i + static_cast<size_t>(j) * ldx] = ph[i] * Z[i + static_cast<size_t>(j) * n];
+    for (int k = n - 3; k >= 0; k--) {
+        if (tau[k] == cplx(0, 0)) continue;
+        int len = n - k - 1;
+        v[0] = 1.0;
+        for (int i = 1; i < len; i++) v[i] = a[(k + 1 + i) + static_cast<size_t>(k) * n];
+        cplx t = tau[k];
+        for (int j = 0; j < n; j++) {
+            cplx* col = X + static_cast<size_t>(j) * ldx + (k + 1);
+            cplx s(0, 0);
+            for (int i = 0; i < len; i++) s += std::conj(v[i]) * col[i];
+            s *= t;
+            for (int i = 0; i < len; i++) col[i] -= v[i] * s;
+        }
+    }
+    for (int i = 0; i < n; i++) w[i] = d[i];
+    return 0;
+}
+
+// zhegv itype=1, jobz='V', uplo='U' (subspace/serial.hpp:26-136): H x = w S x with
+// X^H S X = I.  H and S are read from their upper triangles.  Returns LAPACK-style info
+// (n + j when S's leading minor j is not positive definite).
+int zhegv(int n, const cplx* H, int ldh, const cplx* S, int lds, double* w, cplx* X, int ldx) {
+    if (n == 0) return 0;
+    std::vector<cplx> U(static_cast<size_t>(n) * n), C(static_cast<size_t>(n) * n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i <= j; i++) {
+            U[i + static_cast<size_t>(j) * n] = S[i + static_cast<size_t>(j) * lds];
+            cplx h = H[i + static_cast<size_t>(j) * ldh];
+            C[i + static_cast<size_t>(j) * n] = h;
+            C[j + static_cast<size_t>(i) * n] = std::conj(h);
+        }
+    for (int j = 0; j < n; j++) C[j + static_cast<size_t>(j) * n] = cplx(C[j + static_cast<size_t>(j) * n].real(), 0);
+    int info = zpotrf_upper(n, U.data(), n);
+    if (info) return n + info;
+    for (int j = 0; j < n; j++)
+        for (int i = j + 1; i < n; i++) U[i + static_cast<size_t>(j) * n] = 0;
+    // C <- U^{-H} C U^{-1}: first C <- C U^{-1} (right), then C <- U^{-H} C via (C^H U^{-1})^H
+    ztrsm_right_upper(n, n, U.data(), n, C.data(), n);
+    std::vector<cplx> Ct(static_cast<size_t>(n) * n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) Ct[i + static_cast<size_t>(j) * n] = std::conj(C[j + static_cast<size_t>(i) * n]);
+    ztrsm_right_upper(n, n, U.data(), n, Ct.data(), n);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) C[i + static_cast<size_t>(j) * n] = std::conj(Ct[j + static_cast<size_t>(i) * n]);
+    // Hermitian symmetrise against rounding
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < j; i++) {
+            cplx avg = 0.5 * (C[i + static_cast<size_t>(j) * n] + std::conj(C[j + static_cast<size_t>(i) * n]));
+            C[i + static_cast<size_t>(j) * n] = avg;
+            C[j + static_cast<size_t>(i) * n] = std::conj(avg);
+        }
+    std::vector<cplx> Y(static_cast<size_t>(n) * n);
+    info = zheev(n, C.data(), n, w, Y.data(), n);
+    if (info) return info;
+    // X = U^{-1} Y : solve U X = Y (back substitution)
+    for (int j = 0; j < n; j++) {
+        for (int i = n - 1; i >= 0; i--) {
+            cplx s = Y[i + static_cast<size_t>(j) * n];
+            for (int k = i + 1; k < n; k++) s -= U[i + static_cast<size_t>(k) * n] * X[k + static_cast<size_t>(j) * ldx];
+            X[i + static_cast<size_t>(j) * ldx] = s / U[i + static_cast<size_t>(i) * n];
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Hamiltonian application (hamiltonian_applicator/serial.hpp:264-700, non-gamma).
+// ---------------------------------------------------------------------------
+struct Ham {
+    int dims[3];
+    int N;
+    int npw;
+    const int* fftmap;
+    const double* ekin;
+    std::vector<double> vscaled;  // V(r)/N  (serial.hpp:64-78)
+    int nproj;
+    const cplx* beta;             // npw x nproj, column-major
+    int natoms;
+    std::vector<int> bsize, bstart;
+    std::vector<size_t> boff;
+    std::vector<cplx> dblocks;    // per atom, column-major block (serial.hpp:603-628)
+    Fft3d fft;
+};
+
+void ham_apply(const Ham& h, int nvec, const cplx* psi, cplx* hpsi) {
+    int npw = h.npw;
+#pragma omp parallel
+    {
+        std::vector<cplx> box(h.N);
+#pragma omp for schedule(static)
+        for (int v = 0; v < nvec; v++) {
+            const cplx* p = psi + static_cast<size_t>(v) * npw;
+            cplx* hp = hpsi + static_cast<size_t>(v) * npw;
+            for (int g = 0; g < npw; g++) hp[g] = h.ekin[g] * p[g];             // serial.hpp:451-454
+            std::fill(box.begin(), box.end(), cplx(0, 0));                      // serial.hpp:457
+            for (int g = 0; g < npw; g++)
+                if (h.fftmap[g] >= 0) box[h.fftmap[g]] = p[g];                   // serial.hpp:458-463
+            h.fft.run(box.data(), +1);                                           // FFTW_BACKWARD, raw
+            for (int r = 0; r < h.N; r++) box[r] *= h.vscaled[r];                // serial.hpp:466-468
+            h.fft.run(box.data(), -1);                                           // FFTW_FORWARD
+            for (int g = 0; g < npw; g++)
+                if (h.fftmap[g] >= 0) hp[g] += box[h.fftmap[g]];                 // serial.hpp:472-477
+        }
+    }
+    if (h.nproj == 0) return;
+    // serial.hpp:572-584  P = beta^H psi
+    std::vector<cplx> P(static_cast<size_t>(h.nproj) * nvec), W(static_cast<size_t>(h.nproj) * nvec, cplx(0, 0));
+    zgemm('C', h.nproj, nvec, npw, 1.0, h.beta, npw, psi, npw, 0.0, P.data(), h.nproj);
+    // serial.hpp:657-679  W_a = D_a P_a
+    for (int a = 0; a < h.natoms; a++) {
+        int bs = h.bsize[a];
+        if (bs == 0) continue;
+        const cplx* D = h.dblocks.data() + h.boff[a];
+        int s0 = h.bstart[a];
+        for (int v = 0; v < nvec; v++)
+            for (int i = 0; i < bs; i++) {
+                cplx s(0, 0);
+                for (int j = 0; j < bs; j++) s += D[i + static_cast<size_t>(j) * bs] * P[s0 + j + static_cast<size_t>(v) * h.nproj];
+                W[s0 + i + static_cast<size_t>(v) * h.nproj] = s;
+            }
+    }
+    // serial.hpp:688-700  Hpsi += beta W
+    zgemm('N', npw, nvec, h.nproj, 1.0, h.beta, npw, W.data(), h.nproj, 1.0, hpsi, npw);
+}
+
+// ---------------------------------------------------------------------------
+// KB projectors (projector.hpp:42-199) with the beta-function table
+// (beta_function_table/init.hpp:42-126), q grid (uniform_q_grid.hpp:23-28),
+// Bessel transform (spherical_bessel.hpp:33-53), Simpson (simpson.hpp:22-48),
+// j_l (bessel.hpp:29-190), Lagrange (lagrange.hpp), Y_lm (harmonics.hpp:217-327).
+// ---------------------------------------------------------------------------
+
+// bessel.hpp:29-84
+double sph_bessel(int l, double x) {
+    if (std::abs(x) < kBesselZero) return l == 0 ? 1.0 : 0.0;
+    if (std::abs(x) < static_cast<double>(l) + 1.0) {
+        double xl = (l == 0) ? 1.0 : std::pow(x, l);
+        double df = 1.0;
+        for (int i = 2 * l + 1; i >= 1; i -= 2) df *= i;
+        double h = 0.5 * x * x;
+        double term = 1.0, sum = 1.0;
+        for (int i = 1; i <= 60; i++) {
+            term *= -h / (i * (2.0 * l + 2.0 * i + 1.0));
+            sum += term;
+            if (std::abs(term) < 1e-18 * std::abs(sum)) break;
+        }
+        return (xl / df) * sum;
+    }
+    switch (l) {
+        case 0: return std::sin(x) / x;
+        case 1: return (std::sin(x) / x - std::cos(x)) / x;
+        case 2: return ((3.0 / (x * x) - 1.0) * std::sin(x) - 3.0 * std::cos(x) / x) / x;
+        default: {
+            double jm = std::sin(x) / x, jc = (std::sin(x) / x - std::cos(x)) / x;
+            for (int i = 2; i <= l; i++) {
+                double jp = ((2 * i - 1) * jc / x) - jm;
+                jm = jc;
+                jc = jp;
+            }
+            return jc;
+        }
+    }
+}
+
+// bessel.hpp:100-190
+void sph_bessel_batch(int l, double q, const double* r, int n, double* out) {
+    if (std::abs(q) < kBesselZero) {
+        for (int i = 0; i < n; i++) out[i] = l == 0 ? 1.0 : 0.0;
+        return;
+    }
+    for (int i = 0; i < n; i++) {
+        double x = q * r[i];
+        if (l <= 2) {
+            if (std::abs(x) < 1.0) out[i] = sph_bessel(l, x);
+            else if (l == 0) out[i] = std::sin(x) / x;
+            else if (l == 1) out[i] = (std::sin(x) / x - std::cos(x)) / x;
+            else out[i] = ((3.0 / (x * x) - 1.0) * std::sin(x) - 3.0 * std::cos(x) / x) / x;
+        } else {
+            if (std::abs(x) < static_cast<double>(l) + 1.0) out[i] = sph_bessel(l, x);
+            else {
+                double jm = std::sin(x) / x, jc = (std::sin(x) / x - std::cos(x)) / x;
+                for (int k = 2; k <= l; k++) {
+                    double jp = ((2 * k - 1) * jc / x) - jm;
+                    jm = jc;
+                    jc = jp;
+                }
+                out[i] = jc;
+            }
+        }
+    }
+}
+
+// simpson.hpp:22-48
+double simpson(const std::vector<double>& f, const double* rab, int n) {
+    double s = 0.0;
+    int mesh = std::min(n, static_cast<int>(f.size()));
+    if (mesh < 3) return 0.0;
+    for (int i = 1; i < mesh - 1; i++) s += ((i % 2 == 1) ? 4.0 : 2.0) * f[i] * rab[i];
+    if (mesh % 2 == 1) {
+        s += f[0] * rab[0] + f[mesh - 1] * rab[mesh - 1];
+    } else {
+        s += f[0] * rab[0] - f[mesh - 3] * rab[mesh - 3] * 0.25 + f[mesh - 2] * rab[mesh - 2] +
+             f[mesh - 1] * rab[mesh - 1] * 1.25;
+    }
+    return s / 3.0;
+}
+
+// lagrange.hpp:16-30 (with uniform_q_grid_table.hpp:54-63 for q ~ 0)
+double lagrange4(const std::vector<double>& t, double q, double dq, int nq) {
+    if (q < kZeroVec) return t[0];
+    double fi = q / dq;
+    int b = static_cast<int>(fi);
+    double fp = fi - b;
+    if (b + 3 >= nq) return 0.0;
+    double w1 = 1.0 - fp, w2 = 2.0 - fp, w3 = 3.0 - fp;
+    return t[b] * w1 * w2 * w3 / 6.0 + t[b + 1] * fp * w2 * w3 / 2.0 - t[b + 2] * fp * w1 * w3 / 2.0 +
+           t[b + 3] * fp * w1 * w2 / 6.0;
+}
+
+// harmonics.hpp:50-58
+int lm_index(int l, int m) {
+    if (m == 0) return l * l;
+    if (m > 0) return l * l + 2 * m - 1;
+    return l * l - 2 * m;
+}
+
+// harmonics.hpp:217-327, one vector at a time
+void ylm_batch_one(int lmax, const double* g, double g2, double* out) {
+    const double gm = std::sqrt(g2);
+    const bool degen = gm < 1e-9;
+    const double ct = degen ? 0.0 : g[2] / gm;
+    const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+    const double phi = degen ? std::copysign(kPi / 2.0, g[1]) : std::atan2(g[1], g[0]);
+    int lsz = (lmax + 1) * (lmax + 2) / 2;
+    std::vector<double> L(lsz, 0.0);
+    auto li = [](int l, int m) { return l * (l + 1) / 2 + m; };
+    L[li(0, 0)] = 1.0;
+    if (lmax >= 1) {
+        L[li(1, 0)] = ct;
+        L[li(1, 1)] = -st / std::sqrt(2.0);
+    }
+    for (int l = 2; l <= lmax; l++) {
+        for (int m = 0; m <= l - 2; m++) {
+            const double den = std::sqrt(static_cast<double>(l * l - m * m));
+            L[li(l, m)] = ct * (2 * l - 1) / den * L[li(l - 1, m)] -
+                          std::sqrt(static_cast<double>((l - 1) * (l - 1) - m * m)) / den * L[li(l - 2, m)];
+        }
+        const double df = std::sqrt(static_cast<double>(2 * l - 1));
+        const double pd = L[li(l - 1, l - 1)];
+        L[li(l, l - 1)] = ct * df * pd;
+        L[li(l, l)] = -df / std::sqrt(static_cast<double>(2 * l)) * st * pd;
+    }
+    out[lm_index(0, 0)] = L[li(0, 0)] / std::sqrt(kFourPi);
+    for (int l = 1; l <= lmax; l++) {
+        const double nrm = std::sqrt(static_cast<double>(2 * l + 1) / kFourPi);
+        out[lm_index(l, 0)] = nrm * L[li(l, 0)];
+        const double off = nrm * std::sqrt(2.0);
+        for (int m = 1; m <= l; m++) {
+            const double lv = L[li(l, m)];
+            out[lm_index(l, m)] = off * lv * std::cos(m * phi);
+            out[lm_index(l, -m)] = off * lv * std::sin(m * phi);
+        }
+    }
+}
+
+struct Species {
+    int nmesh;
+    const double* r;
+    const double* rab;
+    int nbeta;
+    const int* lbeta;
+    const int* cutoff;       // number_of_radial_cutoff_points per beta
+    const double* beta_r;    // nbeta x nmesh, r*beta(r) (UPF convention)
+};
+
+}  // namespace
+
+// ===========================================================================
+// extern "C" API (ctypes, tests only)
+// ===========================================================================
+extern "C" {
+
+int orc_good_fft_order(int n, int factor) { return good_fft_order(n, factor); }
+
+void orc_crystal(const double* lat9, double alat, double* recip9, double* volume, double* tpa2) {
+    Crystal c;
+    c.init(lat9, alat);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) recip9[3 * i + j] = c.recip.r[i][j];
+    *volume = c.volume;
+    *tpa2 = c.tpa2;
+}
+
+void orc_grid_dims(const double* lat9, double alat, double ecut, const int* factors, int* dims) {
+    Crystal c;
+    c.init(lat9, alat);
+    grid_dimensions(ecut, c.tpa2, c.lat, c.recip, factors, dims);
+}
+
+// pair.hpp:24-100 + init.hpp:60-96 + g_vectors_grid.hpp:23-104
+void* orc_basis_create(const double* lat9, double alat, double ecutwfc, double ecutrho, int use_exact,
+                       const int* factors) {
+    Basis* b = new Basis;
+    b->c.init(lat9, alat);
+    b->ecutwfc = ecutwfc;
+    b->ecutrho = ecutrho <= 0.0 ? 4.0 * ecutwfc : ecutrho;
+    grid_dimensions(b->ecutrho, b->c.tpa2, b->c.lat, b->c.recip, factors, b->rho);
+    double smooth = std::min((use_exact ? 4.0 : 2.0) * b->ecutwfc, b->ecutrho);
+    grid_dimensions(smooth, b->c.tpa2, b->c.lat, b->c.recip, factors, b->wave);
+    generate_gvectors(*b);
+    return b;
+}
+
+void orc_basis_destroy(void* h) { delete static_cast<Basis*>(h); }
+
+void orc_basis_info(void* h, int* rho, int* wave, int* ng) {
+    Basis* b = static_cast<Basis*>(h);
+    for (int i = 0; i < 3; i++) {
+        rho[i] = b->rho[i];
+        wave[i] = b->wave[i];
+    }
+    *ng = static_cast<int>(b->gv.size());
+}
+
+void orc_basis_gvectors(void* h, int* miller, double* gcart, double* g2) {
+    Basis* b = static_cast<Basis*>(h);
+    for (size_t i = 0; i < b->gv.size(); i++) {
+        for (int d = 0; d < 3; d++) {
+            miller[3 * i + d] = b->gv[i].m[d];
+            gcart[3 * i + d] = b->gv[i].g[d];
+        }
+        g2[i] = b->gv[i].g2;
+    }
+}
+
+// Returns npw; fills up to cap entries.
+int orc_kpoint(void* h, const double* k, int* gidx, double* ekin, int* fftmap, int cap) {
+    Basis* b = static_cast<Basis*>(h);
+    KPoint kp;
+    select_kpoint(*b, k, kp);
+    int n = static_cast<int>(kp.gidx.size());
+    for (int i = 0; i < n && i < cap; i++) {
+        gidx[i] = kp.gidx[i];
+        ekin[i] = kp.ekin[i];
+        fftmap[i] = kp.fftmap[i];
+    }
+    return n;
+}
+
+// 3D FFT in place (interleaved complex), sign -1 forward / +1 backward, unnormalised.
+void orc_fft3d(const int* dims, double* data, int sign) {
+    Fft3d f;
+    f.init(dims);
+    f.run(reinterpret_cast<cplx*>(data), sign);
+}
+
+// H|psi> for nvec vectors. V is the unscaled smooth potential V(r) (Ry).
+// dblocks: per-atom square blocks, column-major, packed in atom order (real values).
+void orc_hamiltonian_apply(const int* dims, int npw, const int* fftmap, const double* ekin,
+                           const double* V, int nproj, const double* beta, int natoms,
+                           const int* block_sizes, const double* dblocks, int nvec,
+                           const double* psi, double* hpsi) {
+    Ham h;
+    for (int i = 0; i < 3; i++) h.dims[i] = dims[i];
+    h.N = dims[0] * dims[1] * dims[2];
+    h.npw = npw;
+    h.fftmap = fftmap;
+    h.ekin = ekin;
+    h.vscaled.resize(h.N);
+    double inv = 1.0 / h.N;
+    for (int r = 0; r < h.N; r++) h.vscaled[r] = V[r] * inv;
+    h.nproj = nproj;
+    h.beta = reinterpret_cast<const cplx*>(beta);
+    h.natoms = natoms;
+    size_t off = 0;
+    int start = 0;
+    for (int a = 0; a < natoms; a++) {
+        int bs = block_sizes[a];
+        h.bsize.push_back(bs);
+        h.bstart.push_back(start);
+        h.boff.push_back(off);
+        for (int q = 0; q < bs * bs; q++) {
+            double dv = dblocks[off + q];
+            h.dblocks.push_back(std::abs(dv) > kDThreshold ? cplx(dv, 0) : cplx(0, 0));
+        }
+        off += static_cast<size_t>(bs) * bs;
+        start += bs;
+    }
+    h.fft.init(dims);
+    ham_apply(h, nvec, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(hpsi));
+}
+
+// Beta table for one species: nq points on dq=0.01 up to 1.1*sqrt(ecutrho)
+// (beta_function_table/init.hpp:45-52).  Exposed for tests.
+int orc_beta_table_size(double ecutrho) {
+    double qmax = std::sqrt(ecutrho) * 1.1;
+    return static_cast<int>(qmax / kQSpacing) + 4;
+}
+
+void orc_beta_table(int nmesh, const double* r, const double* rab, int l, int cutoff,
+                    const double* beta_r, int nq, double* table) {
+    std::vector<double> integrand(cutoff), jl(cutoff);
+    for (int iq = 0; iq < nq; iq++) {
+        double q = iq * kQSpacing;
+        sph_bessel_batch(l, q, r, cutoff, jl.data());
+        for (int i = 0; i < cutoff; i++) integrand[i] = beta_r[i] * r[i] * jl[i];
+        table[iq] = kFourPi * simpson(integrand, rab, cutoff);
+    }
+    (void)nmesh;
+}
+
+double orc_sph_bessel(int l, double x) { return sph_bessel(l, x); }
+
+void orc_ylm(int lmax, int n, const double* g, double* out) {
+    int nlm = (lmax + 1) * (lmax + 1);
+    for (int i = 0; i < n; i++) {
+        double g2 = g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2];
+        ylm_batch_one(lmax, g + 3 * i, g2, out + static_cast<size_t>(i) * nlm);
+    }
+}
+
+// KB projector set at one k-point (projector.hpp:42-199).
+// atom_pos: crystal coordinates (natoms x 3); atom_species: species index per atom.
+// Species arrays are concatenated: nmesh[s], r/rab offsets by cumulative nmesh;
+// nbeta[s], lbeta/cutoff concatenated; beta_r: per species nbeta[s] x nmesh[s].
+// Output: beta_out, npw x nproj complex, projector after projector.
+int orc_kb_projectors(void* hb, const double* k, int npw, const int* gidx, int natoms,
+                      const double* atom_pos, const int* atom_species, int nspecies,
+                      const int* nmesh, const double* r_all, const double* rab_all,
+                      const int* nbeta, const int* lbeta_all, const int* cutoff_all,
+                      const double* beta_r_all, double* beta_out) {
+    Basis* b = static_cast<Basis*>(hb);
+    const Crystal& c = b->c;
+    std::vector<Species> sp(nspecies);
+    {
+        size_t ro = 0, bo = 0, bro = 0;
+        for (int s = 0; s < nspecies; s++) {
+            sp[s].nmesh = nmesh[s];
+            sp[s].r = r_all + ro;
+            sp[s].rab = rab_all + ro;
+            sp[s].nbeta = nbeta[s];
+            sp[s].lbeta = lbeta_all + bo;
+            sp[s].cutoff = cutoff_all + bo;
+            sp[s].beta_r = beta_r_all + bro;
+            ro += nmesh[s];
+            bo += nbeta[s];
+            bro += static_cast<size_t>(nbeta[s]) * nmesh[s];
+        }
+    }
+    int nq = orc_beta_table_size(b->ecutrho);
+    std::vector<std::vector<std::vector<double>>> table(nspecies);
+    for (int s = 0; s < nspecies; s++) {
+        table[s].resize(sp[s].nbeta, std::vector<double>(nq));
+        for (int ib = 0; ib < sp[s].nbeta; ib++)
+            orc_beta_table(sp[s].nmesh, sp[s].r, sp[s].rab, sp[s].lbeta[ib], sp[s].cutoff[ib],
+                           sp[s].beta_r + static_cast<size_t>(ib) * sp[s].nmesh, nq, table[s][ib].data());
+    }
+    const M3& R = c.recip;
+    V3 kc;
+    for (int d = 0; d < 3; d++) kc[d] = k[0] * R.r[0][d] + k[1] * R.r[1][d] + k[2] * R.r[2][d];
+    double isv = 1.0 / std::sqrt(c.volume);
+    std::vector<std::array<double, 3>> kg(npw);
+    std::vector<double> kg2(npw), kgm(npw);
+    for (int g = 0; g < npw; g++) {
+        const V3& gc = b->gv[gidx[g]].g;
+        V3 v{kc.x + gc.x, kc.y + gc.y, kc.z + gc.z};
+        kg[g] = {v.x, v.y, v.z};
+        double m2 = v.x * v.x + v.y * v.y + v.z * v.z;
+        kg2[g] = m2;
+        kgm[g] = std::sqrt(m2) * c.tpa;
+    }
+    int lmax = 0;
+    for (int s = 0; s < nspecies; s++)
+        for (int ib = 0; ib < sp[s].nbeta; ib++) lmax = std::max(lmax, sp[s].lbeta[ib]);
+    int nlm = (lmax + 1) * (lmax + 1);
+    std::vector<double> ylm(static_cast<size_t>(npw) * nlm);
+    for (int g = 0; g < npw; g++) ylm_batch_one(lmax, kg[g].data(), kg2[g], ylm.data() + static_cast<size_t>(g) * nlm);
+    std::vector<std::vector<std::vector<double>>> rad(nspecies);
+    for (int s = 0; s < nspecies; s++) {
+        rad[s].resize(sp[s].nbeta, std::vector<double>(npw));
+        for (int ib = 0; ib < sp[s].nbeta; ib++)
+            for (int g = 0; g < npw; g++) rad[s][ib][g] = lagrange4(table[s][ib], kgm[g], kQSpacing, nq);
+    }
+    cplx* out = reinterpret_cast<cplx*>(beta_out);
+    int col = 0;
+    for (int a = 0; a < natoms; a++) {
+        // crystal.hpp convert_from_crystal_to_cartesian: tau = sum_i x_i a_i (alat units)
+        V3 tau{0, 0, 0};
+        for (int d = 0; d < 3; d++)
+            tau[d] = atom_pos[3 * a] * c.lat.r[0][d] + atom_pos[3 * a + 1] * c.lat.r[1][d] +
+                     atom_pos[3 * a + 2] * c.lat.r[2][d];
+        double kr = kc.dot(tau);
+        std::vector<cplx> sf(npw);
+        for (int g = 0; g < npw; g++) {
+            double ph = kTwoPi * (b->gv[gidx[g]].g.dot(tau) + kr);
+            sf[g] = cplx(std::cos(ph), -std::sin(ph));  // phase_factor.hpp:17-19
+        }
+        int s = atom_species[a];
+        for (int ib = 0; ib < sp[s].nbeta; ib++) {
+            int l = sp[s].lbeta[ib];
+            cplx ang;  // phase_factor.hpp:33-44
+            switch (l % 4) {
+                case 1: ang = cplx(0, -1); break;
+                case 2: ang = cplx(-1, 0); break;
+                case 3: ang = cplx(0, 1); break;
+                default: ang = cplx(1, 0);
+            }
+            cplx pref = isv * ang;
+            std::vector<int> ms{0};
+            for (int m = 1; m <= l; m++) {
+                ms.push_back(m);
+                ms.push_back(-m);
+            }
+            for (int m : ms) {
+                int lm = lm_index(l, m);
+                cplx* pcol = out + static_cast<size_t>(col) * npw;
+                for (int g = 0; g < npw; g++)
+                    pcol[g] = rad[s][ib][g] * ylm[static_cast<size_t>(g) * nlm + lm] * sf[g] * pref;
+                col++;
+            }
+        }
+    }
+    return col;
+}
+
+// zhegv (see above).  H, S: n x n column-major interleaved complex.
+int orc_zhegv(int n, const double* H, const double* S, double* w, double* X) {
+    return zhegv(n, reinterpret_cast<const cplx*>(H), n, reinterpret_cast<const cplx*>(S), n, w,
+                 reinterpret_cast<cplx*>(X), n);
+}
+
+// ---------------------------------------------------------------------------
+// Block Davidson (davidson.hpp:324-841) with the serial subspace manager
+// (subspace_manager/serial.hpp), norm-conserving (S = identity, SV aliases V),
+// full-sphere (not gamma-only).
+//
+// psi_inout: nb x npw complex; if seed_from_psi != 0 the bands seed the basis
+// (davidson.hpp:1000-1045), else kinetic-sorted unit vectors (davidson.hpp:1047-1101).
+// report[0]=iterations run, report[1]=unconverged bands, report[2]=H applications.
+// ---------------------------------------------------------------------------
+namespace {
+struct Dav {
+    const Ham* h;
+    int npw, nb, nocc, mmax;
+    std::vector<cplx> V, HV, Hs, Ss;
+    int happly = 0;
+    void apply(int first, int n) {
+        ham_apply(*h, n, V.data() + static_cast<size_t>(first) * npw, HV.data() + static_cast<size_t>(first) * npw);
+        happly += n;
+    }
+    // serial.hpp:41-553 (S = I)
+    int orthogonalize(int start, int ncand) {
+        if (ncand == 0) return 0;
+        cplx* cand = V.data() + static_cast<size_t>(start) * npw;
+        if (start > 0) {
+            std::vector<cplx> C(static_cast<size_t>(start) * ncand);
+            zgemm('C', start, ncand, npw, 1.0, V.data(), npw, cand, npw, 0.0, C.data(), start);
+            zgemm('N', npw, ncand, start, -1.0, V.data(), npw, C.data(), start, 1.0, cand, npw);
+        }
+        std::vector<cplx> G(static_cast<size_t>(ncand) * ncand);
+        zgemm('C', ncand, ncand, npw, 1.0, cand, npw, cand, npw, 0.0, G.data(), ncand);
+        if (zpotrf_upper(ncand, G.data(), ncand) == 0) {
+            ztrsm_right_upper(npw, ncand, G.data(), ncand, cand, npw);
+            return ncand;
+        }
+        // MGS fallback (serial.hpp:370-552)
+        int acc = 0;
+        for (int c = 0; c < ncand; c++) {
+            int src = start + c, dst = start + acc;
+            if (dst != src)
+                std::memcpy(V.data() + static_cast<size_t>(dst) * npw, V.data() + static_cast<size_t>(src) * npw, sizeof(cplx) * npw);
+            cplx* cur = V.data() + static_cast<size_t>(dst) * npw;
+            int nex = dst - start;
+            if (nex > 0) {
+                std::vector<cplx> co(nex);
+                zgemm('C', nex, 1, npw, 1.0, cand, npw, cur, npw, 0.0, co.data(), nex);
+                zgemm('N', npw, 1, nex, -1.0, cand, npw, co.data(), nex, 1.0, cur, npw);
+            }
+            double n2 = 0;
+            for (int g = 0; g < npw; g++) n2 += std::norm(cur[g]);
+            if (n2 > kNormThreshold) {
+                double inv = 1.0 / std::sqrt(n2);
+                for (int g = 0; g < npw; g++) cur[g] *= inv;
+                acc++;
+            }
+        }
+        return acc;
+    }
+    // serial.hpp:555-678
+    void update(int start, int nnew, int total) {
+        zgemm('C', total, nnew, npw, 1.0, V.data(), npw, HV.data() + static_cast<size_t>(start) * npw, npw, 0.0,
+              Hs.data() + static_cast<size_t>(start) * mmax, mmax);
+        zgemm('C', total, nnew, npw, 1.0, V.data(), npw, V.data() + static_cast<size_t>(start) * npw, npw, 0.0,
+              Ss.data() + static_cast<size_t>(start) * mmax, mmax);
+    }
+    // serial.hpp:680-756 -> subspace/serial.hpp
+    int diag(int n, double* w, cplx* X) {
+        return zhegv(n, Hs.data(), mmax, Ss.data(), mmax, w, X, n);
+    }
+    // serial.hpp:825-945
+    int collapse(int n, bool have_matrices) {
+        std::vector<double> w(mmax);
+        std::vector<cplx> X(static_cast<size_t>(n) * n);
+        int info = -1;
+        if (have_matrices) info = diag(n, w.data(), X.data());
+        if (info != 0) {
+            // build_subspace_matrices_and_diagonalize: rebuild from the vectors
+            update(0, n, n);
+            info = diag(n, w.data(), X.data());
+        }
+        std::vector<cplx> scr(static_cast<size_t>(nb) * npw);
+        zgemm('N', npw, nb, n, 1.0, V.data(), npw, X.data(), n, 0.0, scr.data(), npw);
+        std::memcpy(V.data(), scr.data(), sizeof(cplx) * scr.size());
+        zgemm('N', npw, nb, n, 1.0, HV.data(), npw, X.data(), n, 0.0, scr.data(), npw);
+        std::memcpy(HV.data(), scr.data(), sizeof(cplx) * scr.size());
+        return nb;
+    }
+};
+}  // namespace
+
+int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin, const double* Vr,
+                 int nproj, const double* beta, int natoms, const int* block_sizes,
+                 const double* dblocks, int nb, int nocc, double* psi_inout, int seed_from_psi,
+                 double* eig_out, double tol, int max_iter, int factor, int* report) {
+    // Build the Hamiltonian once (shared with orc_hamiltonian_apply's setup).
+    Ham h;
+    for (int i = 0; i < 3; i++) h.dims[i] = dims[i];
+    h.N = dims[0] * dims[1] * dims[2];
+    h.npw = npw;
+    h.fftmap = fftmap;
+    h.ekin = ekin;
+    h.vscaled.resize(h.N);
+    double inv = 1.0 / h.N;
+    for (int r = 0; r < h.N; r++) h.vscaled[r] = Vr[r] * inv;
+    h.nproj = nproj;
+    h.beta = reinterpret_cast<const cplx*>(beta);
+    h.natoms = natoms;
+    {
+        size_t off = 0;
+        int start = 0;
+        for (int a = 0; a < natoms; a++) {
+            int bs = block_sizes[a];
+            h.bsize.push_back(bs);
+            h.bstart.push_back(start);
+            h.boff.push_back(off);
+            for (int q = 0; q < bs * bs; q++) {
+                double dv = dblocks[off + q];
+                h.dblocks.push_back(std::abs(dv) > kDThreshold ? cplx(dv, 0) : cplx(0, 0));
+            }
+            off += static_cast<size_t>(bs) * bs;
+            start += bs;
+        }
+    }
+    h.fft.init(dims);
+
+    Dav d;
+    d.h = &h;
+    d.npw = npw;
+    d.nb = nb;
+    d.nocc = nocc;
+    int mmax = std::max(std::min(factor * nb, npw), nb);  // davidson.hpp:399-404
+    d.mmax = mmax;
+    d.V.assign(static_cast<size_t>(mmax) * npw, cplx(0, 0));
+    d.HV.assign(static_cast<size_t>(mmax) * npw, cplx(0, 0));
+    d.Hs.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
+    d.Ss.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
+    cplx* psi = reinterpret_cast<cplx*>(psi_inout);
+
+    int basis = 0;
+    if (seed_from_psi) {  // davidson.hpp:1000-1045
+        std::memcpy(d.V.data(), psi, sizeof(cplx) * static_cast<size_t>(nb) * npw);
+        int acc = d.orthogonalize(0, nb);
+        basis = acc;
+        if (basis > 0) d.apply(0, basis);
+    }
+    if (basis < nb) {  // davidson.hpp:1047-1101
+        std::vector<std::pair<double, int>> ke;
+        for (int g = 0; g < npw; g++) ke.push_back({ekin[g], g});
+        std::sort(ke.begin(), ke.end());
+        int start = basis, nc = 0;
+        for (int bi = start; bi < nb; bi++) {
+            cplx* col = d.V.data() + static_cast<size_t>(basis + nc) * npw;
+            std::fill(col, col + npw, cplx(0, 0));
+            col[ke[bi].second] = 1.0;
+            nc++;
+        }
+        int acc = d.orthogonalize(basis, nc);
+        basis += acc;
+        int nnew = basis - start;
+        if (nnew > 0) d.apply(start, nnew);
+    }
+    d.update(0, basis, basis);
+    std::vector<double> prev(nb, 0.0);
+    for (int b = 0; b < nb; b++) prev[b] = d.Hs[static_cast<size_t>(b) * mmax + b].real();  // davidson.hpp:488-493
+    std::vector<double> w(mmax, 0.0);
+    std::vector<cplx> X;
+    int unconv = nb, it = 0;
+    double empty_tol = std::max(kEmptyFactor * tol, kEmptyFloor);
+    bool done = false;
+    for (it = 0; it < max_iter; it++) {
+        X.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
+        int info = d.diag(basis, w.data(), X.data());
+        if (info != 0 && basis > nb) {  // davidson.hpp:520-590
+            basis = d.collapse(basis, false);
+            std::fill(d.Hs.begin(), d.Hs.end(), cplx(0, 0));
+            std::fill(d.Ss.begin(), d.Ss.end(), cplx(0, 0));
+            d.update(0, basis, basis);
+            info = d.diag(basis, w.data(), X.data());
+            if (info != 0)
+                for (int b = 0; b < nb; b++) w[b] = prev[b];
+        }
+        std::vector<bool> conv(nb, true);
+        bool all = true;
+        for (int b = 0; b < nb; b++) {  // davidson.hpp:607-629
+            double ch = std::abs(w[b] - prev[b]);
+            double bt = b < nocc ? tol : empty_tol;
+            bool c = (it != 0 && ch < bt);
+            if (!c) {
+                all = false;
+                conv[b] = false;
+            }
+        }
+        unconv = 0;
+        for (int b = 0; b < nb; b++) unconv += conv[b] ? 0 : 1;
+        if (all) {
+            done = true;
+            break;
+        }
+        // build_correction_vectors (serial.hpp:995-1132)
+        int maxc = mmax - basis;
+        std::vector<int> src;
+        for (int b = 0; b < nb; b++)
+            if (!conv[b] && static_cast<int>(src.size()) < maxc) src.push_back(b);
+        int nc = static_cast<int>(src.size());
+        cplx* corr = d.V.data() + static_cast<size_t>(basis) * npw;
+        if (nc > 0) {
+            std::vector<cplx> pc(static_cast<size_t>(basis) * nc);
+            std::vector<double> ce(nc);
+            for (int c = 0; c < nc; c++) {
+                ce[c] = w[src[c]];
+                std::memcpy(&pc[static_cast<size_t>(c) * basis], X.data() + static_cast<size_t>(src[c]) * basis,
+                            sizeof(cplx) * basis);
+            }
+            zgemm('N', npw, nc, basis, 1.0, d.V.data(), npw, pc.data(), basis, 0.0, corr, npw);
+            for (int c = 0; c < nc; c++) {
+                double ne = -ce[c];
+                cplx* col = corr + static_cast<size_t>(c) * npw;
+                for (int g = 0; g < npw; g++) col[g] *= ne;
+            }
+            zgemm('N', npw, nc, basis, 1.0, d.HV.data(), npw, pc.data(), basis, 1.0, corr, npw);
+            for (int c = 0; c < nc; c++) {  // serial.hpp:1138-1156
+                cplx* col = corr + static_cast<size_t>(c) * npw;
+                for (int g = 0; g < npw; g++) {
+                    double x = ekin[g] - ce[c];
+                    double den = 0.5 * (1.0 + x + std::sqrt(1.0 + (x - 1.0) * (x - 1.0)));
+                    if (den < kPrecondFloor) den = kPrecondFloor;
+                    col[g] /= den;
+                }
+            }
+        }
+        int accepted = d.orthogonalize(basis, nc);
+        if (accepted > 0) d.apply(basis, accepted);
+        int old = basis;
+        basis += accepted;
+        if (accepted > 0) d.update(old, accepted, basis);
+        if (basis >= mmax) {
+            basis = d.collapse(basis, true);
+            std::fill(d.Hs.begin(), d.Hs.end(), cplx(0, 0));
+            std::fill(d.Ss.begin(), d.Ss.end(), cplx(0, 0));
+            d.update(0, basis, basis);
+        }
+        for (int b = 0; b < nb; b++) prev[b] = w[b];
+    }
+    if (!done) {  // davidson.hpp:812-841
+        X.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
+        d.diag(basis, w.data(), X.data());
+    }
+    // extract_eigenvectors (serial.hpp:947-993)
+    zgemm('N', npw, nb, basis, 1.0, d.V.data(), npw, X.data(), basis, 0.0, psi, npw);
+    for (int b = 0; b < nb; b++) eig_out[b] = w[b];
+    report[0] = done ? it + 1 : it;
+    report[1] = done ? 0 : unconv;
+    report[2] = d.happly;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+"""ctypes binding of libq3x.so (the C ABI declared in include/q3x.h).
+
+The library is loaded from the package directory only; if it is missing this raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libq3x.so")
+
+_i = ctypes.c_int
+_l = ctypes.c_long
+_ul = ctypes.c_ulong
+_d = ctypes.c_double
+_p = ctypes.c_void_p
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+
+class DavidsonOptions(ctypes.Structure):
+    _fields_ = [("tolerance_ry", _d), ("max_iterations", _i), ("basis_factor", _i), ("n_occupied", _i),
+                ("precision", _i), ("seed_from_psi", _i), ("final_fp64_sweep", _i)]
+
+
+class DavidsonReport(ctypes.Structure):
+    _fields_ = [("iterations", _i), ("unconverged", _i), ("h_applications", _i), ("fp64_refinements", _i),
+                ("max_residual_norm", _d), ("stage_ms", _d * 8)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/q3x.h
+SIGNATURES = {
+    "q3x_last_error": (ctypes.c_char_p, []),
+    "q3x_version": (ctypes.c_char_p, []),
+    "q3x_context_create": (_i, [_i, _pp]),
+    "q3x_context_destroy": (_i, [_p]),
+    "q3x_context_synchronize": (_i, [_p]),
+    "q3x_basis_create": (_i, [_p, _d, _d, _d, _i, _p, _pp]),
+    "q3x_basis_info": (_i, [_p, _p, _p, _p, _p, _p]),
+    "q3x_basis_gvectors": (_i, [_p, _p, _p, _p]),
+    "q3x_basis_destroy": (_i, [_p]),
+    "q3x_kpoint_create": (_i, [_p, _p, _pp, _p]),
+    "q3x_kpoint_maps": (_i, [_p, _p, _p, _p]),
+    "q3x_kpoint_destroy": (_i, [_p]),
+    "q3x_kb_projectors": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "q3x_hamiltonian_create": (_i, [_p, _p, _pp]),
+    "q3x_hamiltonian_destroy": (_i, [_p]),
+    "q3x_hamiltonian_set_potential": (_i, [_p, _p, _l]),
+    "q3x_hamiltonian_prepare_kpoint": (_i, [_p, _p]),
+    "q3x_hamiltonian_set_projectors": (_i, [_p, _p, _i, _i, _p, _p]),
+    "q3x_hamiltonian_apply": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_device_alloc": (_i, [_p, _ul, _pp]),
+    "q3x_device_free": (_i, [_p, _p]),
+    "q3x_copy_to_device": (_i, [_p, _p, _p, _ul]),
+    "q3x_copy_to_host": (_i, [_p, _p, _p, _ul]),
+    "q3x_profile_enable": (_i, [_p, _i]),
+    "q3x_profile_read": (_i, [_p, _i, _p, _p, _i]),
+    "q3x_profile_kernel_name": (ctypes.c_char_p, [_i]),
+    "q3x_davidson_default_options": (None, [ctypes.POINTER(DavidsonOptions)]),
+    "q3x_davidson_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p, ctypes.POINTER(DavidsonReport)]),
+}
+
+_lib = None
+
+
+class Q3XError(RuntimeError):
+    """A q3x call returned a non-zero status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"q3x status {status}: {message}")
+        self.status = status
+
+
+class StateError(Q3XError):
+    """Call order violated (the reference throws std::logic_error here)."""
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m q3_amd.build` "
+                              "(the HIP library is required; there is no fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int):
+    if status != 0:
+        msg = lib().q3x_last_error().decode(errors="replace")
+        if status == 3:
+            raise StateError(status, msg)
+        raise Q3XError(status, msg)

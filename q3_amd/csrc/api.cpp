@@ -1,0 +1,526 @@
+// C ABI: device context, device buffers, Hamiltonian applicator.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "aux_kernels.h"
+#include "runtime.h"
+#include "zgemm.h"
+
+namespace q3x {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+bool hip_ok(hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+bool DevBuf::ensure(size_t bytes) {
+    if (bytes <= bytes_ && p_) return true;
+    release();
+    if (bytes == 0) bytes = 16;
+    if (!hip_ok(hipMalloc(&p_, bytes), "hipMalloc")) {
+        p_ = nullptr;
+        bytes_ = 0;
+        return false;
+    }
+    bytes_ = bytes;
+    return true;
+}
+
+void DevBuf::release() {
+    if (p_) (void)hipFree(p_);
+    p_ = nullptr;
+    bytes_ = 0;
+}
+
+bool DevBuf::upload(const void* src, size_t bytes, hipStream_t st) {
+    if (!ensure(bytes)) return false;
+    if (bytes == 0) return true;
+    return hip_ok(hipMemcpyAsync(p_, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D") &&
+           hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+int Profiler::mark(hipStream_t st) {
+    if (!on) return -1;
+    if (used == pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        pool.push_back(e);
+    }
+    (void)hipEventRecord(pool[used], st);
+    return static_cast<int>(used++);
+}
+
+bool Profiler::flush(hipStream_t st) {
+    if (!hip_ok(hipStreamSynchronize(st), "profiler sync")) return false;
+    for (const Span& s : spans) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pool[s.e0], pool[s.e1]) == hipSuccess) {
+            total_ms[s.id] += ms;
+            count[s.id] += 1;
+        }
+    }
+    spans.clear();
+    used = 0;
+    return true;
+}
+
+Profiler::~Profiler() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+namespace {
+
+FftPlan make_plan(int n) {
+    FftPlan p{};
+    p.n = n;
+    int r = n, s = 0;
+    while (r % 4 == 0 && s < 8) { p.radix[s++] = 4; r /= 4; }
+    while (r % 2 == 0 && s < 8) { p.radix[s++] = 2; r /= 2; }
+    while (r % 3 == 0 && s < 8) { p.radix[s++] = 3; r /= 3; }
+    while (r % 5 == 0 && s < 8) { p.radix[s++] = 5; r /= 5; }
+    p.nstage = (r == 1) ? s : -1;
+    return p;
+}
+
+bool build_tables(FftTables& t, int n, hipStream_t st) {
+    t.plan = make_plan(n);
+    if (t.plan.nstage < 0) {
+        set_error("FFT length is not 2,3,5-smooth");
+        return false;
+    }
+    std::vector<double> d(2 * n);
+    std::vector<float> f(2 * n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int k = 0; k < n; ++k) {
+        const long double a = -two_pi * k / n;
+        d[2 * k] = static_cast<double>(cosl(a));
+        d[2 * k + 1] = static_cast<double>(sinl(a));
+        f[2 * k] = static_cast<float>(d[2 * k]);
+        f[2 * k + 1] = static_cast<float>(d[2 * k + 1]);
+    }
+    return t.tw64.upload(d.data(), d.size() * sizeof(double), st) &&
+           t.tw32.upload(f.data(), f.size() * sizeof(float), st);
+}
+
+HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
+    HLocalArgs a{};
+    a.p1 = h->fft[0].plan;
+    a.p2 = h->fft[1].plan;
+    a.p3 = h->fft[2].plan;
+    a.tw1 = fp32 ? h->fft[0].tw32.get() : h->fft[0].tw64.get();
+    a.tw2 = fp32 ? h->fft[1].tw32.get() : h->fft[1].tw64.get();
+    a.tw3 = fp32 ? h->fft[2].tw32.get() : h->fft[2].tw64.get();
+    a.nst = h->nst;
+    a.ncol = h->ncol;
+    a.sticks_per_block = h->sticks_per_block;
+    a.rows_per_pass = fp32 ? h->rows_per_pass32 : h->rows_per_pass64;
+    a.stick_off = h->stick_off.as<int>();
+    a.stick_i2 = h->stick_i2.as<int>();
+    a.stick_col = h->stick_col.as<int>();
+    a.col_i1 = h->col_i1.as<int>();
+    a.ent_pw = h->ent_pw.as<int>();
+    a.ent_pos = h->ent_pos.as<int>();
+    a.ekin = h->ekin.as<double>();
+    a.outside = h->outside.as<int>();
+    a.n_outside = h->n_outside;
+    return a;
+}
+
+// Largest band chunk whose stick buffer stays under 2 GiB.
+int band_chunk(const q3x_hamiltonian_s* h, int nvec) {
+    const size_t per = static_cast<size_t>(h->dims[2]) * h->nst * 16;
+    const size_t cap = size_t(2) << 30;
+    int c = static_cast<int>(std::max<size_t>(1, cap / std::max<size_t>(per, 1)));
+    return std::min(c, nvec);
+}
+
+}  // namespace
+
+q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long ldpsi, double* d_hpsi, long ldh,
+                                    int nvec, int precision) {
+    if (!h || !h->kp) {
+        set_error("apply: prepare_kpoint() must be called before apply (serial.hpp:487-492)");
+        return Q3X_ERR_STATE;
+    }
+    if (!h->have_potential) {
+        set_error("apply: set_potential() must be called before apply");
+        return Q3X_ERR_STATE;
+    }
+    if (nvec <= 0) return Q3X_OK;
+    if (precision != Q3X_FP64 && precision != Q3X_MIXED) {
+        set_error("apply: precision must be Q3X_FP64 or Q3X_MIXED");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const bool fp32 = precision == Q3X_MIXED;
+    hipStream_t st = h->ctx->stream;
+    const HLocalArgs args = make_args(h, fp32);
+    const int chunk = band_chunk(h, nvec);
+    const size_t cs = fp32 ? 8 : 16;
+    if (!h->zbuf.ensure(static_cast<size_t>(chunk) * h->dims[2] * h->nst * cs)) return Q3X_ERR_DEVICE;
+    const void* vplane = fp32 ? h->v32.get() : h->v64.get();
+    Profiler& pr = h->ctx->prof;
+    for (int b0 = 0; b0 < nvec; b0 += chunk) {
+        const int nb = std::min(chunk, nvec - b0);
+        hipEvent_t marks[4];
+        int mi[4] = {-1, -1, -1, -1};
+        if (pr.on) {
+            for (int i = 0; i < 4; ++i) {
+                if (pr.used == pr.pool.size()) {
+                    hipEvent_t e;
+                    if (!hip_ok(hipEventCreate(&e), "hipEventCreate")) return Q3X_ERR_DEVICE;
+                    pr.pool.push_back(e);
+                }
+                mi[i] = static_cast<int>(pr.used++);
+                marks[i] = pr.pool[mi[i]];
+            }
+        }
+        if (!hip_ok(launch_hlocal(args, fp32, vplane, h->zbuf.get(), d_psi + 2L * b0 * ldpsi, ldpsi,
+                                  d_hpsi + 2L * b0 * ldh, ldh, nb, st, pr.on ? marks : nullptr),
+                    "H_local kernels"))
+            return Q3X_ERR_DEVICE;
+        pr.span(K_ZINV, mi[0], mi[1]);
+        pr.span(K_PLANE, mi[1], mi[2]);
+        pr.span(K_ZFWD, mi[2], mi[3]);
+    }
+    if (h->have_projectors && h->nproj > 0) {
+        const int np = h->nproj, npw = h->npw;
+        if (!h->P.ensure(static_cast<size_t>(np) * nvec * cs) || !h->W.ensure(static_cast<size_t>(np) * nvec * cs))
+            return Q3X_ERR_DEVICE;
+        const size_t ws = static_cast<size_t>(64) << 20;
+        if (!h->gemm_work.ensure(ws)) return Q3X_ERR_DEVICE;
+        // P = beta^H psi (serial.hpp:572-584)
+        const int m0 = pr.mark(st);
+        hipError_t e = fp32 ? zgemm(GemmTypes::F_D_F_F, 1, np, nvec, npw, 1.0, h->beta32.get(), npw, d_psi, ldpsi,
+                                    0.0, h->P.get(), np, h->gemm_work.get(), ws, st)
+                            : zgemm(GemmTypes::D_D_D_D, 1, np, nvec, npw, 1.0, h->beta64.get(), npw, d_psi, ldpsi,
+                                    0.0, h->P.get(), np, h->gemm_work.get(), ws, st);
+        if (!hip_ok(e, "projector overlap GEMM")) return Q3X_ERR_DEVICE;
+        const int m1 = pr.mark(st);
+        pr.span(K_KB_OVERLAP, m0, m1);
+        // W = D P per atom block (serial.hpp:587-679)
+        if (!hip_ok(launch_dblock_apply(fp32, np, nvec, h->pstart.as<int>(), h->psize.as<int>(), h->poff.as<long>(),
+                                        h->dblocks.as<double>(), h->P.get(), h->W.get(), st),
+                    "D block kernel"))
+            return Q3X_ERR_DEVICE;
+        const int m2 = pr.mark(st);
+        pr.span(K_KB_DBLOCK, m1, m2);
+        // Hpsi += beta W (serial.hpp:688-700)
+        e = fp32 ? zgemm(GemmTypes::F_F_F_D, 0, npw, nvec, np, 1.0, h->beta32.get(), npw, h->W.get(), np, 1.0, d_hpsi,
+                         ldh, nullptr, 0, st)
+                 : zgemm(GemmTypes::D_D_D_D, 0, npw, nvec, np, 1.0, h->beta64.get(), npw, h->W.get(), np, 1.0, d_hpsi,
+                         ldh, nullptr, 0, st);
+        if (!hip_ok(e, "projector update GEMM")) return Q3X_ERR_DEVICE;
+        const int m3 = pr.mark(st);
+        pr.span(K_KB_UPDATE, m2, m3);
+    }
+    return Q3X_OK;
+}
+
+}  // namespace q3x
+
+using namespace q3x;
+
+extern "C" {
+
+const char* q3x_last_error(void) { return g_last_error.c_str(); }
+const char* q3x_version(void) { return "q3x 0.1 (gfx950)"; }
+
+q3x_status q3x_context_create(int device, q3x_context* out) {
+    if (!out) {
+        set_error("q3x_context_create: null out");
+        return Q3X_ERR_ARGUMENT;
+    }
+    int n = 0;
+    if (!hip_ok(hipGetDeviceCount(&n), "hipGetDeviceCount")) return Q3X_ERR_DEVICE;
+    if (device < 0 || device >= n) {
+        set_error("q3x_context_create: no such device");
+        return Q3X_ERR_DEVICE;
+    }
+    if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return Q3X_ERR_DEVICE;
+    auto* c = new q3x_context_s;
+    c->device = device;
+    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+        delete c;
+        return Q3X_ERR_DEVICE;
+    }
+    int lds = 0, cus = 0;
+    (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+    c->max_lds = lds > 0 ? static_cast<size_t>(lds) : 65536;
+    c->num_cus = cus;
+    *out = c;
+    return Q3X_OK;
+}
+
+q3x_status q3x_context_destroy(q3x_context ctx) {
+    if (!ctx) return Q3X_OK;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return Q3X_OK;
+}
+
+q3x_status q3x_context_synchronize(q3x_context ctx) {
+    if (!ctx) {
+        set_error("null context");
+        return Q3X_ERR_ARGUMENT;
+    }
+    return hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize") ? Q3X_OK : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_profile_enable(q3x_context ctx, int enable) {
+    if (!ctx) {
+        set_error("null context");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
+    ctx->prof.on = enable != 0;
+    return Q3X_OK;
+}
+
+q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* count, int reset) {
+    if (!ctx || n < 0) {
+        set_error("q3x_profile_read: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
+    for (int i = 0; i < n && i < K_COUNT; ++i) {
+        if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
+        if (count) count[i] = ctx->prof.count[i];
+    }
+    if (reset)
+        for (int i = 0; i < K_COUNT; ++i) {
+            ctx->prof.total_ms[i] = 0;
+            ctx->prof.count[i] = 0;
+        }
+    return Q3X_OK;
+}
+
+const char* q3x_profile_kernel_name(int id) {
+    static const char* names[K_COUNT] = {"zsticks_inverse", "plane_sandwich", "zsticks_forward",
+                                         "kb_overlap_gemm", "kb_dblock",      "kb_update_gemm"};
+    return (id >= 0 && id < K_COUNT) ? names[id] : "";
+}
+
+q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {
+    if (!ctx || !out) {
+        set_error("q3x_device_alloc: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    return hip_ok(hipMalloc(out, bytes ? bytes : 16), "hipMalloc") ? Q3X_OK : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_device_free(q3x_context ctx, void* p) {
+    (void)ctx;
+    return hip_ok(hipFree(p), "hipFree") ? Q3X_OK : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_copy_to_device(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
+    if (!ctx) return Q3X_ERR_ARGUMENT;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D") &&
+                   hip_ok(hipStreamSynchronize(ctx->stream), "sync")
+               ? Q3X_OK
+               : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
+    if (!ctx) return Q3X_ERR_ARGUMENT;
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H") &&
+                   hip_ok(hipStreamSynchronize(ctx->stream), "sync")
+               ? Q3X_OK
+               : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian* out) {
+    if (!ctx || !b || !out) {
+        set_error("q3x_hamiltonian_create: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    auto* h = new q3x_hamiltonian_s;
+    h->ctx = ctx;
+    h->basis = b;
+    for (int d = 0; d < 3; ++d) h->dims[d] = b->d.wave[d];
+    h->npoints = static_cast<long>(h->dims[0]) * h->dims[1] * h->dims[2];
+    for (int d = 0; d < 3; ++d)
+        if (!build_tables(h->fft[d], h->dims[d], ctx->stream)) {
+            delete h;
+            return Q3X_ERR_DEVICE;
+        }
+    *out = h;
+    return Q3X_OK;
+}
+
+q3x_status q3x_hamiltonian_destroy(q3x_hamiltonian h) {
+    if (h && h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
+    delete h;
+    return Q3X_OK;
+}
+
+q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v, long n) {
+    if (!h || !v || n != h->npoints) {
+        set_error("set_potential: V must hold n1*n2*n3 wave-grid points");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
+    const double inv = 1.0 / static_cast<double>(h->npoints);  // serial.hpp:69-73
+    std::vector<double> d(h->npoints);
+    std::vector<float> f(h->npoints);
+    for (int i1 = 0; i1 < n1; ++i1)
+        for (int i2 = 0; i2 < n2; ++i2)
+            for (int i3 = 0; i3 < n3; ++i3) {
+                const double s = v[(static_cast<long>(i1) * n2 + i2) * n3 + i3] * inv;
+                const long o = (static_cast<long>(i3) * n2 + i2) * n1 + i1;
+                d[o] = s;
+                f[o] = static_cast<float>(s);
+            }
+    if (!h->v64.upload(d.data(), d.size() * sizeof(double), h->ctx->stream) ||
+        !h->v32.upload(f.data(), f.size() * sizeof(float), h->ctx->stream))
+        return Q3X_ERR_DEVICE;
+    h->have_potential = true;
+    return Q3X_OK;
+}
+
+q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
+    if (!h || !k || k->basis != h->basis) {
+        set_error("prepare_kpoint: k-point must come from the Hamiltonian's basis");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const KPointData& K = k->d;
+    const StickLayout& S = K.sticks;
+    hipStream_t st = h->ctx->stream;
+    h->npw = static_cast<int>(K.gidx.size());
+    h->nst = S.nst;
+    h->ncol = S.ncol;
+    std::vector<int> out;
+    for (int p = 0; p < h->npw; ++p)
+        if (K.fftmap[p] < 0) out.push_back(p);
+    h->n_outside = static_cast<int>(out.size());
+    const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
+    h->sticks_per_block = std::max(1, std::min(64, 1536 / n3));
+    // plane kernel LDS: twiddles + [n2][ncol] plane + rows_per_pass rows of n1
+    for (int fp32 = 0; fp32 < 2; ++fp32) {
+        const size_t cs = fp32 ? 8 : 16;
+        const size_t fixed = cs * (n1 + n2 + static_cast<size_t>(n2) * S.ncol);
+        long rp = 0;
+        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * n1));
+        rp = std::min<long>(rp, 16);
+        rp = std::min<long>(rp, n2);
+        if (rp < 1) {
+            set_error("prepare_kpoint: the occupied plane does not fit in LDS for this box");
+            return Q3X_ERR_ARGUMENT;
+        }
+        (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(rp);
+    }
+    bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st) &&
+              h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st) &&
+              h->stick_col.upload(S.stick_col.data(), S.stick_col.size() * sizeof(int), st) &&
+              h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st) &&
+              h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st) &&
+              h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st) &&
+              h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st) &&
+              h->outside.upload(out.data(), out.size() * sizeof(int), st);
+    if (!ok) return Q3X_ERR_DEVICE;
+    h->kp = k;
+    h->have_projectors = false;
+    h->nproj = 0;
+    return Q3X_OK;
+}
+
+q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta, int nproj, int natoms,
+                                          const int* block_sizes, const double* d_blocks) {
+    if (!h || !h->kp) {
+        set_error("set_projectors: prepare_kpoint() first");
+        return Q3X_ERR_STATE;
+    }
+    if (nproj < 0 || natoms < 0 || (nproj > 0 && (!beta || !block_sizes || !d_blocks))) {
+        set_error("set_projectors: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    int total = 0;
+    for (int a = 0; a < natoms; ++a) total += block_sizes[a];
+    if (total != nproj) {
+        set_error("set_projectors: block sizes must sum to nproj");
+        return Q3X_ERR_ARGUMENT;
+    }
+    hipStream_t st = h->ctx->stream;
+    const size_t n = static_cast<size_t>(h->npw) * nproj;
+    std::vector<float> b32(2 * n);
+    for (size_t i = 0; i < 2 * n; ++i) b32[i] = static_cast<float>(beta[i]);
+    std::vector<int> ps(nproj), pz(nproj);
+    std::vector<long> po(nproj);
+    std::vector<double> D;
+    int start = 0;
+    long off = 0;
+    for (int a = 0; a < natoms; ++a) {
+        const int bs = block_sizes[a];
+        for (int q = 0; q < bs * bs; ++q) {
+            const double dv = d_blocks[off + q];
+            D.push_back(std::abs(dv) > 1e-15 ? dv : 0.0);  // d_matrix_element_threshold
+        }
+        for (int i = 0; i < bs; ++i) {
+            ps[start + i] = start;
+            pz[start + i] = bs;
+            po[start + i] = off;
+        }
+        start += bs;
+        off += static_cast<long>(bs) * bs;
+    }
+    if (D.empty()) D.push_back(0.0);
+    bool ok = h->beta64.upload(beta, n * 16, st) && h->beta32.upload(b32.data(), n * 8, st) &&
+              h->dblocks.upload(D.data(), D.size() * sizeof(double), st) &&
+              h->pstart.upload(ps.data(), ps.size() * sizeof(int), st) &&
+              h->psize.upload(pz.data(), pz.size() * sizeof(int), st) &&
+              h->poff.upload(po.data(), po.size() * sizeof(long), st);
+    if (!ok) return Q3X_ERR_DEVICE;
+    h->nproj = nproj;
+    h->have_projectors = true;
+    return Q3X_OK;
+}
+
+q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi, int nvec,
+                                        int precision) {
+    if (!h) {
+        set_error("apply_device: null Hamiltonian");
+        return Q3X_ERR_ARGUMENT;
+    }
+    return hamiltonian_apply_device(h, static_cast<const double*>(d_psi), h->npw, static_cast<double*>(d_hpsi), h->npw,
+                                    nvec, precision);
+}
+
+q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* hpsi, int nvec, int precision) {
+    if (!h || !h->kp) {
+        set_error("apply: prepare_kpoint() must be called before apply (serial.hpp:487-492)");
+        return Q3X_ERR_STATE;
+    }
+    if (nvec < 0 || (nvec > 0 && (!psi || !hpsi))) {
+        set_error("apply: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (nvec == 0) return Q3X_OK;
+    hipStream_t st = h->ctx->stream;
+    const size_t bytes = static_cast<size_t>(h->npw) * nvec * 16;
+    if (!h->io_psi.ensure(bytes) || !h->io_hpsi.ensure(bytes)) return Q3X_ERR_DEVICE;
+    if (!hip_ok(hipMemcpyAsync(h->io_psi.get(), psi, bytes, hipMemcpyHostToDevice, st), "H2D psi"))
+        return Q3X_ERR_DEVICE;
+    q3x_status s = hamiltonian_apply_device(h, h->io_psi.as<double>(), h->npw, h->io_hpsi.as<double>(), h->npw, nvec,
+                                            precision);
+    if (s != Q3X_OK) return s;
+    if (!hip_ok(hipMemcpyAsync(hpsi, h->io_hpsi.get(), bytes, hipMemcpyDeviceToHost, st), "D2H hpsi") ||
+        !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+        return Q3X_ERR_DEVICE;
+    return Q3X_OK;
+}
+
+}  // extern "C"

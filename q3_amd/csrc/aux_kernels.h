@@ -1,0 +1,17 @@
+// Small kernels around the H-apply and Davidson iteration (aux_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace q3x {
+
+hipError_t launch_dblock_apply(bool fp32, int nproj, int nvec, const int* pstart, const int* psize, const long* poff,
+                               const double* D, const void* P, void* W, hipStream_t st);
+hipError_t launch_teter(int npw, int ncol, const double* ekin, const double* eps, double* R, long ldr,
+                        hipStream_t st);
+hipError_t launch_column_norm2(int npw, int ncol, const double* X, long ldx, double* out, hipStream_t st);
+hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const double* s, double* Y, long ldy,
+                             hipStream_t st);
+hipError_t launch_zero(double* p, long ndouble, hipStream_t st);
+hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st);
+
+}  // namespace q3x

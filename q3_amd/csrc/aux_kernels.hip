@@ -1,0 +1,137 @@
+// Small fused kernels around the H-apply and the Davidson iteration.
+#include <hip/hip_runtime.h>
+
+#include "aux_kernels.h"
+
+namespace q3x {
+
+// W = D P for block-diagonal D (one real p_a x p_a block per atom, column-major, packed in
+// atom order) -- serial.hpp:587-679.  P, W: nproj x nvec complex.
+template <typename T>
+__global__ void dblock_apply(int nproj, int nvec, const int* __restrict__ pstart, const int* __restrict__ psize,
+                             const long* __restrict__ poff, const double* __restrict__ D, const T* __restrict__ P,
+                             T* __restrict__ W) {
+    const long idx = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<long>(nproj) * nvec) return;
+    const int v = static_cast<int>(idx / nproj);
+    const int p = static_cast<int>(idx - static_cast<long>(v) * nproj);
+    const int s = pstart[p], bs = psize[p];
+    const double* blk = D + poff[p];
+    const int row = p - s;
+    double re = 0.0, im = 0.0;
+    const T* col = P + 2L * (static_cast<long>(v) * nproj + s);
+    for (int q = 0; q < bs; ++q) {
+        const double d = blk[row + q * bs];
+        re += d * static_cast<double>(col[2 * q]);
+        im += d * static_cast<double>(col[2 * q + 1]);
+    }
+    W[2 * idx] = static_cast<T>(re);
+    W[2 * idx + 1] = static_cast<T>(im);
+}
+
+hipError_t launch_dblock_apply(bool fp32, int nproj, int nvec, const int* pstart, const int* psize, const long* poff,
+                               const double* D, const void* P, void* W, hipStream_t st) {
+    const long total = static_cast<long>(nproj) * nvec;
+    if (total == 0) return hipSuccess;
+    dim3 g(static_cast<unsigned>((total + 255) / 256));
+    if (fp32)
+        hipLaunchKernelGGL(dblock_apply<float>, g, dim3(256), 0, st, nproj, nvec, pstart, psize, poff, D,
+                           static_cast<const float*>(P), static_cast<float*>(W));
+    else
+        hipLaunchKernelGGL(dblock_apply<double>, g, dim3(256), 0, st, nproj, nvec, pstart, psize, poff, D,
+                           static_cast<const double*>(P), static_cast<double*>(W));
+    return hipGetLastError();
+}
+
+// Teter-style preconditioner of the correction columns (serial.hpp:1138-1156):
+//   r_g /= max(0.5 (1 + x + sqrt(1 + (x-1)^2)), 1e-4),  x = T_g - e_c
+__global__ void teter_precondition(int npw, int ncol, const double* __restrict__ ekin, const double* __restrict__ eps,
+                                   double* __restrict__ R, long ldr) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= npw || c >= ncol) return;
+    const double x = ekin[g] - eps[c];
+    double den = 0.5 * (1.0 + x + sqrt(1.0 + (x - 1.0) * (x - 1.0)));
+    if (den < 1e-4) den = 1e-4;
+    double* col = R + 2L * c * ldr;
+    col[2 * g] /= den;
+    col[2 * g + 1] /= den;
+}
+
+hipError_t launch_teter(int npw, int ncol, const double* ekin, const double* eps, double* R, long ldr,
+                        hipStream_t st) {
+    if (npw == 0 || ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(teter_precondition, dim3((npw + 255) / 256, ncol), dim3(256), 0, st, npw, ncol, ekin, eps, R,
+                       ldr);
+    return hipGetLastError();
+}
+
+// Residual columns and their squared norms in one pass:
+//   R_c = HV x_c - e_c V x_c is formed by the caller as two GEMMs; this reduces |R_c|^2
+//   deterministically (one workgroup per column, fixed-order tree).
+__global__ void __launch_bounds__(256) column_norm2(int npw, const double* __restrict__ X, long ldx,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const int c = blockIdx.x;
+    const double* col = X + 2L * c * ldx;
+    double s = 0.0;
+    for (int g = threadIdx.x; g < npw; g += 256) s += col[2 * g] * col[2 * g] + col[2 * g + 1] * col[2 * g + 1];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+hipError_t launch_column_norm2(int npw, int ncol, const double* X, long ldx, double* out, hipStream_t st) {
+    if (ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(column_norm2, dim3(ncol), dim3(256), 0, st, npw, X, ldx, out);
+    return hipGetLastError();
+}
+
+// Y (npw x ncol) = X, with column c scaled by s[c] (real).  ld in complex elements.
+__global__ void scale_copy_columns(int npw, const double* __restrict__ X, long ldx, const double* __restrict__ s,
+                                   double* __restrict__ Y, long ldy) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= npw) return;
+    const double f = s ? s[c] : 1.0;
+    Y[2 * (g + c * ldy)] = f * X[2 * (g + c * ldx)];
+    Y[2 * (g + c * ldy) + 1] = f * X[2 * (g + c * ldx) + 1];
+}
+
+hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const double* s, double* Y, long ldy,
+                             hipStream_t st) {
+    if (npw == 0 || ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(scale_copy_columns, dim3((npw + 255) / 256, ncol), dim3(256), 0, st, npw, X, ldx, s, Y, ldy);
+    return hipGetLastError();
+}
+
+// Zero-fill of a device range with a kernel (keeps memsets out of captured graphs).
+__global__ void fill_zero(double* __restrict__ p, long n) {
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+hipError_t launch_zero(double* p, long ndouble, hipStream_t st) {
+    if (ndouble <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_zero, dim3(static_cast<unsigned>((ndouble + 255) / 256)), dim3(256), 0, st, p, ndouble);
+    return hipGetLastError();
+}
+
+// Unit vectors e_{idx[c]} in the columns of V (kinetic seed, davidson.hpp:1079-1090).
+__global__ void set_unit_columns(const int* __restrict__ idx, int ncol, double* __restrict__ V, long ldv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    V[2 * (idx[c] + static_cast<long>(c) * ldv)] = 1.0;
+}
+
+hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st) {
+    if (ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(set_unit_columns, dim3((ncol + 63) / 64), dim3(64), 0, st, idx, ncol, V, ldv);
+    return hipGetLastError();
+}
+
+}  // namespace q3x

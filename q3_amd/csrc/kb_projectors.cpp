@@ -1,0 +1,233 @@
+// Kleinman-Bylander projector set <k+G|beta> at one k-point, built on the host.
+//
+// beta_p(k+G) = Omega^-1/2 * beta_l(|k+G|) * Y_lm(k+G) * exp(-i 2pi (k+G).tau) * (-i)^l
+// (projector.hpp:42-199).  beta_l(q) is interpolated (4-point Lagrange, lagrange.hpp:16-30)
+// from a table on q = 0, 0.01, ... up to 1.1 sqrt(ecutrho) (+4 points)
+// (beta_function_table/init.hpp:45-52, uniform_q_grid.hpp:23-28), each entry the spherical
+// Bessel transform 4 pi \int r beta_r(r) r j_l(qr) dr by Simpson's rule on the radial mesh
+// (spherical_bessel.hpp:33-53, simpson.hpp:22-48).  Projector order: atom, beta, then
+// m = 0, +1, -1, +2, -2, ... (harmonics.hpp:40-69).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "q3x_internal.h"
+
+namespace q3x {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kFourPi = 4.0 * kPi;
+constexpr double kDq = 0.01;  // default_q_grid_spacing (tolerances.hpp:109)
+
+// j_l(x): ascending series below x = l + 1, closed forms / upward recurrence above
+// (bessel.hpp:29-190).
+double jl_series(int l, double x) {
+    double xl = l == 0 ? 1.0 : std::pow(x, l);
+    double dfact = 1.0;
+    for (int i = 2 * l + 1; i >= 1; i -= 2) dfact *= i;
+    double h = 0.5 * x * x, term = 1.0, sum = 1.0;
+    for (int i = 1; i <= 60; ++i) {
+        term *= -h / (i * (2.0 * l + 2.0 * i + 1.0));
+        sum += term;
+        if (std::abs(term) < 1e-18 * std::abs(sum)) break;
+    }
+    return (xl / dfact) * sum;
+}
+
+double jl(int l, double x) {
+    if (std::abs(x) < 1e-8) return l == 0 ? 1.0 : 0.0;
+    const double cross = l <= 2 ? 1.0 : l + 1.0;
+    if (std::abs(x) < cross) return jl_series(l, x);
+    const double s = std::sin(x), c = std::cos(x);
+    if (l == 0) return s / x;
+    if (l == 1) return (s / x - c) / x;
+    if (l == 2) return ((3.0 / (x * x) - 1.0) * s - 3.0 * c / x) / x;
+    double jm = s / x, jc = (s / x - c) / x;
+    for (int i = 2; i <= l; ++i) {
+        double jp = ((2 * i - 1) * jc / x) - jm;
+        jm = jc;
+        jc = jp;
+    }
+    return jc;
+}
+
+double simpson_rab(const std::vector<double>& f, const double* rab, int n) {
+    if (n < 3) return 0.0;
+    double s = 0.0;
+    for (int i = 1; i < n - 1; ++i) s += ((i & 1) ? 4.0 : 2.0) * f[i] * rab[i];
+    if (n & 1)
+        s += f[0] * rab[0] + f[n - 1] * rab[n - 1];
+    else
+        s += f[0] * rab[0] - f[n - 3] * rab[n - 3] * 0.25 + f[n - 2] * rab[n - 2] + f[n - 1] * rab[n - 1] * 1.25;
+    return s / 3.0;
+}
+
+std::vector<double> radial_table(const double* r, const double* rab, int cutoff, int l, const double* beta_r,
+                                 int nq) {
+    std::vector<double> t(nq), f(cutoff);
+    for (int iq = 0; iq < nq; ++iq) {
+        const double q = iq * kDq;
+        for (int i = 0; i < cutoff; ++i) {
+            const double j = (std::abs(q) < 1e-8) ? (l == 0 ? 1.0 : 0.0) : jl(l, q * r[i]);
+            f[i] = beta_r[i] * r[i] * j;
+        }
+        t[iq] = kFourPi * simpson_rab(f, rab, cutoff);
+    }
+    return t;
+}
+
+double interp4(const std::vector<double>& t, double q) {
+    const int nq = static_cast<int>(t.size());
+    if (q < 1e-10) return t[0];
+    const double fi = q / kDq;
+    const int b = static_cast<int>(fi);
+    const double u = fi - b;
+    if (b + 3 >= nq) return 0.0;
+    const double w1 = 1.0 - u, w2 = 2.0 - u, w3 = 3.0 - u;
+    return t[b] * w1 * w2 * w3 / 6.0 + t[b + 1] * u * w2 * w3 / 2.0 - t[b + 2] * u * w1 * w3 / 2.0 +
+           t[b + 3] * u * w1 * w2 / 6.0;
+}
+
+inline int lm_slot(int l, int m) { return m == 0 ? l * l : (m > 0 ? l * l + 2 * m - 1 : l * l - 2 * m); }
+
+// Real Y_lm with the Condon-Shortley phase via normalised associated-Legendre recursion
+// (harmonics.hpp:217-327).
+void real_ylm(int lmax, const double v[3], double v2, double* out) {
+    const double vm = std::sqrt(v2);
+    const bool flat = vm < 1e-9;
+    const double ct = flat ? 0.0 : v[2] / vm;
+    const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
+    const double phi = flat ? std::copysign(kPi / 2.0, v[1]) : std::atan2(v[1], v[0]);
+    std::vector<double> Q((lmax + 1) * (lmax + 2) / 2, 0.0);
+    auto at = [](int l, int m) { return l * (l + 1) / 2 + m; };
+    Q[at(0, 0)] = 1.0;
+    if (lmax >= 1) {
+        Q[at(1, 0)] = ct;
+        Q[at(1, 1)] = -st / std::sqrt(2.0);
+    }
+    for (int l = 2; l <= lmax; ++l) {
+        for (int m = 0; m <= l - 2; ++m) {
+            const double den = std::sqrt(static_cast<double>(l * l - m * m));
+            Q[at(l, m)] = ct * (2 * l - 1) / den * Q[at(l - 1, m)] -
+                          std::sqrt(static_cast<double>((l - 1) * (l - 1) - m * m)) / den * Q[at(l - 2, m)];
+        }
+        const double df = std::sqrt(static_cast<double>(2 * l - 1));
+        const double prev = Q[at(l - 1, l - 1)];
+        Q[at(l, l - 1)] = ct * df * prev;
+        Q[at(l, l)] = -df / std::sqrt(static_cast<double>(2 * l)) * st * prev;
+    }
+    out[0] = Q[at(0, 0)] / std::sqrt(kFourPi);
+    for (int l = 1; l <= lmax; ++l) {
+        const double nrm = std::sqrt(static_cast<double>(2 * l + 1) / kFourPi);
+        out[lm_slot(l, 0)] = nrm * Q[at(l, 0)];
+        const double off = nrm * std::sqrt(2.0);
+        for (int m = 1; m <= l; ++m) {
+            out[lm_slot(l, m)] = off * Q[at(l, m)] * std::cos(m * phi);
+            out[lm_slot(l, -m)] = off * Q[at(l, m)] * std::sin(m * phi);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace q3x
+
+using namespace q3x;
+
+extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, const double* atom_pos,
+                                        const int* atom_species, int nspecies, const int* nmesh,
+                                        const double* r_all, const double* rab_all, const int* nbeta,
+                                        const int* l_all, const int* cutoff_all, const double* beta_r_all,
+                                        double* beta_out, int* nproj_out) {
+    if (!b || !k || natoms < 0 || nspecies < 1 || !nmesh || !nbeta) {
+        set_error("q3x_kb_projectors: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const BasisData& B = b->d;
+    const KPointData& K = k->d;
+    const Lattice& L = B.lat;
+    const int npw = static_cast<int>(K.gidx.size());
+    // species tables
+    struct Sp {
+        const double *r, *rab, *beta_r;
+        int nmesh, nbeta;
+        const int *l, *cut;
+    };
+    std::vector<Sp> sp(nspecies);
+    size_t ro = 0, bo = 0, bro = 0;
+    int lmax = 0;
+    for (int s = 0; s < nspecies; ++s) {
+        sp[s] = {r_all + ro, rab_all + ro, beta_r_all + bro, nmesh[s], nbeta[s], l_all + bo, cutoff_all + bo};
+        for (int i = 0; i < nbeta[s]; ++i) {
+            if (l_all[bo + i] < 0 || l_all[bo + i] > 3 || cutoff_all[bo + i] > nmesh[s]) {
+                set_error("q3x_kb_projectors: projector l must be 0..3 and cutoff <= nmesh");
+                return Q3X_ERR_ARGUMENT;
+            }
+            lmax = std::max(lmax, l_all[bo + i]);
+        }
+        ro += nmesh[s];
+        bo += nbeta[s];
+        bro += static_cast<size_t>(nbeta[s]) * nmesh[s];
+    }
+    int nproj = 0;
+    for (int a = 0; a < natoms; ++a) {
+        int s = atom_species[a];
+        if (s < 0 || s >= nspecies) {
+            set_error("q3x_kb_projectors: atom species out of range");
+            return Q3X_ERR_ARGUMENT;
+        }
+        for (int i = 0; i < sp[s].nbeta; ++i) nproj += 2 * sp[s].l[i] + 1;
+    }
+    if (nproj_out) *nproj_out = nproj;
+    if (!beta_out) return Q3X_OK;
+
+    const int nq = static_cast<int>(std::sqrt(B.ecutrho) * 1.1 / kDq) + 4;
+    std::vector<std::vector<std::vector<double>>> table(nspecies);
+    for (int s = 0; s < nspecies; ++s)
+        for (int i = 0; i < sp[s].nbeta; ++i)
+            table[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
+                                            sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
+    double kc[3];
+    for (int d = 0; d < 3; ++d) kc[d] = K.k[0] * L.b[0][d] + K.k[1] * L.b[1][d] + K.k[2] * L.b[2][d];
+    const int nlm = (lmax + 1) * (lmax + 1);
+    std::vector<double> ylm(static_cast<size_t>(npw) * nlm), qabs(npw);
+    for (int p = 0; p < npw; ++p) {
+        const double* g = B.g[K.gidx[p]].g;
+        const double v[3] = {kc[0] + g[0], kc[1] + g[1], kc[2] + g[2]};
+        const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        qabs[p] = std::sqrt(v2) * L.tpa;
+        real_ylm(lmax, v, v2, &ylm[static_cast<size_t>(p) * nlm]);
+    }
+    const double inv_sqrt_vol = 1.0 / std::sqrt(L.volume);
+    auto* out = reinterpret_cast<zcplx*>(beta_out);
+    std::vector<double> radial(npw);
+    std::vector<zcplx> phase(npw);
+    int col = 0;
+    for (int a = 0; a < natoms; ++a) {
+        const int s = atom_species[a];
+        double tau[3];
+        for (int d = 0; d < 3; ++d)
+            tau[d] = atom_pos[3 * a] * L.a[0][d] + atom_pos[3 * a + 1] * L.a[1][d] + atom_pos[3 * a + 2] * L.a[2][d];
+        const double kdt = kc[0] * tau[0] + kc[1] * tau[1] + kc[2] * tau[2];
+        for (int p = 0; p < npw; ++p) {
+            const double* g = B.g[K.gidx[p]].g;
+            const double ph = kTwoPi * ((g[0] * tau[0] + g[1] * tau[1] + g[2] * tau[2]) + kdt);
+            phase[p] = zcplx(std::cos(ph), -std::sin(ph));
+        }
+        for (int i = 0; i < sp[s].nbeta; ++i) {
+            const int l = sp[s].l[i];
+            for (int p = 0; p < npw; ++p) radial[p] = interp4(table[s][i], qabs[p]);
+            static const zcplx minus_i_pow[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
+            const zcplx pref = inv_sqrt_vol * minus_i_pow[l % 4];
+            for (int mi = 0; mi < 2 * l + 1; ++mi) {
+                const int m = mi == 0 ? 0 : ((mi & 1) ? (mi + 1) / 2 : -(mi / 2));
+                const int slot = lm_slot(l, m);
+                zcplx* dst = out + static_cast<size_t>(col) * npw;
+                for (int p = 0; p < npw; ++p) dst[p] = radial[p] * ylm[static_cast<size_t>(p) * nlm + slot] * phase[p] * pref;
+                ++col;
+            }
+        }
+    }
+    return Q3X_OK;
+}

@@ -1,0 +1,100 @@
+// Device runtime objects behind the C ABI: context, device buffers, Hamiltonian state.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fft_lds.h"
+#include "hlocal.h"
+#include "q3x_internal.h"
+
+namespace q3x {
+
+bool hip_ok(hipError_t e, const char* what);
+
+// Owning device allocation; grows, never shrinks.
+class DevBuf {
+  public:
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    bool ensure(size_t bytes);
+    void release();
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p_);
+    }
+    void* get() const { return p_; }
+    size_t size() const { return bytes_; }
+    bool upload(const void* src, size_t bytes, hipStream_t st);
+
+  private:
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Per-kernel-family timing with HIP events recorded on the context stream around each launch.
+enum KernelId { K_ZINV = 0, K_PLANE, K_ZFWD, K_KB_OVERLAP, K_KB_DBLOCK, K_KB_UPDATE, K_COUNT };
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    struct Span {
+        int id, e0, e1;
+    };
+    std::vector<Span> spans;
+    double total_ms[K_COUNT] = {};
+    long count[K_COUNT] = {};
+    int mark(hipStream_t st);  // records the next pool event, returns its index (-1 if off)
+    void span(int id, int e0, int e1) {
+        if (on && e0 >= 0 && e1 >= 0) spans.push_back({id, e0, e1});
+    }
+    bool flush(hipStream_t st);
+    ~Profiler();
+};
+
+struct FftTables {
+    FftPlan plan;
+    DevBuf tw64, tw32;
+};
+
+}  // namespace q3x
+
+struct q3x_context_s {
+    q3x::Profiler prof;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t max_lds = 65536;
+    int num_cus = 0;
+};
+
+struct q3x_hamiltonian_s {
+    q3x_context ctx = nullptr;
+    const q3x_basis_s* basis = nullptr;
+    int dims[3] = {0, 0, 0};
+    long npoints = 0;
+    q3x::FftTables fft[3];
+    // potential V(r)/N, plane-major [i3][i2][i1]
+    q3x::DevBuf v64, v32;
+    bool have_potential = false;
+    // prepared k-point
+    const q3x_kpoint_s* kp = nullptr;
+    int npw = 0, nst = 0, ncol = 0, n_outside = 0;
+    int sticks_per_block = 16, rows_per_pass64 = 8, rows_per_pass32 = 8;
+    q3x::DevBuf stick_off, stick_i2, stick_col, col_i1, ent_pw, ent_pos, ekin, outside;
+    // projectors of the prepared k-point
+    int nproj = 0;
+    bool have_projectors = false;
+    q3x::DevBuf beta64, beta32, dblocks, pstart, psize, poff;
+    // scratch
+    q3x::DevBuf zbuf, P, W, gemm_work, io_psi, io_hpsi;
+};
+
+namespace q3x {
+// Applies H to nvec device vectors (ld = npw) on the context stream.
+q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long ldpsi, double* d_hpsi, long ldh,
+                                    int nvec, int precision);
+}  // namespace q3x

@@ -1,0 +1,339 @@
+// Host-side setup of the path: lattice, FFT box sizing, G sphere, per-k plane-wave lists,
+// FFT-box maps and the z-stick layout the pencil FFT kernels run on.
+//
+// The G sphere and per-k lists must agree bit for bit with the reference, so the
+// floating-point expressions below keep the reference's operation order and the sorts use
+// std::sort with the reference's comparators on the reference's input order.  FP
+// contraction is off for this file (no FMA), as in an x86-64 build of the reference.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+#include <stdexcept>
+
+#include "q3x_internal.h"
+
+namespace q3x {
+
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr double kCutoffSlack = 1e-8;  // energy_cutoff_rounding_tolerance (tolerances.hpp:129)
+}  // namespace
+
+// Crystal::compute_derived_quantities (crystal.hpp:156-177) with Matrix3x3::determinant and
+// inverse (matrix_3x3.hpp:51-71): B = (A^-1)^T.
+Lattice make_lattice(const double rows[9], double alat) {
+    Lattice L{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) L.a[i][j] = rows[3 * i + j];
+    const double(&r)[3][3] = L.a;
+    const double det = r[0][0] * (r[1][1] * r[2][2] - r[1][2] * r[2][1]) -
+                       r[0][1] * (r[1][0] * r[2][2] - r[1][2] * r[2][0]) +
+                       r[0][2] * (r[1][0] * r[2][1] - r[1][1] * r[2][0]);
+    double inv[3][3];
+    inv[0][0] = (r[1][1] * r[2][2] - r[1][2] * r[2][1]) / det;
+    inv[0][1] = (r[0][2] * r[2][1] - r[0][1] * r[2][2]) / det;
+    inv[0][2] = (r[0][1] * r[1][2] - r[0][2] * r[1][1]) / det;
+    inv[1][0] = (r[1][2] * r[2][0] - r[1][0] * r[2][2]) / det;
+    inv[1][1] = (r[0][0] * r[2][2] - r[0][2] * r[2][0]) / det;
+    inv[1][2] = (r[0][2] * r[1][0] - r[0][0] * r[1][2]) / det;
+    inv[2][0] = (r[1][0] * r[2][1] - r[1][1] * r[2][0]) / det;
+    inv[2][1] = (r[0][1] * r[2][0] - r[0][0] * r[2][1]) / det;
+    inv[2][2] = (r[0][0] * r[1][1] - r[0][1] * r[1][0]) / det;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) L.b[i][j] = inv[j][i];
+    L.alat = alat;
+    L.tpa = kTwoPi / alat;
+    L.tpa2 = L.tpa * L.tpa;
+    L.volume = std::abs(det) * alat * alat * alat;
+    return L;
+}
+
+int smooth_fft_length(int n, int factor) {
+    for (int c = n;; ++c) {
+        if (factor > 1 && c % factor) continue;
+        int rest = c;
+        for (int p : {2, 3, 5})
+            while (rest % p == 0) rest /= p;
+        if (rest == 1) return c;
+    }
+}
+
+static inline void g_of_miller(const Lattice& L, int i1, int i2, int i3, double g[3]) {
+    for (int d = 0; d < 3; ++d) g[d] = i1 * L.b[0][d] + i2 * L.b[1][d] + i3 * L.b[2][d];
+}
+static inline double norm2(const double v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; }
+
+// FFTGridSizer::calculate_grid_dimensions (grid_sizer.hpp:101-160).
+void fft_box_for_cutoff(const Lattice& L, double ecut_ry, const int factors[3], int dims[3]) {
+    const double gmax2 = ecut_ry / L.tpa2;
+    const double gmax = std::sqrt(gmax2);
+    int reach[3];
+    for (int d = 0; d < 3; ++d) reach[d] = static_cast<int>(std::floor(gmax * std::sqrt(norm2(L.a[d])))) + 1;
+    int extent[3] = {0, 0, 0};
+    for (int i3 = -reach[2]; i3 <= reach[2]; ++i3)
+        for (int i2 = -reach[1]; i2 <= reach[1]; ++i2)
+            for (int i1 = -reach[0]; i1 <= reach[0]; ++i1) {
+                double g[3];
+                g_of_miller(L, i1, i2, i3, g);
+                if (norm2(g) < gmax2 + kCutoffSlack) {
+                    extent[0] = std::max(extent[0], std::abs(i1));
+                    extent[1] = std::max(extent[1], std::abs(i2));
+                    extent[2] = std::max(extent[2], std::abs(i3));
+                }
+            }
+    for (int d = 0; d < 3; ++d) dims[d] = smooth_fft_length(2 * extent[d] + 1, factors ? factors[d] : 1);
+}
+
+// G sphere of the density grid (g_vectors_grid.hpp:23-104, serial full sphere): i1, i2, i3
+// nested in that order, |G|^2 <= ecutrho/(2pi/a)^2 + 1e-8, then std::sort by |G|^2.
+static void build_sphere(BasisData& B) {
+    const Lattice& L = B.lat;
+    const double cut = B.ecutrho / L.tpa2 + kCutoffSlack;
+    const int h1 = B.rho[0] / 2, h2 = B.rho[1] / 2, h3 = B.rho[2] / 2;
+    B.g.clear();
+    B.g.reserve(static_cast<size_t>(B.rho[0]) * B.rho[1] * B.rho[2] / 4);
+    for (int i1 = -h1; i1 <= h1; ++i1)
+        for (int i2 = -h2; i2 <= h2; ++i2)
+            for (int i3 = -h3; i3 <= h3; ++i3) {
+                GEntry e;
+                g_of_miller(L, i1, i2, i3, e.g);
+                e.g2 = norm2(e.g);
+                if (e.g2 <= cut) {
+                    e.m[0] = i1;
+                    e.m[1] = i2;
+                    e.m[2] = i3;
+                    B.g.push_back(e);
+                }
+            }
+    std::sort(B.g.begin(), B.g.end(), [](const GEntry& x, const GEntry& y) { return x.g2 < y.g2; });
+}
+
+static int wrap(int m, int n) { return m >= 0 ? m : m + n; }
+
+// Stick layout for the pencil FFT.  Permutation-only bookkeeping: values are never reordered
+// in HBM, the kernels gather/scatter through ent_pw.
+static void build_sticks(const BasisData& B, KPointData& kp) {
+    const int n1 = B.wave[0], n2 = B.wave[1], n3 = B.wave[2];
+    StickLayout& S = kp.sticks;
+    const int npw = static_cast<int>(kp.gidx.size());
+    // distinct columns (wrapped i1) present
+    std::vector<char> has_col(n1, 0);
+    for (int p = 0; p < npw; ++p) {
+        if (kp.fftmap[p] < 0) continue;
+        has_col[wrap(B.g[kp.gidx[p]].m[0], n1)] = 1;
+    }
+    std::vector<int> col_of_i1(n1, -1);
+    S.col_i1.clear();
+    for (int i = 0; i < n1; ++i)
+        if (has_col[i]) {
+            col_of_i1[i] = static_cast<int>(S.col_i1.size());
+            S.col_i1.push_back(i);
+        }
+    S.ncol = static_cast<int>(S.col_i1.size());
+    // sticks keyed by (column, wrapped i2)
+    std::vector<int> stick_id(static_cast<size_t>(S.ncol) * n2, -1);
+    std::vector<int> pw_stick(npw, -1), pw_z(npw, -1);
+    for (int p = 0; p < npw; ++p) {
+        if (kp.fftmap[p] < 0) continue;
+        const GEntry& e = B.g[kp.gidx[p]];
+        int c = col_of_i1[wrap(e.m[0], n1)];
+        int key = c * n2 + wrap(e.m[1], n2);
+        stick_id[key] = 0;
+        pw_stick[p] = key;
+        pw_z[p] = wrap(e.m[2], n3);
+    }
+    S.stick_col.clear();
+    S.stick_i2.clear();
+    for (int key = 0; key < S.ncol * n2; ++key)
+        if (stick_id[key] == 0) {
+            stick_id[key] = static_cast<int>(S.stick_col.size());
+            S.stick_col.push_back(key / n2);
+            S.stick_i2.push_back(key % n2);
+        }
+    S.nst = static_cast<int>(S.stick_col.size());
+    // entries: counting sort by stick, then by z within a stick
+    std::vector<int> count(S.nst + 1, 0);
+    for (int p = 0; p < npw; ++p)
+        if (pw_stick[p] >= 0) count[stick_id[pw_stick[p]] + 1]++;
+    for (int s = 0; s < S.nst; ++s) count[s + 1] += count[s];
+    S.stick_off = count;
+    int nent = count[S.nst];
+    S.ent_pw.assign(nent, -1);
+    S.ent_pos.assign(nent, -1);
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    for (int p = 0; p < npw; ++p) {
+        if (pw_stick[p] < 0) continue;
+        int s = stick_id[pw_stick[p]];
+        S.ent_pw[fill[s]++] = p;
+    }
+    for (int s = 0; s < S.nst; ++s) {
+        auto first = S.ent_pw.begin() + S.stick_off[s];
+        auto last = S.ent_pw.begin() + S.stick_off[s + 1];
+        std::sort(first, last, [&](int x, int y) { return pw_z[x] < pw_z[y]; });
+        for (int j = S.stick_off[s]; j < S.stick_off[s + 1]; ++j) S.ent_pos[j] = s * n3 + pw_z[S.ent_pw[j]];
+    }
+}
+
+// PlaneWaveSelector::select (plane_wave_selector.hpp:41-109) and the G -> FFT-box map
+// (planewave_potential_utils.hpp:141-164, fft_grid_serial.hpp:301-306).
+void build_kpoint(const BasisData& B, const double k[3], KPointData& kp) {
+    const Lattice& L = B.lat;
+    for (int d = 0; d < 3; ++d) kp.k[d] = k[d];
+    double kc[3];
+    for (int d = 0; d < 3; ++d) kc[d] = k[0] * L.b[0][d] + k[1] * L.b[1][d] + k[2] * L.b[2][d];
+    const double cut = B.ecutwfc / L.tpa2 + kCutoffSlack;
+    std::vector<int> sel;
+    std::vector<double> kin;
+    for (int i = 0; i < static_cast<int>(B.g.size()); ++i) {
+        const double v[3] = {kc[0] + B.g[i].g[0], kc[1] + B.g[i].g[1], kc[2] + B.g[i].g[2]};
+        const double q2 = norm2(v);
+        if (q2 <= cut) {
+            sel.push_back(i);
+            kin.push_back(q2 * L.tpa2);
+        }
+    }
+    std::vector<int> order(sel.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&kin](int x, int y) { return kin[x] < kin[y]; });
+    const int npw = static_cast<int>(sel.size());
+    kp.gidx.resize(npw);
+    kp.ekin.resize(npw);
+    kp.fftmap.resize(npw);
+    const int n1 = B.wave[0], n2 = B.wave[1], n3 = B.wave[2];
+    for (int p = 0; p < npw; ++p) {
+        kp.gidx[p] = sel[order[p]];
+        kp.ekin[p] = kin[order[p]];
+        const int* m = B.g[kp.gidx[p]].m;
+        if (std::abs(m[0]) <= n1 / 2 && std::abs(m[1]) <= n2 / 2 && std::abs(m[2]) <= n3 / 2)
+            kp.fftmap[p] = (wrap(m[0], n1) * n2 + wrap(m[1], n2)) * n3 + wrap(m[2], n3);
+        else
+            kp.fftmap[p] = -1;
+    }
+    build_sticks(B, kp);
+}
+
+}  // namespace q3x
+
+using namespace q3x;
+
+// ------------------------------------------------------------------ C ABI ----
+extern "C" {
+
+q3x_status q3x_basis_create(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
+                            double ecutrho_ry, int use_exact_wavefunction_grid,
+                            const int fft_factors[3], q3x_basis* out) {
+    if (!lattice_rows || !out || !(alat_bohr > 0) || !(ecutwfc_ry > 0)) {
+        set_error("q3x_basis_create: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    try {
+        auto* b = new q3x_basis_s;
+        BasisData& B = b->d;
+        B.lat = make_lattice(lattice_rows, alat_bohr);
+        if (!(B.lat.volume > 1e-10)) {
+            delete b;
+            set_error("q3x_basis_create: singular lattice");
+            return Q3X_ERR_ARGUMENT;
+        }
+        B.ecutwfc = ecutwfc_ry;
+        B.ecutrho = ecutrho_ry > 0.0 ? ecutrho_ry : 4.0 * ecutwfc_ry;
+        int one[3] = {1, 1, 1};
+        const int* f = fft_factors ? fft_factors : one;
+        for (int d = 0; d < 3; ++d)
+            if (f[d] < 1 || f[d] > 6) {
+                delete b;
+                set_error("q3x_basis_create: fft factor outside 1..6");
+                return Q3X_ERR_ARGUMENT;
+            }
+        fft_box_for_cutoff(B.lat, B.ecutrho, f, B.rho);
+        // pair.hpp:86-97: wave grid for min(4 or 2 x ecutwfc, ecutrho)
+        double smooth = std::min((use_exact_wavefunction_grid ? 4.0 : 2.0) * B.ecutwfc, B.ecutrho);
+        fft_box_for_cutoff(B.lat, smooth, f, B.wave);
+        build_sphere(B);
+        *out = b;
+        return Q3X_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("q3x_basis_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
+}
+
+q3x_status q3x_basis_info(q3x_basis b, int rho_dims[3], int wave_dims[3], int* ng, double* volume,
+                          double* tpa2) {
+    if (!b) {
+        set_error("q3x_basis_info: null basis");
+        return Q3X_ERR_ARGUMENT;
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (rho_dims) rho_dims[d] = b->d.rho[d];
+        if (wave_dims) wave_dims[d] = b->d.wave[d];
+    }
+    if (ng) *ng = static_cast<int>(b->d.g.size());
+    if (volume) *volume = b->d.lat.volume;
+    if (tpa2) *tpa2 = b->d.lat.tpa2;
+    return Q3X_OK;
+}
+
+q3x_status q3x_basis_gvectors(q3x_basis b, int* miller, double* g_cart, double* g2) {
+    if (!b) {
+        set_error("q3x_basis_gvectors: null basis");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const auto& G = b->d.g;
+    for (size_t i = 0; i < G.size(); ++i) {
+        for (int d = 0; d < 3; ++d) {
+            if (miller) miller[3 * i + d] = G[i].m[d];
+            if (g_cart) g_cart[3 * i + d] = G[i].g[d];
+        }
+        if (g2) g2[i] = G[i].g2;
+    }
+    return Q3X_OK;
+}
+
+q3x_status q3x_basis_destroy(q3x_basis b) {
+    delete b;
+    return Q3X_OK;
+}
+
+q3x_status q3x_kpoint_create(q3x_basis b, const double k_crystal[3], q3x_kpoint* out, int* npw) {
+    if (!b || !k_crystal || !out) {
+        set_error("q3x_kpoint_create: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    try {
+        auto* k = new q3x_kpoint_s;
+        k->basis = b;
+        build_kpoint(b->d, k_crystal, k->d);
+        if (npw) *npw = static_cast<int>(k->d.gidx.size());
+        *out = k;
+        return Q3X_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("q3x_kpoint_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
+}
+
+q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* fft_map) {
+    if (!k) {
+        set_error("q3x_kpoint_maps: null k-point");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const auto& d = k->d;
+    for (size_t i = 0; i < d.gidx.size(); ++i) {
+        if (g_index) g_index[i] = d.gidx[i];
+        if (kinetic_ry) kinetic_ry[i] = d.ekin[i];
+        if (fft_map) fft_map[i] = d.fftmap[i];
+    }
+    return Q3X_OK;
+}
+
+q3x_status q3x_kpoint_destroy(q3x_kpoint k) {
+    delete k;
+    return Q3X_OK;
+}
+
+}  // extern "C"

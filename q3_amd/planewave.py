@@ -1,0 +1,251 @@
+"""Host-side mirror of Q3's plane-wave Hamiltonian and Davidson call surface.
+
+Same names, argument meaning and error behaviour as the reference's interfaces, over the
+C ABI of libq3x.so:
+
+* ``Crystal``                      -- crystal.hpp:156-177 (lattice rows in alat units)
+* ``PlanewaveBasis``               -- basis/planewave/init.hpp:60-271 (generate, compute_gk)
+* ``KPoint``                       -- the per-k list of PlaneWaveSelector::select
+* ``PlanewaveHamiltonianApplicator`` -- hamiltonian_applicator/init.hpp (update_potential,
+  prepare_kpoint, apply_batch; apply before prepare_kpoint raises like serial.hpp:487-492)
+* ``PlanewaveDavidsonDiagonalizer`` -- diagonalizer/planewave/init.hpp:65-176 (set_tolerance,
+  set_maximum_iterations, set_basis_expansion_factor, solve, number_of_unconverged_bands)
+
+Arrays: complex band blocks are numpy ``complex128`` of shape (npw, nvec) in Fortran order
+(column per band, ld = npw), exactly the reference's layout.  Energies in Ry.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+FP64 = 0
+MIXED = 1
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Crystal:
+    """Lattice rows a1, a2, a3 in alat units, alat in bohr; atoms in crystal coordinates."""
+    lattice: np.ndarray
+    alat: float
+    positions: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))
+    species: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+    def cartesian_positions_bohr(self):
+        return (np.asarray(self.positions) @ np.asarray(self.lattice)) * self.alat
+
+    @property
+    def number_of_atoms(self):
+        return len(self.positions)
+
+
+class Context:
+    """One device and one HIP stream (default priority)."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().q3x_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        check(lib().q3x_context_synchronize(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().q3x_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class PlanewaveBasis:
+    def __init__(self, crystal: Crystal, ecutwfc_ry: float, ecutrho_ry: float = 0.0,
+                 use_exact_wavefunction_grid: bool = True, fft_factors=(1, 1, 1)):
+        self.crystal = crystal
+        lat = np.ascontiguousarray(crystal.lattice, dtype=np.float64).ravel()
+        fac = np.ascontiguousarray(fft_factors, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().q3x_basis_create(_ptr(lat), float(crystal.alat), float(ecutwfc_ry), float(ecutrho_ry),
+                                     int(use_exact_wavefunction_grid), _ptr(fac), ctypes.byref(h)))
+        self.h = h
+        rho = np.zeros(3, np.int32)
+        wave = np.zeros(3, np.int32)
+        ng = np.zeros(1, np.int32)
+        vol = np.zeros(1)
+        tpa2 = np.zeros(1)
+        check(lib().q3x_basis_info(h, _ptr(rho), _ptr(wave), _ptr(ng), _ptr(vol), _ptr(tpa2)))
+        self.rho_dims = tuple(int(x) for x in rho)
+        self.wave_dims = tuple(int(x) for x in wave)
+        self.number_of_g_vectors = int(ng[0])
+        self.cell_volume = float(vol[0])
+        self.two_pi_over_alat_squared = float(tpa2[0])
+        self.ecutwfc = ecutwfc_ry
+        self.ecutrho = ecutrho_ry if ecutrho_ry > 0 else 4.0 * ecutwfc_ry
+
+    def g_vectors(self):
+        n = self.number_of_g_vectors
+        m = np.zeros((n, 3), np.int32)
+        g = np.zeros((n, 3))
+        g2 = np.zeros(n)
+        check(lib().q3x_basis_gvectors(self.h, _ptr(m), _ptr(g), _ptr(g2)))
+        return m, g, g2
+
+    def compute_gk(self, k_crystal):
+        return KPoint(self, k_crystal)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().q3x_basis_destroy(self.h)
+            self.h = None
+
+
+class KPoint:
+    def __init__(self, basis: PlanewaveBasis, k_crystal):
+        self.basis = basis
+        self.k = np.ascontiguousarray(k_crystal, dtype=np.float64)
+        h = ctypes.c_void_p()
+        npw = np.zeros(1, np.int32)
+        check(lib().q3x_kpoint_create(basis.h, _ptr(self.k), ctypes.byref(h), _ptr(npw)))
+        self.h = h
+        self.npw = int(npw[0])
+        self.g_vector_indices = np.zeros(self.npw, np.int32)
+        self.kinetic_energy_squared = np.zeros(self.npw)
+        self.fft_map = np.zeros(self.npw, np.int32)
+        check(lib().q3x_kpoint_maps(h, _ptr(self.g_vector_indices), _ptr(self.kinetic_energy_squared),
+                                    _ptr(self.fft_map)))
+
+    def kb_projectors(self, crystal: Crystal, species):
+        """Projector set <k+G|beta> (npw x nproj, Fortran order).  species: list of dicts with
+        r, rab, l (list), cutoff (list), beta_r (nbeta x nmesh)."""
+        pos = np.ascontiguousarray(crystal.positions, dtype=np.float64)
+        asp = np.ascontiguousarray(crystal.species, dtype=np.int32)
+        nmesh = np.ascontiguousarray([len(s["r"]) for s in species], dtype=np.int32)
+        r_all = np.ascontiguousarray(np.concatenate([s["r"] for s in species]), dtype=np.float64)
+        rab_all = np.ascontiguousarray(np.concatenate([s["rab"] for s in species]), dtype=np.float64)
+        nbeta = np.ascontiguousarray([len(s["l"]) for s in species], dtype=np.int32)
+        l_all = np.ascontiguousarray(np.concatenate([s["l"] for s in species]), dtype=np.int32)
+        cut_all = np.ascontiguousarray(np.concatenate([s["cutoff"] for s in species]), dtype=np.int32)
+        br_all = np.ascontiguousarray(np.concatenate([np.asarray(s["beta_r"]).ravel() for s in species]),
+                                      dtype=np.float64)
+        nproj = np.zeros(1, np.int32)
+        args = (self.basis.h, self.h, len(asp), _ptr(pos), _ptr(asp), len(species), _ptr(nmesh), _ptr(r_all),
+                _ptr(rab_all), _ptr(nbeta), _ptr(l_all), _ptr(cut_all), _ptr(br_all))
+        check(lib().q3x_kb_projectors(*args, None, _ptr(nproj)))
+        out = np.zeros((int(nproj[0]), self.npw), np.complex128)
+        check(lib().q3x_kb_projectors(*args, _ptr(out), _ptr(nproj)))
+        return np.asfortranarray(out.T)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().q3x_kpoint_destroy(self.h)
+            self.h = None
+
+
+class PlanewaveHamiltonianApplicator:
+    """H|psi> on the GPU for one prepared k-point (serial.hpp semantics)."""
+
+    def __init__(self, context: Context, basis: PlanewaveBasis):
+        self.context = context
+        self.basis = basis
+        h = ctypes.c_void_p()
+        check(lib().q3x_hamiltonian_create(context.h, basis.h, ctypes.byref(h)))
+        self.h = h
+        self.kpoint = None
+        self.nproj = 0
+
+    def update_potential(self, v_r: np.ndarray):
+        v = np.ascontiguousarray(v_r, dtype=np.float64).ravel()
+        check(lib().q3x_hamiltonian_set_potential(self.h, _ptr(v), v.size))
+
+    def prepare_kpoint(self, kpoint: KPoint):
+        check(lib().q3x_hamiltonian_prepare_kpoint(self.h, kpoint.h))
+        self.kpoint = kpoint
+        self.nproj = 0
+
+    def set_projectors(self, beta: np.ndarray, block_sizes, d_blocks):
+        b = np.asfortranarray(beta, dtype=np.complex128)
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        db = np.ascontiguousarray(d_blocks, dtype=np.float64)
+        nproj = b.shape[1] if b.ndim == 2 else 0
+        check(lib().q3x_hamiltonian_set_projectors(self.h, _ptr(b), nproj, len(bs), _ptr(bs), _ptr(db)))
+        self.nproj = nproj
+
+    def apply_batch(self, psi: np.ndarray, precision: int = FP64) -> np.ndarray:
+        psi = np.asfortranarray(psi, dtype=np.complex128)
+        if psi.ndim == 1:
+            psi = psi[:, None]
+        out = np.zeros_like(psi, order="F")
+        check(lib().q3x_hamiltonian_apply(self.h, _ptr(psi), _ptr(out), psi.shape[1], precision))
+        return out
+
+    def apply_device(self, d_psi: int, d_hpsi: int, nvec: int, precision: int = FP64):
+        """psi / hpsi already resident in HBM (device pointers, complex128, ld = npw)."""
+        check(lib().q3x_hamiltonian_apply_device(self.h, ctypes.c_void_p(d_psi), ctypes.c_void_p(d_hpsi), nvec,
+                                                 precision))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().q3x_hamiltonian_destroy(self.h)
+            self.h = None
+
+
+class PlanewaveDavidsonDiagonalizer:
+    """Block Davidson on the GPU (davidson.hpp:324-841 control flow)."""
+
+    def __init__(self, applicator: PlanewaveHamiltonianApplicator):
+        self.applicator = applicator
+        self.options = _lib.DavidsonOptions()
+        lib().q3x_davidson_default_options(ctypes.byref(self.options))
+        self.report = _lib.DavidsonReport()
+
+    def set_tolerance(self, tol_ry: float):
+        self.options.tolerance_ry = tol_ry
+
+    def set_maximum_iterations(self, n: int):
+        self.options.max_iterations = n
+
+    def set_basis_expansion_factor(self, f: int):
+        self.options.basis_factor = f
+
+    def number_of_unconverged_bands(self) -> int:
+        return self.report.unconverged
+
+    def solve(self, number_of_bands: int, number_of_occupied: int, psi0: np.ndarray | None = None,
+              precision: int = FP64, final_fp64_sweep: bool = True):
+        """Returns (eigenvalues [Ry], psi (npw x nb))."""
+        kp = self.applicator.kpoint
+        if kp is None:
+            raise _lib.StateError(3, "prepare_kpoint() must be called before solve")
+        nb = number_of_bands
+        self.options.n_occupied = number_of_occupied
+        self.options.precision = precision
+        self.options.final_fp64_sweep = int(final_fp64_sweep)
+        if psi0 is None:
+            psi = np.zeros((kp.npw, nb), np.complex128, order="F")
+            self.options.seed_from_psi = 0
+        else:
+            psi = np.asfortranarray(psi0, dtype=np.complex128).copy(order="F")
+            self.options.seed_from_psi = 1
+        eig = np.zeros(nb)
+        self.report = _lib.DavidsonReport()
+        check(lib().q3x_davidson_solve(self.applicator.h, nb, ctypes.byref(self.options), _ptr(psi), _ptr(eig),
+                                       ctypes.byref(self.report)))
+        return eig, psi
+
+    def report_dict(self):
+        r = self.report
+        names = ["seed", "orthogonalize", "hamiltonian_application", "subspace_update", "diagonalize",
+                 "collapse", "residual", "total"]
+        return {"iterations": r.iterations, "unconverged": r.unconverged, "h_applications": r.h_applications,
+                "fp64_refinements": r.fp64_refinements, "max_residual_norm": r.max_residual_norm,
+                "stage_ms": {n: r.stage_ms[i] for i, n in enumerate(names)}}
