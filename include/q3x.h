@@ -115,6 +115,13 @@ q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* h
  * context stream, returns after queueing (call q3x_context_synchronize to wait). */
 q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi,
                                         int nvec, int precision);
+/* complex GEMM on device buffers, stream-ordered (stands in for GPUBLAS::gemm,
+ * src/layer_0_core/math/linear_algebra/gpu/blas/init.hpp, on MFMA):
+ *   C (m x n) = alpha op(A) B + beta C,  transa 'N' (A m x k) or 'C' (A k x m, op = A^H),
+ *   alpha, beta real; Q3X_FP64: complex128 throughout; Q3X_MIXED: complex128 operands rounded to
+ *   complex64 for the products, fp32 accumulation, complex128 C. */
+q3x_status q3x_zgemm(q3x_context ctx, char transa, int m, int n, int k, double alpha, const void* A,
+                     long lda, const void* B, long ldb, double beta, void* C, long ldc, int precision);
 /* device scratch helpers for callers that keep psi resident */
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out);
 q3x_status q3x_device_free(q3x_context ctx, void* p);
@@ -129,6 +136,14 @@ q3x_status q3x_profile_enable(q3x_context ctx, int enable);
 /* synchronizes the stream; fills up to n totals (ms) and launch counts; reset != 0 zeroes them */
 q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* count, int reset);
 const char* q3x_profile_kernel_name(int id);
+
+/* ---- fp64 generalized Hermitian eigensolver on host arrays (the Davidson subspace solve;
+ *      stands in for GPUEigensolver::generalized_hermitian_eigenvalue_solve,
+ *      src/layer_0_core/math/linear_algebra/gpu/lapack/init.hpp, with zhegv itype-1 semantics of
+ *      subspace/serial.hpp:26-136): H x = w S x, upper triangles of H and S (n x n, column-major)
+ *      read; all n eigenvalues ascending; the lowest nvec eigenvectors into X (n x nvec),
+ *      X^H S X = I.  Returns Q3X_OK or Q3X_ERR_NUMERIC (S not positive definite). */
+q3x_status q3x_hegv(int n, const double* H, const double* S, double* w, double* X, int nvec);
 
 /* ---- block Davidson (PlanewaveDavidsonDiagonalizer::solve,
  *      src/layer_4_computation/diagonalizer/planewave/davidson.hpp:324-841, with the subspace

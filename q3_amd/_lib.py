@@ -51,6 +51,7 @@ SIGNATURES = {
     "q3x_hamiltonian_set_projectors": (_i, [_p, _p, _i, _i, _p, _p]),
     "q3x_hamiltonian_apply": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_zgemm": (_i, [_p, ctypes.c_char, _i, _i, _i, _d, _p, _l, _p, _l, _d, _p, _l, _i]),
     "q3x_device_alloc": (_i, [_p, _ul, _pp]),
     "q3x_device_free": (_i, [_p, _p]),
     "q3x_copy_to_device": (_i, [_p, _p, _p, _ul]),
@@ -58,6 +59,7 @@ SIGNATURES = {
     "q3x_profile_enable": (_i, [_p, _i]),
     "q3x_profile_read": (_i, [_p, _i, _p, _p, _i]),
     "q3x_profile_kernel_name": (ctypes.c_char_p, [_i]),
+    "q3x_hegv": (_i, [_i, _p, _p, _p, _p, _i]),
     "q3x_davidson_default_options": (None, [ctypes.POINTER(DavidsonOptions)]),
     "q3x_davidson_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p, ctypes.POINTER(DavidsonReport)]),
 }

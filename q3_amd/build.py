@@ -24,7 +24,9 @@ HIP_FLAGS = [
 # per-file extras: the setup code must not contract a*b+c into FMA (bit-exact G maps)
 EXTRA = {
     "setup.cpp": ["-ffp-contract=off"],
-    "dense.cpp": ["-fopenmp", "-mavx2", "-mfma"],
+    # subspace eigensolver: reassociation lets the reductions vectorise (accuracy is to rounding)
+    "dense.cpp": ["-fopenmp", "-mavx2", "-mfma", "-fassociative-math", "-fno-signed-zeros", "-fno-trapping-math",
+                  "-fno-math-errno"],
     "davidson.cpp": ["-fopenmp"],
 }
 SOURCES = ["setup.cpp", "kb_projectors.cpp", "api.cpp", "hlocal.hip", "zgemm.hip", "aux_kernels.hip",

@@ -314,6 +314,28 @@ const char* q3x_profile_kernel_name(int id) {
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 
+q3x_status q3x_zgemm(q3x_context ctx, char transa, int m, int n, int k, double alpha, const void* A, long lda,
+                     const void* B, long ldb, double beta, void* C, long ldc, int precision) {
+    if (!ctx || m < 0 || n < 0 || k < 0 || (transa != 'N' && transa != 'C') ||
+        (precision != Q3X_FP64 && precision != Q3X_MIXED)) {
+        set_error("q3x_zgemm: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const long rows_a = transa == 'N' ? m : k;
+    if (lda < std::max<long>(1, rows_a) || ldb < std::max(1, k) || ldc < std::max(1, m)) {
+        set_error("q3x_zgemm: leading dimension too small");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const size_t ws = size_t(64) << 20;
+    if (!ctx->gemm_work.ensure(ws)) return Q3X_ERR_DEVICE;
+    const GemmTypes t = precision == Q3X_FP64 ? GemmTypes::D_D_D_D : GemmTypes::D_D_F_D;
+    return hip_ok(zgemm(t, transa == 'C' ? 1 : 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                        ctx->gemm_work.get(), ws, ctx->stream),
+                  "q3x_zgemm")
+               ? Q3X_OK
+               : Q3X_ERR_DEVICE;
+}
+
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {
     if (!ctx || !out) {
         set_error("q3x_device_alloc: invalid argument");

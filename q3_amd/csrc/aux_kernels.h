@@ -11,6 +11,9 @@ hipError_t launch_teter(int npw, int ncol, const double* ekin, const double* eps
 hipError_t launch_column_norm2(int npw, int ncol, const double* X, long ldx, double* out, hipStream_t st);
 hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const double* s, double* Y, long ldy,
                              hipStream_t st);
+// Y = a_c X + beta Y per column (a null -> 1)
+hipError_t launch_axpby(int npw, int ncol, const double* a, const double* X, long ldx, double beta, double* Y,
+                        long ldy, hipStream_t st);
 hipError_t launch_zero(double* p, long ndouble, hipStream_t st);
 hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st);
 

@@ -109,6 +109,25 @@ hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const
     return hipGetLastError();
 }
 
+// Y = a_c X + beta Y column by column (a_c = a[c], or 1 when a is null).  X may alias Y.
+__global__ void axpby_columns(int npw, const double* __restrict__ a, const double* X, long ldx, double beta,
+                              double* Y, long ldy) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= npw) return;
+    const double f = a ? a[c] : 1.0;
+    const long ox = 2 * (g + c * ldx), oy = 2 * (g + c * ldy);
+    Y[oy] = f * X[ox] + beta * Y[oy];
+    Y[oy + 1] = f * X[ox + 1] + beta * Y[oy + 1];
+}
+
+hipError_t launch_axpby(int npw, int ncol, const double* a, const double* X, long ldx, double beta, double* Y,
+                        long ldy, hipStream_t st) {
+    if (npw == 0 || ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(axpby_columns, dim3((npw + 255) / 256, ncol), dim3(256), 0, st, npw, a, X, ldx, beta, Y, ldy);
+    return hipGetLastError();
+}
+
 // Zero-fill of a device range with a kernel (keeps memsets out of captured graphs).
 __global__ void fill_zero(double* __restrict__ p, long n) {
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -1,11 +1,421 @@
-// Block Davidson driver (placeholder until the device implementation lands).
+// Block Davidson with a device-resident basis (PlanewaveDavidsonDiagonalizer::solve,
+// davidson.hpp:324-841, with the subspace manager of subspace_manager/serial.hpp).
+//
+// The basis V, H V, the correction block and the Gram blocks live in HBM; every product over
+// the plane-wave dimension is an MFMA complex GEMM.  What stays on the host is what the
+// reference also decides on the host: the M x M generalized eigenproblem (fp64), the n_c x n_c
+// Cholesky of the Cholesky-QR (fp64), the convergence test on eigenvalue changes and the
+// collapse decision.  Control flow and constants are the reference's:
+//   M_max = max(min(factor*nb, npw), nb)                           davidson.hpp:399-404
+//   iteration 0 never converges; occupied |de| < tol, empty < max(5 tol, 1e-5)  :607-629
+//   corrections r = (HV - e V) c for unconverged bands, Teter preconditioner  serial.hpp:995-1156
+//   projection + Cholesky-QR, modified Gram-Schmidt fallback (norm^2 > 1e-14)  serial.hpp:41-553
+//   collapse to nb Ritz vectors when the basis is full, rebuild H_sub/S_sub   serial.hpp:825-945
+//   zhegv failure -> forced collapse and retry, else previous eigenvalues   davidson.hpp:520-590
+//
+// Mixed precision: the H-apply runs in complex64 (FFT, V multiply, projector GEMMs); the
+// basis, Gram products, Cholesky and Rayleigh-Ritz stay fp64.  With final_fp64_sweep the
+// converged mixed basis is re-applied with the fp64 H and the iteration continues in fp64
+// until the fp64 test passes, so the returned eigenpairs satisfy the fp64 criterion.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "aux_kernels.h"
+#include "dense.h"
 #include "runtime.h"
+#include "zgemm.h"
+
+namespace q3x {
+namespace {
+
+constexpr double kNormThreshold = 1e-14;  // iterative_solver_norm_threshold (tolerances.hpp:39)
+constexpr double kEmptyFactor = 5.0;      // tolerances.hpp:65
+constexpr double kEmptyFloor = 1e-5;      // tolerances.hpp:66
+
+using clk = std::chrono::steady_clock;
+
+struct Solver {
+    q3x_hamiltonian h;
+    hipStream_t st;
+    int npw, nb, nocc, mmax;
+    int precision;
+    DevBuf V, HV, R, C, G, Hs_d, Ss_d, X_d, eps_d, work, norms_d, idx_d;
+    size_t work_bytes = size_t(256) << 20;
+    std::vector<zc> Hs, Ss;  // host M_max x M_max, ld M_max
+    int h_applications = 0;
+    double stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    q3x_status status = Q3X_OK;
+
+    double* Vc(int c) { return V.as<double>() + 2L * c * npw; }
+    double* HVc(int c) { return HV.as<double>() + 2L * c * npw; }
+
+    bool ok(hipError_t e, const char* what) {
+        if (hip_ok(e, what)) return true;
+        status = Q3X_ERR_DEVICE;
+        return false;
+    }
+    bool sync() { return ok(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+    double now_ms() { return std::chrono::duration<double, std::milli>(clk::now().time_since_epoch()).count(); }
+
+    // C (m x n) = alpha * op(A) * B + beta * C, fp64 everywhere
+    bool gemm(int opa, int m, int n, int k, double alpha, const void* A, long lda, const void* B, long ldb,
+              double beta, void* Cm, long ldc) {
+        return ok(zgemm(GemmTypes::D_D_D_D, opa, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, work.get(),
+                        work_bytes, st),
+                  "Davidson GEMM");
+    }
+
+    bool apply(int first, int n, int prec) {
+        if (n <= 0) return true;
+        const double t0 = now_ms();
+        q3x_status s = hamiltonian_apply_device(h, Vc(first), npw, HVc(first), npw, n, prec);
+        if (s != Q3X_OK) {
+            status = s;
+            return false;
+        }
+        h_applications += n;
+        if (!sync()) return false;
+        stage[2] += now_ms() - t0;
+        return true;
+    }
+
+    // H_sub[:total, start:start+nnew] = V^H HV_new ; S_sub likewise (serial.hpp:555-678)
+    bool update(int start, int nnew, int total) {
+        if (nnew <= 0) return true;
+        const double t0 = now_ms();
+        double* hs = Hs_d.as<double>() + 2L * start * mmax;
+        double* ss = Ss_d.as<double>() + 2L * start * mmax;
+        if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, HVc(start), npw, 0.0, hs, mmax)) return false;
+        if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, Vc(start), npw, 0.0, ss, mmax)) return false;
+        const size_t bytes = static_cast<size_t>(nnew) * mmax * 16;
+        if (!ok(hipMemcpyAsync(Hs.data() + static_cast<size_t>(start) * mmax, hs, bytes, hipMemcpyDeviceToHost, st),
+                "D2H H_sub") ||
+            !ok(hipMemcpyAsync(Ss.data() + static_cast<size_t>(start) * mmax, ss, bytes, hipMemcpyDeviceToHost, st),
+                "D2H S_sub") ||
+            !sync())
+            return false;
+        stage[3] += now_ms() - t0;
+        return true;
+    }
+
+    int diag(int n, std::vector<double>& w, std::vector<zc>& X, int nvec) {
+        const double t0 = now_ms();
+        X.assign(static_cast<size_t>(n) * std::max(nvec, 1), zc(0, 0));
+        w.assign(std::max(n, mmax), 0.0);
+        int info = hegv(n, Hs.data(), mmax, Ss.data(), mmax, w.data(), X.data(), n, nvec);
+        stage[4] += now_ms() - t0;
+        return info;
+    }
+
+    // Orthogonalise candidates V[:, start:start+nc] against V[:, :start] and among themselves
+    // (serial.hpp:41-553).  Returns accepted count (compacted), -1 on device error.
+    int orthogonalize(int start, int nc) {
+        if (nc <= 0) return 0;
+        const double t0 = now_ms();
+        double* cand = Vc(start);
+        if (start > 0) {
+            if (!gemm(1, start, nc, npw, 1.0, Vc(0), npw, cand, npw, 0.0, C.get(), start)) return -1;
+            if (!gemm(0, npw, nc, start, -1.0, Vc(0), npw, C.get(), start, 1.0, cand, npw)) return -1;
+        }
+        if (!gemm(1, nc, nc, npw, 1.0, cand, npw, cand, npw, 0.0, G.get(), nc)) return -1;
+        std::vector<zc> Gh(static_cast<size_t>(nc) * nc);
+        if (!ok(hipMemcpyAsync(Gh.data(), G.get(), Gh.size() * 16, hipMemcpyDeviceToHost, st), "D2H Gram") || !sync())
+            return -1;
+        int accepted = nc;
+        if (chol_upper(nc, Gh.data(), nc) == 0) {
+            tri_inv_upper(nc, Gh.data(), nc);  // cand <- cand U^-1 (the ztrsm R,U,N,N of serial.hpp:337-348)
+            if (!ok(hipMemcpyAsync(G.get(), Gh.data(), Gh.size() * 16, hipMemcpyHostToDevice, st), "H2D U^-1"))
+                return -1;
+            if (!gemm(0, npw, nc, nc, 1.0, cand, npw, G.get(), nc, 0.0, R.get(), npw)) return -1;
+            if (!ok(hipMemcpyAsync(cand, R.get(), static_cast<size_t>(npw) * nc * 16, hipMemcpyDeviceToDevice, st),
+                    "D2D orthonormal block"))
+                return -1;
+        } else {
+            // modified Gram-Schmidt fallback (serial.hpp:370-552)
+            accepted = 0;
+            std::vector<double> n2(1);
+            for (int c = 0; c < nc; ++c) {
+                const int src = start + c, dst = start + accepted;
+                if (dst != src &&
+                    !ok(hipMemcpyAsync(Vc(dst), Vc(src), static_cast<size_t>(npw) * 16, hipMemcpyDeviceToDevice, st),
+                        "D2D MGS move"))
+                    return -1;
+                const int nex = dst - start;
+                if (nex > 0) {
+                    if (!gemm(1, nex, 1, npw, 1.0, cand, npw, Vc(dst), npw, 0.0, C.get(), nex)) return -1;
+                    if (!gemm(0, npw, 1, nex, -1.0, cand, npw, C.get(), nex, 1.0, Vc(dst), npw)) return -1;
+                }
+                if (!ok(launch_column_norm2(npw, 1, Vc(dst), npw, norms_d.as<double>(), st), "norm") ||
+                    !ok(hipMemcpyAsync(n2.data(), norms_d.get(), 8, hipMemcpyDeviceToHost, st), "D2H norm") || !sync())
+                    return -1;
+                if (n2[0] > kNormThreshold) {
+                    const double inv = 1.0 / std::sqrt(n2[0]);
+                    if (!ok(hipMemcpyAsync(eps_d.get(), &inv, 8, hipMemcpyHostToDevice, st), "H2D scale") ||
+                        !ok(launch_axpby(npw, 1, eps_d.as<double>(), Vc(dst), npw, 0.0, Vc(dst), npw, st), "scale") ||
+                        !sync())
+                        return -1;
+                    ++accepted;
+                }
+            }
+        }
+        if (!sync()) return -1;
+        stage[1] += now_ms() - t0;
+        return accepted;
+    }
+
+    // V[:, :nb] <- V[:, :n] X[:, :nb], same for HV (serial.hpp:825-945)
+    bool rotate(int n, const std::vector<zc>& X, int ncols) {
+        if (!ok(hipMemcpyAsync(X_d.get(), X.data(), static_cast<size_t>(n) * ncols * 16, hipMemcpyHostToDevice, st),
+                "H2D rotation"))
+            return false;
+        for (double* base : {Vc(0), HVc(0)}) {
+            if (!gemm(0, npw, ncols, n, 1.0, base, npw, X_d.get(), n, 0.0, R.get(), npw)) return false;
+            if (!ok(hipMemcpyAsync(base, R.get(), static_cast<size_t>(npw) * ncols * 16, hipMemcpyDeviceToDevice, st),
+                    "D2D rotated"))
+                return false;
+        }
+        return sync();
+    }
+
+    bool rebuild_subspace(int n) {
+        std::fill(Hs.begin(), Hs.end(), zc(0, 0));
+        std::fill(Ss.begin(), Ss.end(), zc(0, 0));
+        return update(0, n, n);
+    }
+
+    int collapse(int n, bool have_matrices) {
+        const double t0 = now_ms();
+        std::vector<double> w;
+        std::vector<zc> X;
+        int info = -1;
+        if (have_matrices) info = diag(n, w, X, nb);
+        if (info != 0) {
+            if (!update(0, n, n)) return -1;  // build_subspace_matrices_and_diagonalize
+            info = diag(n, w, X, nb);
+            if (info != 0) {
+                status = Q3X_ERR_NUMERIC;
+                set_error("Davidson: subspace eigensolver failed on a rebuilt subspace");
+                return -1;
+            }
+        }
+        if (!rotate(n, X, nb)) return -1;
+        if (!rebuild_subspace(nb)) return -1;
+        stage[5] += now_ms() - t0;
+        return nb;
+    }
+};
+
+}  // namespace
+}  // namespace q3x
+
+using namespace q3x;
+
 extern "C" void q3x_davidson_default_options(q3x_davidson_options* o) {
-    o->tolerance_ry = 1e-2; o->max_iterations = 20; o->basis_factor = 2; o->n_occupied = 0;
-    o->precision = Q3X_FP64; o->seed_from_psi = 0; o->final_fp64_sweep = 1;
+    if (!o) return;
+    o->tolerance_ry = 1e-2;  // initial_eigenvalue_tolerance (tolerances.hpp:224)
+    o->max_iterations = 20;  // davidson.hpp:1156
+    o->basis_factor = 2;     // davidson.hpp:864
+    o->n_occupied = 0;
+    o->precision = Q3X_FP64;
+    o->seed_from_psi = 0;
+    o->final_fp64_sweep = 1;
 }
-extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian, int, const q3x_davidson_options*, double*, double*,
-                                         q3x_davidson_report*) {
-    q3x::set_error("q3x_davidson_solve: not built yet");
-    return Q3X_ERR_INTERNAL;
+
+extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
+                                         double* psi_inout, double* eig_out, q3x_davidson_report* report) {
+    if (!h || !opt || !psi_inout || !eig_out) {
+        set_error("q3x_davidson_solve: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!h->kp || !h->have_potential) {
+        set_error("q3x_davidson_solve: set_potential() and prepare_kpoint() must precede solve");
+        return Q3X_ERR_STATE;
+    }
+    if (nb <= 0 || nb > h->npw || opt->basis_factor < 1 || opt->max_iterations < 0 ||
+        (opt->precision != Q3X_FP64 && opt->precision != Q3X_MIXED)) {
+        set_error("q3x_davidson_solve: need 0 < nb <= npw, basis_factor >= 1, a valid precision");
+        return Q3X_ERR_ARGUMENT;
+    }
+    Solver s;
+    s.h = h;
+    s.st = h->ctx->stream;
+    s.npw = h->npw;
+    s.nb = nb;
+    s.nocc = std::max(0, std::min(opt->n_occupied, nb));
+    s.mmax = std::max(std::min(opt->basis_factor * nb, s.npw), nb);
+    s.precision = opt->precision;
+    const int npw = s.npw, mmax = s.mmax;
+    const double t_start = s.now_ms();
+    const size_t col = static_cast<size_t>(npw) * 16;
+    if (!s.V.ensure(col * mmax) || !s.HV.ensure(col * mmax) || !s.R.ensure(col * std::max(nb, 1)) ||
+        !s.C.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.G.ensure(static_cast<size_t>(mmax) * mmax * 16) ||
+        !s.Hs_d.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.Ss_d.ensure(static_cast<size_t>(mmax) * mmax * 16) ||
+        !s.X_d.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) ||
+        !s.norms_d.ensure(static_cast<size_t>(mmax) * 8) || !s.idx_d.ensure(static_cast<size_t>(nb) * 4) ||
+        !s.work.ensure(s.work_bytes))
+        return Q3X_ERR_DEVICE;
+    s.Hs.assign(static_cast<size_t>(mmax) * mmax, zc(0, 0));
+    s.Ss.assign(static_cast<size_t>(mmax) * mmax, zc(0, 0));
+    int prec = opt->precision;
+
+    // ---- seed (davidson.hpp:1000-1101)
+    double t0 = s.now_ms();
+    int basis = 0;
+    if (opt->seed_from_psi) {
+        if (!s.ok(hipMemcpyAsync(s.Vc(0), psi_inout, col * nb, hipMemcpyHostToDevice, s.st), "H2D psi")) return s.status;
+        const int acc = s.orthogonalize(0, nb);
+        if (acc < 0) return s.status;
+        basis = acc;
+        if (!s.apply(0, basis, prec)) return s.status;
+    }
+    if (basis < nb) {
+        const KPointData& K = h->kp->d;
+        std::vector<std::pair<double, int>> ke(npw);
+        for (int g = 0; g < npw; ++g) ke[g] = {K.ekin[g], g};
+        std::sort(ke.begin(), ke.end());
+        const int start = basis, nc = nb - basis;
+        std::vector<int> idx(nc);
+        for (int c = 0; c < nc; ++c) idx[c] = ke[start + c].second;
+        if (!s.ok(launch_zero(s.Vc(start), 2L * npw * nc, s.st), "zero seed") ||
+            !s.ok(hipMemcpyAsync(s.idx_d.get(), idx.data(), nc * 4, hipMemcpyHostToDevice, s.st), "H2D seed idx") ||
+            !s.ok(launch_unit_columns(s.idx_d.as<int>(), nc, s.Vc(start), npw, s.st), "unit columns"))
+            return s.status;
+        const int acc = s.orthogonalize(start, nc);
+        if (acc < 0) return s.status;
+        basis += acc;
+        if (!s.apply(start, basis - start, prec)) return s.status;
+    }
+    if (!s.update(0, basis, basis)) return s.status;
+    s.stage[0] += s.now_ms() - t0;
+
+    std::vector<double> prev(nb, 0.0), w;
+    for (int b = 0; b < nb && b < basis; ++b) prev[b] = s.Hs[static_cast<size_t>(b) * mmax + b].real();
+    std::vector<zc> X;
+    const double tol = opt->tolerance_ry;
+    const double empty_tol = std::max(kEmptyFactor * tol, kEmptyFloor);
+    int it = 0, unconv = nb, iterations_run = 0, refinements = 0;
+    bool done = false;
+    int max_iter = opt->max_iterations;
+    for (;;) {
+        for (it = 0; it < max_iter; ++it) {
+            ++iterations_run;
+            int info = s.diag(basis, w, X, basis);
+            if (info != 0 && basis > nb) {  // davidson.hpp:520-590
+                basis = s.collapse(basis, false);
+                if (basis < 0) return s.status;
+                info = s.diag(basis, w, X, basis);
+                if (info != 0)
+                    for (int b = 0; b < nb; ++b) w[b] = prev[b];
+            }
+            t0 = s.now_ms();
+            std::vector<int> src;
+            unconv = 0;
+            for (int b = 0; b < nb; ++b) {
+                const double ch = std::abs(w[b] - prev[b]);
+                const double bt = b < s.nocc ? tol : empty_tol;
+                const bool c = (it != 0 && ch < bt);
+                if (!c) {
+                    ++unconv;
+                    if (static_cast<int>(src.size()) < mmax - basis) src.push_back(b);
+                }
+            }
+            if (unconv == 0) {
+                done = true;
+                s.stage[6] += s.now_ms() - t0;
+                break;
+            }
+            // corrections into V[:, basis:basis+nc]: (HV - e V) X_c, preconditioned
+            const int nc = static_cast<int>(src.size());
+            std::vector<zc> Xc(static_cast<size_t>(basis) * nc * 2);
+            std::vector<double> ec(nc);
+            for (int c = 0; c < nc; ++c) {
+                ec[c] = w[src[c]];
+                for (int r = 0; r < basis; ++r) {
+                    const zc x = X[static_cast<size_t>(src[c]) * basis + r];
+                    Xc[static_cast<size_t>(c) * basis + r] = x;                                  // for HV
+                    Xc[static_cast<size_t>(basis) * nc + static_cast<size_t>(c) * basis + r] = -ec[c] * x;  // for V
+                }
+            }
+            if (nc > 0) {
+                double* corr = s.Vc(basis);
+                double* xd = s.X_d.as<double>();
+                if (!s.ok(hipMemcpyAsync(xd, Xc.data(), Xc.size() * 16, hipMemcpyHostToDevice, s.st), "H2D coeffs") ||
+                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps"))
+                    return s.status;
+                if (!s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xd + 2L * basis * nc, basis, 0.0, corr, npw) ||
+                    !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xd, basis, 1.0, corr, npw) ||
+                    !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), s.eps_d.as<double>(), corr, npw, s.st),
+                          "Teter"))
+                    return s.status;
+            }
+            if (!s.sync()) return s.status;
+            s.stage[6] += s.now_ms() - t0;
+            const int accepted = s.orthogonalize(basis, nc);
+            if (accepted < 0) return s.status;
+            if (!s.apply(basis, accepted, prec)) return s.status;
+            const int old = basis;
+            basis += accepted;
+            if (!s.update(old, accepted, basis)) return s.status;
+            if (basis >= mmax) {
+                basis = s.collapse(basis, true);
+                if (basis < 0) return s.status;
+            }
+            for (int b = 0; b < nb; ++b) prev[b] = w[b];
+        }
+        if (!done) {  // davidson.hpp:812-827: final subspace solve after the cap
+            if (s.diag(basis, w, X, basis) != 0) {
+                set_error("Davidson: final subspace eigensolve failed");
+                return Q3X_ERR_NUMERIC;
+            }
+        }
+        if (prec == Q3X_MIXED && opt->final_fp64_sweep && refinements == 0) {
+            // fp64 polish: Ritz vectors of the mixed solve, H re-applied in fp64, iterate on
+            ++refinements;
+            if (!s.rotate(basis, X, nb)) return s.status;
+            basis = nb;
+            prec = Q3X_FP64;
+            if (!s.apply(0, nb, prec)) return s.status;
+            if (!s.rebuild_subspace(nb)) return s.status;
+            for (int b = 0; b < nb; ++b) prev[b] = w[b];
+            done = false;
+            max_iter = std::max(opt->max_iterations, 1);
+            continue;
+        }
+        break;
+    }
+    // extract (serial.hpp:947-993): psi = V X[:, :nb]; then the fp64 residual sweep
+    // r_n = H psi_n - e_n psi_n, reported as max_n ||r_n||
+    if (!s.rotate(basis, X, nb)) return s.status;
+    double maxres = 0.0;
+    {
+        std::vector<double> e(nb), n2(nb);
+        for (int b = 0; b < nb; ++b) e[b] = -w[b];
+        double* res = s.R.as<double>();
+        if (!s.ok(hipMemcpyAsync(s.eps_d.get(), e.data(), nb * 8, hipMemcpyHostToDevice, s.st), "H2D e") ||
+            !s.ok(launch_axpby(npw, nb, s.eps_d.as<double>(), s.Vc(0), npw, 0.0, res, npw, s.st), "r = -e psi") ||
+            !s.ok(launch_axpby(npw, nb, nullptr, s.HVc(0), npw, 1.0, res, npw, s.st), "r += H psi") ||
+            !s.ok(launch_column_norm2(npw, nb, res, npw, s.norms_d.as<double>(), s.st), "norms") ||
+            !s.ok(hipMemcpyAsync(n2.data(), s.norms_d.get(), nb * 8, hipMemcpyDeviceToHost, s.st), "D2H norms") ||
+            !s.sync())
+            return s.status;
+        for (int b = 0; b < nb; ++b) maxres = std::max(maxres, std::sqrt(n2[b]));
+    }
+    if (!s.ok(hipMemcpyAsync(psi_inout, s.Vc(0), col * nb, hipMemcpyDeviceToHost, s.st), "D2H psi") || !s.sync())
+        return s.status;
+    for (int b = 0; b < nb; ++b) eig_out[b] = w[b];
+    s.stage[7] = s.now_ms() - t_start;
+    if (report) {
+        report->iterations = iterations_run;
+        report->unconverged = done ? 0 : unconv;
+        report->h_applications = s.h_applications;
+        report->fp64_refinements = refinements;
+        report->max_residual_norm = maxres;
+        for (int i = 0; i < 8; ++i) report->stage_ms[i] = s.stage[i];
+    }
+    return Q3X_OK;
 }

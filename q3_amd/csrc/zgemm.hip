@@ -26,6 +26,8 @@ struct Mma;
 template <>
 struct Mma<double> {
     typedef double acc_t __attribute__((ext_vector_type(4)));
+    // accumulator register r of lane (li + 16 lk) holds row 4 r + lk (rows interleaved)
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
     static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
@@ -33,6 +35,8 @@ struct Mma<double> {
 template <>
 struct Mma<float> {
     typedef float acc_t __attribute__((ext_vector_type(4)));
+    // accumulator register r of lane (li + 16 lk) holds row 4 lk + r (rows blocked)
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
     static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
@@ -129,7 +133,7 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, int kch
         __syncthreads();
     }
 
-    // ---- epilogue: lane holds D[4*lk + r][li] of each 16x16 tile
+    // ---- epilogue: lane holds D[row(lk, r)][li] of each 16x16 tile
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -139,7 +143,7 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, int kch
             if (gj >= n) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm * 32 + a * 16 + 4 * lk + r;
+                const int gi = m0 + wm * 32 + a * 16 + Mma<TM>::row(lk, r);
                 if (gi >= m) continue;
                 const TM v = acc[a][b][r];
                 if (work) {

@@ -68,6 +68,50 @@ class Context:
         self.close()
 
 
+class DeviceArray:
+    """A device allocation of the context (complex128 band blocks resident in HBM)."""
+
+    def __init__(self, context: Context, nbytes: int):
+        self.context = context
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(lib().q3x_device_alloc(context.h, self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, context: Context, a: np.ndarray):
+        a = np.asfortranarray(a)
+        d = cls(context, a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.asfortranarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().q3x_copy_to_device(self.context.h, ctypes.c_void_p(self.ptr), _ptr(a), a.nbytes))
+
+    def download(self, shape, dtype=np.complex128) -> np.ndarray:
+        out = np.zeros(shape, dtype, order="F")
+        assert out.nbytes <= self.nbytes
+        check(lib().q3x_copy_to_host(self.context.h, _ptr(out), ctypes.c_void_p(self.ptr), out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().q3x_device_free(self.context.h, ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+def zgemm(context: Context, transa: str, m: int, n: int, k: int, alpha: float, A: DeviceArray, lda: int,
+          B: DeviceArray, ldb: int, beta: float, C: DeviceArray, ldc: int, precision: int = FP64):
+    """C = alpha op(A) B + beta C on device buffers (MFMA complex GEMM)."""
+    check(lib().q3x_zgemm(context.h, transa.encode(), m, n, k, alpha, ctypes.c_void_p(A.ptr), lda,
+                          ctypes.c_void_p(B.ptr), ldb, beta, ctypes.c_void_p(C.ptr), ldc, precision))
+
+
 class PlanewaveBasis:
     def __init__(self, crystal: Crystal, ecutwfc_ry: float, ecutrho_ry: float = 0.0,
                  use_exact_wavefunction_grid: bool = True, fft_factors=(1, 1, 1)):

@@ -1,0 +1,282 @@
+"""Pins of the FP64 oracle (CPU): reference known answers, closed forms, brute force and
+scipy.  Each test names what it pins; a plausible mistake in the oracle (dropped term,
+wrong sign, transposed operand, wrong index wrap) fails at least one of them."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special
+
+from oracle import oracle as O
+from q3_amd import workloads as W
+
+FCC = W.FCC
+A_SI = 10.263
+
+
+# ---------------------------------------------------------------- grid sizer ----
+def test_good_fft_order_reference_known_answers():
+    # tests/unit/layer_3_representation/basis/planewave/internal/fft/test_grid_sizer.cpp:83-118
+    for n in (1, 2, 4, 6, 16, 30):
+        assert O.good_fft_order(n) == n
+    for n, want in ((7, 8), (11, 12), (13, 15), (17, 18), (31, 32), (14, 15)):
+        assert O.good_fft_order(n) == want
+    assert O.good_fft_order(7, 2) == 8
+    assert O.good_fft_order(7, 3) == 9
+    assert O.good_fft_order(10, 4) == 12
+
+
+@pytest.mark.parametrize("ecut", [192.0, 96.0])
+def test_supercell_grid_is_twice_primitive(ecut):
+    # test_grid_sizer.cpp:226-245 (factors 2 primitive / 4 supercell as test_grid_sizer.cpp:186-204 finds)
+    prim = O.grid_dims(FCC, A_SI, ecut, (2, 2, 2))
+    sup = O.grid_dims(FCC, 2 * A_SI, ecut, (4, 4, 4))
+    assert sup == tuple(2 * d for d in prim)
+
+
+def test_box_sizes_of_the_workloads():
+    # the survey's brute-force restatement: cfg1 30^3, cfg2 96^3, cfg4 48^3
+    assert O.grid_dims(FCC, 10.26, 120.0, (2, 2, 2)) == (30, 30, 30)
+    assert O.grid_dims(np.eye(3), 20.52, 200.0, (1, 1, 1)) == (96, 96, 96)
+    assert O.grid_dims(np.eye(3), 10.26, 160.0, (4, 4, 4)) == (48, 48, 48)
+
+
+# ---------------------------------------------------------------- G sphere -----
+def _brute_sphere(lat, alat, ecut_ry, dims):
+    recip = np.linalg.inv(np.asarray(lat)).T
+    tpa2 = (2 * math.pi / alat) ** 2
+    h = [d // 2 for d in dims]
+    r = [np.arange(-x, x + 1) for x in h]
+    m = np.array(np.meshgrid(*r, indexing="ij")).reshape(3, -1).T
+    g = m @ recip
+    g2 = np.einsum("ij,ij->i", g, g)
+    keep = g2 <= ecut_ry / tpa2 + 1e-8
+    return m[keep], g2[keep]
+
+
+@pytest.mark.parametrize("lat,alat,ecut,fac", [(FCC, 10.26, 30.0, (2, 2, 2)), (np.eye(3), 10.26, 40.0, (4, 4, 4)),
+                                               (FCC, 7.1, 22.0, (1, 1, 1))])
+def test_sphere_matches_brute_force(lat, alat, ecut, fac):
+    b = O.Basis(lat, alat, ecut, 0.0, True, fac)
+    m, g, g2 = b.gvectors()
+    bm, bg2 = _brute_sphere(lat, alat, 4 * ecut, b.rho_dims)
+    assert len(m) == len(bm)
+    assert {tuple(x) for x in m} == {tuple(x) for x in bm}
+    assert np.all(np.diff(g2) >= 0)                      # sorted by |G|^2
+    assert tuple(m[0]) == (0, 0, 0) and g2[0] == 0.0     # test_g_vectors_grid.cpp:33-41
+    np.testing.assert_allclose(np.sort(g2), np.sort(bg2), rtol=1e-13, atol=1e-12)
+
+
+def test_kpoint_lists_match_brute_force():
+    b = O.Basis(FCC, 10.26, 30.0, 0.0, True, (2, 2, 2))
+    m, g, g2 = b.gvectors()
+    tpa2 = (2 * math.pi / 10.26) ** 2
+    recip = np.linalg.inv(FCC).T
+    for k in ([0, 0, 0], [0.125, -0.375, 0.375], [0.5, 0.5, 0.5]):
+        gi, ek, fm = b.kpoint(k)
+        kc = np.asarray(k) @ recip
+        q2 = np.einsum("ij,ij->i", g + kc, g + kc)
+        want = np.nonzero(q2 <= 30.0 / tpa2 + 1e-8)[0]
+        assert set(gi.tolist()) == set(want.tolist())
+        assert np.all(np.diff(ek) >= 0)
+        np.testing.assert_allclose(ek, q2[gi] * tpa2, rtol=1e-14)
+        if not any(k):
+            assert ek[0] == 0.0                          # test_g_vectors_grid.cpp:43-57
+        else:
+            assert ek[0] > 0.0
+        d = b.wave_dims
+        mm = m[gi]
+        w = np.where(mm < 0, mm + np.array(d), mm)
+        np.testing.assert_array_equal(fm, (w[:, 0] * d[1] + w[:, 1]) * d[2] + w[:, 2])
+
+
+def test_cfg_plane_wave_counts():
+    # survey restatement: cfg1 725 at Gamma, cfg2 51,627 (full sphere Gamma)
+    b1 = O.Basis(FCC, 10.26, 30.0, 0.0, True, (2, 2, 2))
+    assert len(b1.kpoint([0, 0, 0])[0]) == 725
+    b2 = O.Basis(np.eye(3), 20.52, 50.0, 0.0, True, (1, 1, 1))
+    assert len(b2.kpoint([0, 0, 0])[0]) == 51627
+
+
+# ---------------------------------------------------------------- FFT ----------
+@pytest.mark.parametrize("dims", [(30, 30, 30), (12, 20, 30), (8, 9, 25), (48, 6, 5)])
+def test_fft_matches_numpy_fftw_convention(dims):
+    rng = np.random.default_rng(sum(dims))
+    a = rng.standard_normal(dims) + 1j * rng.standard_normal(dims)
+    fwd = O.fft3d(a, -1)
+    np.testing.assert_allclose(fwd, np.fft.fftn(a), rtol=0, atol=1e-11 * np.abs(fwd).max())
+    bwd = O.fft3d(a, +1)
+    ref = np.fft.ifftn(a) * a.size
+    np.testing.assert_allclose(bwd, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+
+
+def test_fft_round_trip_and_parseval():
+    # test_fft_grid.cpp:13-44 (round trip) and :56-83 (Parseval)
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal((16, 16, 16)) + 0j
+    f = O.fft3d(a, -1)
+    back = O.fft3d(f, +1) / a.size
+    np.testing.assert_allclose(back, a, atol=1e-12)
+    assert abs(np.sum(np.abs(a) ** 2) - np.sum(np.abs(f) ** 2) / a.size) <= 1e-10 * np.sum(np.abs(a) ** 2)
+
+
+# ---------------------------------------------------------------- radial math --
+def test_spherical_bessel_against_scipy():
+    for l in range(4):
+        for x in (0.01, 0.3, 0.99, 1.0, 2.5, 4.0, 7.3, 15.0, 40.0):
+            want = scipy.special.spherical_jn(l, x)
+            assert abs(O.sph_bessel(l, x) - want) <= 1e-13 * max(1.0, abs(want)) + 1e-15
+        # below bessel_argument_zero_threshold (tolerances.hpp:158) the reference returns the x=0 limit
+        for x in (0.0, 1e-9):
+            assert O.sph_bessel(l, x) == (1.0 if l == 0 else 0.0)
+
+
+def _ylm_closed(l, m, v):
+    # harmonics.hpp:85-141 closed forms (Condon-Shortley phase kept)
+    x, y, z = v / np.linalg.norm(v)
+    fp = 4 * math.pi
+    if l == 0:
+        return 1 / math.sqrt(fp)
+    if l == 1:
+        c = math.sqrt(3 / fp)
+        return {-1: -c * y, 0: c * z, 1: -c * x}[m]
+    if l == 2:
+        c = math.sqrt(15 / fp)
+        return {-2: c * x * y, -1: -c * y * z, 0: math.sqrt(5 / fp) / 2 * (3 * z * z - 1), 1: -c * x * z,
+                2: c / 2 * (x * x - y * y)}[m]
+    if l == 3:
+        p = math.pi
+        return {-3: -math.sqrt(35 / (32 * p)) * y * (3 * x * x - y * y),
+                -2: math.sqrt(105 / (4 * p)) * x * y * z,
+                -1: -math.sqrt(21 / (32 * p)) * y * (5 * z * z - 1),
+                0: math.sqrt(7 / (16 * p)) * z * (5 * z * z - 3),
+                1: -math.sqrt(21 / (32 * p)) * x * (5 * z * z - 1),
+                2: math.sqrt(105 / (16 * p)) * z * (x * x - y * y),
+                3: -math.sqrt(35 / (32 * p)) * x * (x * x - 3 * y * y)}[m]
+
+
+def test_real_ylm_against_closed_forms_and_order():
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal((20, 3))
+    out = O.ylm(3, g)
+    for i in range(len(g)):
+        for l in range(4):
+            for m in range(-l, l + 1):
+                slot = l * l if m == 0 else (l * l + 2 * m - 1 if m > 0 else l * l - 2 * m)  # harmonics.hpp:50-58
+                assert abs(out[i, slot] - _ylm_closed(l, m, g[i])) <= 1e-13
+
+
+def test_beta_table_gaussian_closed_form():
+    """beta(r) = r^l exp(-r^2/2s^2): 4 pi int r^2 beta j_l(qr) dr = 4 pi sqrt(pi/2) s^(2l+3) q^l exp(-q^2 s^2/2)."""
+    sp = W.synthetic_species(0)
+    for ib, (l, s) in enumerate(((0, W.SIGMA_S), (1, W.SIGMA_P))):
+        t = O.beta_table(sp["r"], sp["rab"], l, sp["cutoff"][ib], sp["beta_r"][ib], 200.0)
+        q = 0.01 * np.arange(len(t))
+        want = 4 * math.pi * math.sqrt(math.pi / 2) * s ** (2 * l + 3) * q ** l * np.exp(-q * q * s * s / 2)
+        assert np.abs(t - want).max() <= 1e-9 * np.abs(want).max()
+
+
+# ---------------------------------------------------------------- dense eigen --
+@pytest.mark.parametrize("n", [1, 2, 5, 17, 40])
+def test_zhegv_against_scipy(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    H = A + A.conj().T
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    S = B @ B.conj().T + n * np.eye(n)
+    info, w, X = O.zhegv(H, S)
+    assert info == 0
+    np.testing.assert_allclose(w, scipy.linalg.eigh(H, S, eigvals_only=True), rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(X.conj().T @ S @ X, np.eye(n), atol=1e-11)
+    np.testing.assert_allclose(H @ X, S @ X * w, atol=1e-10 * max(1, np.abs(w).max()))
+
+
+def test_zhegv_degenerate_and_failure():
+    # 2x2 known answer (subspace_manager/test_serial.cpp:39-82): eigenvalues 1 and 3
+    info, w, X = O.zhegv(np.array([[2, 1], [1, 2]], complex), np.eye(2, dtype=complex))
+    assert info == 0 and np.allclose(w, [1, 3], atol=1e-14)
+    # degenerate spectrum: orthonormal eigenvectors still
+    info, w, X = O.zhegv(np.diag([1.0, 1.0, 1.0, 2.0]).astype(complex), np.eye(4, dtype=complex))
+    assert info == 0 and np.allclose(X.conj().T @ X, np.eye(4), atol=1e-13)
+    # S not positive definite -> LAPACK-style info = n + j
+    info, _, _ = O.zhegv(np.eye(2, dtype=complex), np.array([[1, 0], [0, -1]], complex))
+    assert info == 2 + 2
+
+
+# ---------------------------------------------------------------- H|psi> -------
+def _small_case(ecut_ry=10.0, k=(0.125, 0.25, -0.375)):
+    from q3_amd.planewave import Crystal
+    cry = Crystal(FCC.copy(), 10.26, np.array([[0, 0, 0], [0.25, 0.25, 0.25]]), np.zeros(2, np.int32))
+    b = O.Basis(cry.lattice, cry.alat, ecut_ry, 0.0, True, (2, 2, 2))
+    gi, ek, fm = b.kpoint(k)
+    m, g, g2 = b.gvectors()
+    sp = [W.synthetic_species(0)]
+    beta = b.kb_projectors(k, gi, cry.positions, cry.species, sp)
+    bs, db = W.d_blocks_for(cry, sp)
+
+    class _B:  # minimal basis view for workloads.local_potential
+        def g_vectors(self):
+            return m, g, g2
+        wave_dims = b.wave_dims
+        cell_volume = abs(np.linalg.det(cry.lattice)) * cry.alat ** 3
+    V = W.local_potential(cry, _B(), sp)
+    return b, gi, ek, fm, m, V, beta, bs, db
+
+
+def _dense_h(b, gi, ek, m, V, beta, bs, db):
+    """Brute-force H: T + V(G-G') + beta D beta^H with V(dG) = fftn(V)[dG]/N (the FFTW pair)."""
+    d = np.array(b.wave_dims)
+    Vg = np.fft.fftn(V) / V.size
+    mm = m[gi]
+    dm = mm[:, None, :] - mm[None, :, :]
+    w = np.mod(dm, d)
+    H = Vg[w[..., 0], w[..., 1], w[..., 2]].astype(complex)
+    H += np.diag(ek)
+    D = scipy.linalg.block_diag(*[db[o:o + s * s].reshape(s, s, order="F")
+                                  for o, s in zip(np.cumsum([0] + [x * x for x in bs[:-1]]), bs)])
+    H += beta @ D @ beta.conj().T
+    return H
+
+
+def test_hamiltonian_apply_matches_dense_brute_force():
+    b, gi, ek, fm, m, V, beta, bs, db = _small_case()
+    H = _dense_h(b, gi, ek, m, V, beta, bs, db)
+    assert np.abs(H - H.conj().T).max() < 1e-12 * np.abs(H).max()
+    rng = np.random.default_rng(1)
+    psi = np.asfortranarray(rng.standard_normal((len(gi), 3)) + 1j * rng.standard_normal((len(gi), 3)))
+    got = O.hamiltonian_apply(b.wave_dims, fm, ek, V, beta, bs, db, psi)
+    want = H @ psi
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_hamiltonian_apply_constant_potential():
+    b, gi, ek, fm, m, V, beta, bs, db = _small_case()
+    rng = np.random.default_rng(2)
+    psi = np.asfortranarray(rng.standard_normal((len(gi), 2)) + 0j)
+    got = O.hamiltonian_apply(b.wave_dims, fm, ek, np.full(V.shape, 0.25), None, None, None, psi)
+    np.testing.assert_allclose(got, (ek + 0.25)[:, None] * psi, atol=1e-13)
+
+
+def test_davidson_matches_dense_eigh():
+    """Davidson vs dense diagonalisation; reference bounds (test_scf_integration.cpp:504-516)
+    are occupied <= 1e-6 Ry, empty <= 1e-4 Ry; the oracle is held to 2e-8 Ry (1e-8 Ha) occupied."""
+    b, gi, ek, fm, m, V, beta, bs, db = _small_case(12.0)
+    H = _dense_h(b, gi, ek, m, V, beta, bs, db)
+    exact = scipy.linalg.eigh(H, eigvals_only=True)
+    nb, nocc = 8, 4
+    eig, psi, rep = O.davidson(b.wave_dims, fm, ek, V, beta, bs, db, nb, nocc, None, tol=1e-10, max_iter=200)
+    assert rep["unconverged"] == 0
+    assert np.abs(eig[:nocc] - exact[:nocc]).max() <= 2e-8
+    assert np.abs(eig - exact[:nb]).max() <= 1e-4
+    # Ritz vectors orthonormal and residuals small for occupied bands
+    np.testing.assert_allclose(psi.conj().T @ psi, np.eye(nb), atol=1e-10)
+    R = H @ psi[:, :nocc] - psi[:, :nocc] * eig[:nocc]
+    assert np.linalg.norm(R, axis=0).max() < 1e-3
+
+
+def test_davidson_capped_leaves_bands_unconverged():
+    # DavidsonCountsTheBandsACappedSolveLeavesUnconverged (test_scf_integration.cpp:423-516)
+    b, gi, ek, fm, m, V, beta, bs, db = _small_case(12.0)
+    eig, psi, rep = O.davidson(b.wave_dims, fm, ek, V, beta, bs, db, 8, 4, None, tol=1e-13, max_iter=1)
+    assert 0 < rep["unconverged"] <= 8
