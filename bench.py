@@ -39,7 +39,35 @@ def parse():
     p.add_argument("--no-davidson", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--davidson-iters", type=int, default=60)
+    p.add_argument("--mode", choices=["full", "hpsi", "copy"], default="full",
+                   help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
+                        "copy: a plain 1 GiB device copy, the reference for HBM byte counters")
+    p.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     return p.parse_args()
+
+
+def copy_reference(steps):
+    """Plain streaming copy of 1 GiB, steps times (HBM byte-counter calibration)."""
+    from q3_amd import Context
+    from q3_amd._lib import check, lib
+    ctx = Context(0)
+    L = lib()
+    nbytes = 1 << 30
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    check(L.q3x_device_alloc(ctx.h, nbytes, ctypes.byref(a)))
+    check(L.q3x_device_alloc(ctx.h, nbytes, ctypes.byref(b)))
+    check(L.q3x_device_copy(ctx.h, b, a, nbytes))
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        check(L.q3x_device_copy(ctx.h, b, a, nbytes))
+    ctx.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    print(json.dumps({"mode": "copy", "bytes_read": nbytes, "bytes_written": nbytes, "ms_per_copy": dt * 1e3,
+                      "GB/s": 2 * nbytes / dt / 1e9}))
+    L.q3x_device_free(ctx.h, a)
+    L.q3x_device_free(ctx.h, b)
+    return 0
 
 
 def kernel_bytes(name, nvec, npw, nst, n3, npts, cs, sv):
@@ -59,6 +87,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
+    if args.mode == "copy":
+        return copy_reference(args.steps)
 
     from q3_amd import FP64, MIXED, Context, PlanewaveDavidsonDiagonalizer, PlanewaveHamiltonianApplicator
     from q3_amd import workloads as W
@@ -91,7 +121,10 @@ def main():
 
     nst = None
     results = {}
-    for prec, label in ((FP64, "fp64"), (MIXED, "mixed")):
+    precs = [(FP64, "fp64"), (MIXED, "mixed")]
+    if args.precision != "both":
+        precs = [p for p in precs if p[1] == args.precision]
+    for prec, label in precs:
         for _ in range(args.warmup):
             h.apply_device(d_psi.value, d_hpsi.value, nb, prec)
         ctx.synchronize()
@@ -114,6 +147,10 @@ def main():
                 kern[name] = {"avg_ms": tot[i] / cnt[i], "launches": int(cnt[i])}
         results[label] = {"ms_per_step": 1e3 * dt / args.steps, "hpsi_per_s": nb * args.steps / dt, "kernels": kern}
         log(f"{label}: {1e3 * dt / args.steps:.3f} ms/step, {nb * args.steps / dt:.0f} Hpsi/s, kernels {kern}")
+    if args.mode == "hpsi":
+        print(json.dumps({"mode": "hpsi", **{k: {"hpsi_per_s": v["hpsi_per_s"], "ms_per_step": v["ms_per_step"],
+                                                 "kernels": v["kernels"]} for k, v in results.items()}}))
+        return 0
 
     # sticks for the byte model
     from q3_amd._lib import lib as _l  # noqa: F401

@@ -125,6 +125,9 @@ q3x_status q3x_zgemm(q3x_context ctx, char transa, int m, int n, int k, double a
 /* device scratch helpers for callers that keep psi resident */
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out);
 q3x_status q3x_device_free(q3x_context ctx, void* p);
+/* stream-ordered device-to-device copy by a plain streaming kernel (bytes multiple of 16);
+ * also the reference kernel for calibrating HBM byte counters */
+q3x_status q3x_device_copy(q3x_context ctx, void* dst, const void* src, unsigned long bytes);
 q3x_status q3x_copy_to_device(q3x_context ctx, void* dst, const void* src, unsigned long bytes);
 q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigned long bytes);
 

@@ -54,6 +54,7 @@ SIGNATURES = {
     "q3x_zgemm": (_i, [_p, ctypes.c_char, _i, _i, _i, _d, _p, _l, _p, _l, _d, _p, _l, _i]),
     "q3x_device_alloc": (_i, [_p, _ul, _pp]),
     "q3x_device_free": (_i, [_p, _p]),
+    "q3x_device_copy": (_i, [_p, _p, _p, _ul]),
     "q3x_copy_to_device": (_i, [_p, _p, _p, _ul]),
     "q3x_copy_to_host": (_i, [_p, _p, _p, _ul]),
     "q3x_profile_enable": (_i, [_p, _i]),

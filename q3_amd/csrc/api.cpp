@@ -82,15 +82,33 @@ Profiler::~Profiler() {
 
 namespace {
 
+// Radix plan for one FFT length.  Every transform is run by one wave64, so each stage must
+// have n / R <= 64 butterflies per transform.  Search all orderings of supported radices for
+// the fewest stages; ties go to the earlier radices of `pref` (large butterflies balanced for
+// 64-lane waves: 96 = 12 * 8, 48 = 12 * 4, 144 = 12 * 12).
+bool plan_search(int n, int rem, int depth, int maxdepth, int* cur, FftPlan& best) {
+    static const int pref[] = {12, 8, 16, 10, 15, 9, 6, 5, 4, 3, 2};
+    if (rem == 1) {
+        best.nstage = depth;
+        for (int i = 0; i < depth; ++i) best.radix[i] = cur[i];
+        return true;
+    }
+    if (depth == maxdepth) return false;
+    for (int q : pref)
+        if (rem % q == 0 && n / q <= 64) {
+            cur[depth] = q;
+            if (plan_search(n, rem / q, depth + 1, maxdepth, cur, best)) return true;
+        }
+    return false;
+}
+
 FftPlan make_plan(int n) {
     FftPlan p{};
     p.n = n;
-    int r = n, s = 0;
-    while (r % 4 == 0 && s < 8) { p.radix[s++] = 4; r /= 4; }
-    while (r % 2 == 0 && s < 8) { p.radix[s++] = 2; r /= 2; }
-    while (r % 3 == 0 && s < 8) { p.radix[s++] = 3; r /= 3; }
-    while (r % 5 == 0 && s < 8) { p.radix[s++] = 5; r /= 5; }
-    p.nstage = (r == 1) ? s : -1;
+    p.nstage = -1;
+    int cur[8];
+    for (int maxd = 1; maxd <= 8 && n >= 2 && n <= 1024; ++maxd)
+        if (plan_search(n, n, 0, maxd, cur, p)) break;
     return p;
 }
 
@@ -344,6 +362,15 @@ q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {
     return hip_ok(hipMalloc(out, bytes ? bytes : 16), "hipMalloc") ? Q3X_OK : Q3X_ERR_DEVICE;
 }
 
+q3x_status q3x_device_copy(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
+    if (!ctx || (bytes % 16) != 0) {
+        set_error("q3x_device_copy: bytes must be a multiple of 16");
+        return Q3X_ERR_ARGUMENT;
+    }
+    return hip_ok(launch_copy16(src, dst, static_cast<long>(bytes / 16), ctx->stream), "copy kernel") ? Q3X_OK
+                                                                                                      : Q3X_ERR_DEVICE;
+}
+
 q3x_status q3x_device_free(q3x_context ctx, void* p) {
     (void)ctx;
     return hip_ok(hipFree(p), "hipFree") ? Q3X_OK : Q3X_ERR_DEVICE;
@@ -430,14 +457,14 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
         if (K.fftmap[p] < 0) out.push_back(p);
     h->n_outside = static_cast<int>(out.size());
     const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
-    h->sticks_per_block = std::max(1, std::min(64, 1536 / n3));
-    // plane kernel LDS: twiddles + [n2][ncol] plane + rows_per_pass rows of n1
+    h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
+    // plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of 8 waves
     for (int fp32 = 0; fp32 < 2; ++fp32) {
         const size_t cs = fp32 ? 8 : 16;
         const size_t fixed = cs * (n1 + n2 + static_cast<size_t>(n2) * S.ncol);
         long rp = 0;
-        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * n1));
-        rp = std::min<long>(rp, 16);
+        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * n1 * 8));
+        rp = std::min<long>(rp, 4);
         rp = std::min<long>(rp, n2);
         if (rp < 1) {
             set_error("prepare_kpoint: the occupied plane does not fit in LDS for this box");

@@ -128,6 +128,20 @@ hipError_t launch_axpby(int npw, int ncol, const double* a, const double* X, lon
     return hipGetLastError();
 }
 
+// Plain streaming copy (16-byte vectors): the reference point for HBM byte counters.
+__global__ void copy16(const double2* __restrict__ src, double2* __restrict__ dst, long n) {
+    for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<long>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st) {
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(copy16, dim3(4096), dim3(256), 0, st, static_cast<const double2*>(src),
+                       static_cast<double2*>(dst), n16);
+    return hipGetLastError();
+}
+
 // Zero-fill of a device range with a kernel (keeps memsets out of captured graphs).
 __global__ void fill_zero(double* __restrict__ p, long n) {
     const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -14,6 +14,11 @@
 // gather) restated sphere-aware: only the S occupied z-sticks and the occupied i1 columns
 // of each plane are transformed, and the full box is never written to HBM.
 //
+// Work split: HBM loads and stores are done by the whole workgroup (coalesced), the 1D
+// transforms by single wave64s that own whole columns / rows / sticks, so a plane needs
+// four workgroup barriers; the stages inside a transform only order the wave's own LDS
+// accesses.
+//
 // T = double (FP64) or float (mixed precision: psi and hpsi stay complex128 in HBM, the box
 // and transforms are complex64).
 #include <hip/hip_runtime.h>
@@ -28,6 +33,21 @@ __device__ __forceinline__ void load_twiddles(cx<T>* dst, const cx<T>* src, int 
     for (int i = tid; i < n; i += nthr) dst[i] = src[i];
 }
 
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Sticks of this block are split over its waves in contiguous runs.
+template <typename T>
+__device__ __forceinline__ void wave_stick_fft(cx<T>* buf, const FftPlan& p, const cx<T>* tw, int sign, int nsb,
+                                               int wave, int nw, int lane) {
+    const int per = (nsb + nw - 1) / nw;
+    const int first = wave * per;
+    const int cnt = min(per, nsb - first);
+    if (cnt > 0) fft_lds<T, 1, true>(buf + first * p.n, p, tw, sign, cnt, 1, p.n, lane, 64);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) zsticks_inverse(HLocalArgs a, const double* __restrict__ psi, long ldpsi,
                                                        cx<T>* __restrict__ zbuf) {
@@ -35,6 +55,7 @@ __global__ void __launch_bounds__(256) zsticks_inverse(HLocalArgs a, const doubl
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* buf = tw + a.p3.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
     const int n3 = a.p3.n;
     const int s0 = blockIdx.x * a.sticks_per_block;
     const int nsb = min(a.sticks_per_block, a.nst - s0);
@@ -50,7 +71,8 @@ __global__ void __launch_bounds__(256) zsticks_inverse(HLocalArgs a, const doubl
         buf[a.ent_pos[j] - base] = cx<T>{T(src[2 * p]), T(src[2 * p + 1])};
     }
     __syncthreads();
-    fft_lds<T, 2>(buf, a.p3, tw, +1, nsb, 1, n3, tid, nthr);
+    wave_stick_fft(buf, a.p3, tw, +1, nsb, wave, nw, lane);
+    __syncthreads();
     cx<T>* out = zbuf + static_cast<long>(band) * n3 * a.nst;
     for (int i = tid; i < nsb * n3; i += nthr) {
         const int z = i / nsb, sl = i - z * nsb;
@@ -64,47 +86,57 @@ __global__ void __launch_bounds__(512) plane_sandwich(HLocalArgs a, const T* __r
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int n1 = a.p1.n, n2 = a.p2.n, n3 = a.p3.n;
     const int ncol = a.ncol;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
+    const int rbw = a.rows_per_pass;  // rows per wave per pass
     cx<T>* tw1 = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* tw2 = tw1 + n1;
-    cx<T>* P = tw2 + n2;           // [n2][ncol]
-    cx<T>* rows = P + n2 * ncol;   // [rows_per_pass][n1]
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    cx<T>* P = tw2 + n2;                             // [ncol][n2], columns contiguous
+    cx<T>* rows = P + n2 * ncol + wave * rbw * n1;   // this wave's [rbw][n1]
     const int z = blockIdx.x, band = blockIdx.y;
     load_twiddles(tw1, reinterpret_cast<const cx<T>*>(a.tw1), n1, tid, nthr);
     load_twiddles(tw2, reinterpret_cast<const cx<T>*>(a.tw2), n2, tid, nthr);
     for (int i = tid; i < n2 * ncol; i += nthr) P[i] = cx<T>{T(0), T(0)};
     __syncthreads();
     cx<T>* zp = zbuf + (static_cast<long>(band) * n3 + z) * a.nst;
-    for (int s = tid; s < a.nst; s += nthr) P[a.stick_i2[s] * ncol + a.stick_col[s]] = zp[s];
+    for (int s = tid; s < a.nst; s += nthr) P[a.stick_col[s] * n2 + a.stick_i2[s]] = zp[s];
     __syncthreads();
-    // backward along i2 for every occupied column
-    fft_lds<T, 4>(P, a.p2, tw2, +1, ncol, ncol, 1, tid, nthr);
+    // backward along i2: wave w owns columns w, w + nw, ...
+    const int my_cols = wave < ncol ? (ncol - 1 - wave) / nw + 1 : 0;
+    if (my_cols > 0) fft_lds<T, 1, true>(P + wave * n2, a.p2, tw2, +1, my_cols, 1, nw * n2, lane, 64);
+    __syncthreads();
+    // rows: wave w owns rows w, w + nw, ... in passes of rbw rows
     const T* vz = vplane + static_cast<long>(z) * n2 * n1;
-    const int rp = a.rows_per_pass;
-    for (int r0 = 0; r0 < n2; r0 += rp) {
-        const int nr = min(rp, n2 - r0);
-        for (int i = tid; i < nr * n1; i += nthr) rows[i] = cx<T>{T(0), T(0)};
-        __syncthreads();
-        for (int i = tid; i < nr * ncol; i += nthr) {
+    const int my_rows = wave < n2 ? (n2 - 1 - wave) / nw + 1 : 0;
+    for (int k0 = 0; k0 < my_rows; k0 += rbw) {
+        const int nr = min(rbw, my_rows - k0);
+        for (int i = lane; i < nr * n1; i += 64) rows[i] = cx<T>{T(0), T(0)};
+        wave_sync();
+        for (int i = lane; i < nr * ncol; i += 64) {
             const int rr = i / ncol, c = i - rr * ncol;
-            rows[rr * n1 + a.col_i1[c]] = P[(r0 + rr) * ncol + c];
+            const int i2 = wave + nw * (k0 + rr);
+            rows[rr * n1 + a.col_i1[c]] = P[c * n2 + i2];
         }
-        __syncthreads();
-        fft_lds<T, 4>(rows, a.p1, tw1, +1, nr, 1, n1, tid, nthr);
-        for (int i = tid; i < nr * n1; i += nthr) {
+        wave_sync();
+        fft_lds<T, 1, true>(rows, a.p1, tw1, +1, nr, 1, n1, lane, 64);
+        for (int i = lane; i < nr * n1; i += 64) {
             const int rr = i / n1, x = i - rr * n1;
-            rows[i] = cscale(rows[i], vz[(r0 + rr) * n1 + x]);
+            const int i2 = wave + nw * (k0 + rr);
+            rows[i] = cscale(rows[i], vz[i2 * n1 + x]);
         }
-        __syncthreads();
-        fft_lds<T, 4>(rows, a.p1, tw1, -1, nr, 1, n1, tid, nthr);
-        for (int i = tid; i < nr * ncol; i += nthr) {
+        wave_sync();
+        fft_lds<T, 1, true>(rows, a.p1, tw1, -1, nr, 1, n1, lane, 64);
+        for (int i = lane; i < nr * ncol; i += 64) {
             const int rr = i / ncol, c = i - rr * ncol;
-            P[(r0 + rr) * ncol + c] = rows[rr * n1 + a.col_i1[c]];
+            const int i2 = wave + nw * (k0 + rr);
+            P[c * n2 + i2] = rows[rr * n1 + a.col_i1[c]];
         }
-        __syncthreads();
+        wave_sync();
     }
-    fft_lds<T, 4>(P, a.p2, tw2, -1, ncol, ncol, 1, tid, nthr);
-    for (int s = tid; s < a.nst; s += nthr) zp[s] = P[a.stick_i2[s] * ncol + a.stick_col[s]];
+    __syncthreads();
+    if (my_cols > 0) fft_lds<T, 1, true>(P + wave * n2, a.p2, tw2, -1, my_cols, 1, nw * n2, lane, 64);
+    __syncthreads();
+    for (int s = tid; s < a.nst; s += nthr) zp[s] = P[a.stick_col[s] * n2 + a.stick_i2[s]];
 }
 
 template <typename T>
@@ -115,6 +147,7 @@ __global__ void __launch_bounds__(256) zsticks_forward(HLocalArgs a, const cx<T>
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* buf = tw + a.p3.n;
     const int tid = threadIdx.x, nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nthr >> 6;
     const int n3 = a.p3.n;
     const int s0 = blockIdx.x * a.sticks_per_block;
     const int nsb = min(a.sticks_per_block, a.nst - s0);
@@ -126,7 +159,8 @@ __global__ void __launch_bounds__(256) zsticks_forward(HLocalArgs a, const cx<T>
         buf[sl * n3 + z] = in[static_cast<long>(z) * a.nst + s0 + sl];
     }
     __syncthreads();
-    fft_lds<T, 2>(buf, a.p3, tw, -1, nsb, 1, n3, tid, nthr);
+    wave_stick_fft(buf, a.p3, tw, -1, nsb, wave, nw, lane);
+    __syncthreads();
     const double* src = psi + band * ldpsi * 2;
     double* dst = hpsi + band * ldh * 2;
     const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
@@ -152,27 +186,41 @@ __global__ void kinetic_only(const int* __restrict__ idx, int n, const double* _
     hpsi[2 * (b * ldh + p) + 1] = ekin[p] * psi[2 * (b * ldpsi + p) + 1];
 }
 
+size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which) {
+    const size_t cs = fp32 ? 8 : 16;
+    if (which == 0) return cs * (a.p3.n + static_cast<size_t>(a.sticks_per_block) * a.p3.n);
+    // plane kernel: twiddles, plane, one row buffer per wave (8 waves)
+    return cs * (a.p1.n + a.p2.n + static_cast<size_t>(a.p2.n) * a.ncol +
+                 static_cast<size_t>(8) * a.rows_per_pass * a.p1.n);
+}
+
 template <typename T>
 hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, const double* psi, long ldpsi,
                            double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks) {
     if (marks) (void)hipEventRecord(marks[0], st);
-    const size_t cs = sizeof(cx<T>);
+    const bool fp32 = sizeof(T) == 4;
     const int nsb_blocks = (a.nst + a.sticks_per_block - 1) / a.sticks_per_block;
     dim3 g1(nsb_blocks, nvec);
-    const size_t lds_z = cs * (a.p3.n + static_cast<size_t>(a.sticks_per_block) * a.p3.n);
-    hipLaunchKernelGGL(zsticks_inverse<T>, g1, dim3(256), lds_z, st, a, psi, ldpsi, static_cast<cx<T>*>(zbuf));
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (marks) (void)hipEventRecord(marks[1], st);
-    const size_t lds_p = cs * (a.p1.n + a.p2.n + static_cast<size_t>(a.p2.n) * a.ncol +
-                               static_cast<size_t>(a.rows_per_pass) * a.p1.n);
+    const size_t lds_z = hlocal_lds_bytes(a, fp32, 0);
+    const size_t lds_p = hlocal_lds_bytes(a, fp32, 1);
     static bool lds_raised = false;
-    if (!lds_raised && lds_p > 65536) {
+    hipError_t e;
+    if (!lds_raised) {
         e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plane_sandwich<T>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_p));
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zsticks_inverse<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zsticks_forward<T>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
+    hipLaunchKernelGGL(zsticks_inverse<T>, g1, dim3(256), lds_z, st, a, psi, ldpsi, static_cast<cx<T>*>(zbuf));
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (marks) (void)hipEventRecord(marks[1], st);
     hipLaunchKernelGGL(plane_sandwich<T>, dim3(a.p3.n, nvec), dim3(512), lds_p, st, a, vplane,
                        static_cast<cx<T>*>(zbuf));
     e = hipGetLastError();
@@ -189,12 +237,6 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
         e = hipGetLastError();
     }
     return e;
-}
-
-size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which) {
-    const size_t cs = fp32 ? 8 : 16;
-    if (which == 0) return cs * (a.p3.n + static_cast<size_t>(a.sticks_per_block) * a.p3.n);
-    return cs * (a.p1.n + a.p2.n + static_cast<size_t>(a.p2.n) * a.ncol + static_cast<size_t>(a.rows_per_pass) * a.p1.n);
 }
 
 hipError_t launch_hlocal(const HLocalArgs& a, bool fp32, const void* vplane, void* zbuf, const double* psi,

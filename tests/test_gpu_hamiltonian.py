@@ -84,6 +84,22 @@ def test_constant_potential_closed_form(gpu_context):
     assert rel_err(got, want) <= 1e-13
 
 
+@pytest.mark.parametrize("alat,ecut", [(8.0, 20.0), (9.3, 31.0), (6.1, 44.0)])
+def test_triclinic_boxes_fp64(gpu_context, alat, ecut):
+    """Unequal, non-cubic boxes (n1 != n2 != n3, radix plans with 2/3/5 stages)."""
+    from q3_amd.planewave import Crystal
+    lat = np.array([[1.0, 0.0, 0.0], [0.31, 1.07, 0.0], [0.22, 0.13, 1.29]])
+    c = Crystal(lat, alat, np.array([[0.0, 0.0, 0.0], [0.27, 0.31, 0.23]]), np.zeros(2, np.int32))
+    w = W.Workload("triclinic", c, ecut, (1, 1, 1), np.array([[0.1, -0.2, 0.3]]), 6, 3, [W.synthetic_species(0)])
+    case = kcase(w, 0)
+    assert len(set(case.basis.wave_dims)) == 3
+    psi = W.random_psi(case.kp.kinetic_energy_squared, 6, seed=4)
+    h = _applicator(gpu_context, case)
+    got = h.apply_batch(psi, FP64)
+    ref = _oracle(case, psi)
+    assert rel_err(got, ref) <= 1e-10
+
+
 def test_cfg4_kpoint_fp64(gpu_context):
     case = kcase(W.cfg4(), 7)
     psi = W.random_psi(case.kp.kinetic_energy_squared, 6, seed=9)
