@@ -84,10 +84,11 @@ namespace {
 
 // Radix plan for one FFT length.  Every transform is run by one wave64, so each stage must
 // have n / R <= 64 butterflies per transform.  Search all orderings of supported radices for
-// the fewest stages; ties go to the earlier radices of `pref` (large butterflies balanced for
-// 64-lane waves: 96 = 12 * 8, 48 = 12 * 4, 144 = 12 * 12).
+// the fewest stages; ties go to the earlier radices of `pref`.  Radices stop at 8: larger
+// composite butterflies need more registers than an FP64 stage can hold without spilling
+// (96 = 8 * 6 * 2, 48 = 8 * 6, 30 = 6 * 5).
 bool plan_search(int n, int rem, int depth, int maxdepth, int* cur, FftPlan& best) {
-    static const int pref[] = {12, 8, 16, 10, 15, 9, 6, 5, 4, 3, 2};
+    static const int pref[] = {8, 6, 5, 4, 3, 2};
     if (rem == 1) {
         best.nstage = depth;
         for (int i = 0; i < depth; ++i) best.radix[i] = cur[i];

@@ -1,6 +1,7 @@
 // Mixed-radix complex FFT on data held in LDS, for gfx950.
 //
-// Stockham autosort, in place.  A stage of radix R (R in {2,3,4,5,6,8,9,10,12,15,16}) maps
+// Stockham autosort, in place.  A stage of radix R (R in {2,3,4,5,6,8}; larger composite
+// butterflies exist below but push FP64 kernels past the register file) maps
 // thread tid to butterfly j = tid mod m (m = n/R) and transform group tid / m; the index
 // arithmetic and the R-1 twiddles of a thread are computed once per stage, and the butterfly
 // itself is an unrolled register DFT (composite radices as two levels of small DFTs with
@@ -178,12 +179,6 @@ __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int
     const int jn = j % ns;
     const int base = (j - jn) * R + jn;  // (j / ns) * ns * R + jn
     const int twb = jn * (n / (ns * R));
-    cx<T> w[R];
-#pragma unroll
-    for (int r = 1; r < R; ++r) {
-        const cx<T> t = tw[r * twb];
-        w[r] = {t.x, sgn < T(0) ? t.y : -t.y};
-    }
     for (int t0 = 0; t0 < ntrans; t0 += groups * BPT) {
         cx<T> v[BPT][R];
 #pragma unroll
@@ -193,7 +188,10 @@ __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int
                 const cx<T>* src = buf + t * ts + j * es;
                 v[b][0] = src[0];
 #pragma unroll
-                for (int r = 1; r < R; ++r) v[b][r] = cmul(src[r * m * es], w[r]);
+                for (int r = 1; r < R; ++r) {
+                    const cx<T> w = tw[r * twb];  // LDS twiddle table, fetched per use
+                    v[b][r] = sgn < T(0) ? cmul(src[r * m * es], w) : cmulc(src[r * m * es], w);
+                }
                 Dft<T, R>::run(v[b], sgn);
             }
         }
@@ -228,11 +226,6 @@ __device__ void fft_lds(cx<T>* buf, const FftPlan& p, const cx<T>* tw, int sign,
             case 5: fft_stage<T, 5, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
             case 6: fft_stage<T, 6, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
             case 8: fft_stage<T, 8, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 9: fft_stage<T, 9, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 10: fft_stage<T, 10, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 12: fft_stage<T, 12, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 15: fft_stage<T, 15, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 16: fft_stage<T, 16, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
             default: break;
         }
         ns *= R;
