@@ -46,6 +46,18 @@ def parse():
     return p.parse_args()
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of the FP64 kernel at this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (each counter scaled by its copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("bytes_per_launch", {}).get(f"q3x::{kernel}<double>")
+
+
 def copy_reference(steps):
     """Plain streaming copy of 1 GiB, steps times (HBM byte-counter calibration)."""
     from q3_amd import Context
@@ -173,7 +185,7 @@ def main():
         b = kernel_bytes(dom, nb, npw, nst, n3, npts, 16, 8)
         ach = b / (kf[dom]["avg_ms"] * 1e-3) / 1e9
         roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": b}
+                "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "algorithmic_bytes_per_launch": b}
     per_kernel = {}
     for label, cs, sv, peak in (("fp64", 16, 8, FP64_MFMA_TFLOPS), ("mixed", 8, 4, FP32_MFMA_TFLOPS)):
         for name, k in results[label]["kernels"].items():

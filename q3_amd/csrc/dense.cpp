@@ -24,12 +24,24 @@ namespace q3x {
 namespace {
 inline size_t ix(int i, int j, int ld) { return static_cast<size_t>(i) + static_cast<size_t>(j) * ld; }
 
+// Real and imaginary planes, column-major n x n, in one allocation.  The imaginary plane
+// starts 2 KiB (mod 4 KiB) after the real one: with equal 4 KiB offsets, a load of im[i]
+// right after a store to re[i] is taken by x86 cores as a possible store-to-load forward and
+// stalls (4K aliasing), which slowed the AXPY loops below several times over.
 struct Split {
     int n = 0;
-    std::vector<double> re, im;  // column-major n x n
-    explicit Split(int n_) : n(n_), re(static_cast<size_t>(n_) * n_, 0.0), im(static_cast<size_t>(n_) * n_, 0.0) {}
-    double* cr(int j) { return re.data() + static_cast<size_t>(j) * n; }
-    double* ci(int j) { return im.data() + static_cast<size_t>(j) * n; }
+    std::vector<double> buf;
+    double* re = nullptr;
+    double* im = nullptr;
+    explicit Split(int n_) : n(n_) {
+        const size_t nn = static_cast<size_t>(n_) * n_;
+        const size_t shift = (256 + 512 - nn % 512) % 512;  // (nn + shift) * 8 = 2048 mod 4096
+        buf.assign(2 * nn + shift + 8, 0.0);
+        re = buf.data();
+        im = buf.data() + nn + shift;
+    }
+    double* cr(int j) { return re + static_cast<size_t>(j) * n; }
+    double* ci(int j) { return im + static_cast<size_t>(j) * n; }
 };
 
 // Upper Cholesky in place on split storage: A = U^H U.  Right-looking: after column j is
@@ -426,86 +438,85 @@ int hegv(int n, const zc* H, int ldh, const zc* S, int lds, double* w, zc* X, in
     }
     // 3. Householder tridiagonalisation.  Reflector k acts on rows k+1..n-1; its vector
     // (v0 = 1) is kept in column k below the sub-diagonal, tau[k] real.
+    // One parallel region for all n-2 steps.  Column j of the trailing block belongs to
+    // thread j mod T for the whole reduction, and both halves of a step touch only owned
+    // columns: p_j = t * (column j)^H v (the block is Hermitian, so this is row j of B v), then
+    // column j -= v conj(q_j) + q conj(v_j).  The block therefore stays in the owners' caches;
+    // a step costs two barriers.
     std::vector<double> tau(n, 0.0), vr(n), vi(n), pr(n), pi(n);
     std::vector<double> offr(n, 0.0), offi(n, 0.0);
-    const int nthreads = omp_get_max_threads();
-    for (int k = 0; k < n - 2; ++k) {
-        const int len = n - k - 1;
-        double* xr = C.cr(k) + k + 1;
-        double* xi = C.ci(k) + k + 1;
-        double xn2 = 0.0;
-        for (int i = 0; i < len; ++i) xn2 += xr[i] * xr[i] + xi[i] * xi[i];
-        const double xnorm = std::sqrt(xn2);
-        if (xnorm == 0.0) continue;
-        const double aabs = std::hypot(xr[0], xi[0]);
-        const double phr = aabs == 0.0 ? 1.0 : xr[0] / aabs, phi = aabs == 0.0 ? 0.0 : xi[0] / aabs;
-        const double br = -phr * xnorm, bi = -phi * xnorm;  // beta = -phase * |x|
-        const double v0r = xr[0] - br, v0i = xi[0] - bi;
-        const double v0n = v0r * v0r + v0i * v0i;
-        vr[0] = 1.0;
-        vi[0] = 0.0;
-        double vn2 = 1.0;
-        for (int i = 1; i < len; ++i) {
-            vr[i] = (xr[i] * v0r + xi[i] * v0i) / v0n;
-            vi[i] = (xi[i] * v0r - xr[i] * v0i) / v0n;
-            vn2 += vr[i] * vr[i] + vi[i] * vi[i];
-        }
-        const double t = 2.0 / vn2;
-        tau[k] = t;
-        const int off = k + 1;
-        const int nt = std::max(1, std::min(nthreads, len / 24));
-#pragma omp parallel num_threads(nt)
-        {
-            const int T = omp_get_num_threads(), tid = omp_get_thread_num();
-            const int r0 = (len * tid) / T, r1 = (len * (tid + 1)) / T;
-            // p = t * B v  (B = trailing block, full Hermitian storage), rows r0..r1
-            for (int i = r0; i < r1; ++i) {
-                pr[i] = 0.0;
-                pi[i] = 0.0;
-            }
-            for (int j = 0; j < len; ++j) {
-                const double* br_ = C.cr(off + j) + off;
-                const double* bi_ = C.ci(off + j) + off;
-                const double a = vr[j], b = vi[j];
+    const int nthreads = std::max(1, std::min(omp_get_max_threads(), n / 16));
+#pragma omp parallel num_threads(nthreads)
+    {
+        const int T = omp_get_num_threads(), tid = omp_get_thread_num();
+        for (int k = 0; k < n - 2; ++k) {
+            const int len = n - k - 1, off = k + 1;
+            double* xr = C.cr(k) + off;
+            double* xi = C.ci(k) + off;
+#pragma omp single
+            {
+                double xn2 = 0.0;
+                for (int i = 0; i < len; ++i) xn2 += xr[i] * xr[i] + xi[i] * xi[i];
+                const double xnorm = std::sqrt(xn2);
+                if (xnorm == 0.0) {
+                    tau[k] = 0.0;
+                } else {
+                    const double aabs = std::hypot(xr[0], xi[0]);
+                    const double phr = aabs == 0.0 ? 1.0 : xr[0] / aabs, phi = aabs == 0.0 ? 0.0 : xi[0] / aabs;
+                    const double br = -phr * xnorm, bi = -phi * xnorm;  // beta = -phase * |x|
+                    const double v0r = xr[0] - br, v0i = xi[0] - bi;
+                    const double v0n = v0r * v0r + v0i * v0i;
+                    vr[0] = 1.0;
+                    vi[0] = 0.0;
+                    double vn2 = 1.0;
+                    for (int i = 1; i < len; ++i) {
+                        vr[i] = (xr[i] * v0r + xi[i] * v0i) / v0n;
+                        vi[i] = (xi[i] * v0r - xr[i] * v0i) / v0n;
+                        vn2 += vr[i] * vr[i] + vi[i] * vi[i];
+                    }
+                    for (int i = 1; i < len; ++i) {  // reflector kept in column k
+                        xr[i] = vr[i];
+                        xi[i] = vi[i];
+                    }
+                    tau[k] = 2.0 / vn2;
+                    offr[off] = br;
+                    offi[off] = bi;
+                }
+            }  // implicit barrier: v, tau visible
+            const double t = tau[k];
+            if (t != 0.0) {
+                // p = t * B v over owned columns
+                for (int j = (tid - off % T + T) % T; j < len; j += T) {
+                    const double* br_ = C.cr(off + j) + off;
+                    const double* bi_ = C.ci(off + j) + off;
+                    double sr = 0.0, si = 0.0;
+#pragma omp simd reduction(+ : sr, si)
+                    for (int i = 0; i < len; ++i) {  // conj(B[i][j]) v[i]
+                        sr += br_[i] * vr[i] + bi_[i] * vi[i];
+                        si += br_[i] * vi[i] - bi_[i] * vr[i];
+                    }
+                    pr[j] = t * sr;
+                    pi[j] = t * si;
+                }
+#pragma omp barrier
+                double vhp = 0.0;  // v^H p is real; every thread forms it in the same order
+                for (int i = 0; i < len; ++i) vhp += vr[i] * pr[i] + vi[i] * pi[i];
+                const double K = 0.5 * t * vhp;
+                // B -= v q^H + q v^H with q = p - K v, owned columns
+                for (int j = (tid - off % T + T) % T; j < len; j += T) {
+                    double* cr = C.cr(off + j) + off;
+                    double* ci = C.ci(off + j) + off;
+                    const double wr = vr[j], wi = vi[j];
+                    const double qr = pr[j] - K * wr, qi = pi[j] - K * wi;
 #pragma omp simd
-                for (int i = r0; i < r1; ++i) {
-                    pr[i] += br_[i] * a - bi_[i] * b;
-                    pi[i] += br_[i] * b + bi_[i] * a;
+                    for (int i = 0; i < len; ++i) {
+                        const double sr = pr[i] - K * vr[i], si = pi[i] - K * vi[i];
+                        cr[i] -= (vr[i] * qr + vi[i] * qi) + (sr * wr + si * wi);
+                        ci[i] -= (vi[i] * qr - vr[i] * qi) + (si * wr - sr * wi);
+                    }
                 }
             }
 #pragma omp barrier
-#pragma omp single
-            {
-                double vhp = 0.0;  // v^H p is real
-                for (int i = 0; i < len; ++i) {
-                    pr[i] *= t;
-                    pi[i] *= t;
-                    vhp += vr[i] * pr[i] + vi[i] * pi[i];
-                }
-                const double K = 0.5 * t * vhp;
-                for (int i = 0; i < len; ++i) {
-                    pr[i] -= K * vr[i];
-                    pi[i] -= K * vi[i];
-                }
-            }
-            // B -= v q^H + q v^H, columns split over threads
-            const int c0 = (len * tid) / T, c1 = (len * (tid + 1)) / T;
-            for (int j = c0; j < c1; ++j) {
-                double* cr = C.cr(off + j) + off;
-                double* ci = C.ci(off + j) + off;
-                const double qr = pr[j], qi = pi[j], wr = vr[j], wi = vi[j];
-#pragma omp simd
-                for (int i = 0; i < len; ++i) {
-                    cr[i] -= (vr[i] * qr + vi[i] * qi) + (pr[i] * wr + pi[i] * wi);
-                    ci[i] -= (vi[i] * qr - vr[i] * qi) + (pi[i] * wr - pr[i] * wi);
-                }
-            }
-        }
-        offr[k + 1] = br;
-        offi[k + 1] = bi;
-        for (int i = 1; i < len; ++i) {
-            xr[i] = vr[i];
-            xi[i] = vi[i];
         }
     }
     for (int k = 0; k < n - 2; ++k)
