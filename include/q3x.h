@@ -147,6 +147,14 @@ const char* q3x_profile_kernel_name(int id);
  *      read; all n eigenvalues ascending; the lowest nvec eigenvectors into X (n x nvec),
  *      X^H S X = I.  Returns Q3X_OK or Q3X_ERR_NUMERIC (S not positive definite). */
 q3x_status q3x_hegv(int n, const double* H, const double* S, double* w, double* X, int nvec);
+/* the same problem on device arrays (the GPU path of generalized_hermitian_eigenvalue_solve,
+ * math/linear_algebra/gpu/lapack/init.hpp:77-82), solved on the device: H, S complex128 with
+ * leading dimensions ldh, lds (upper triangles read); the lowest nvec eigenvalues ascending into
+ * host w, their S-orthonormal eigenvectors into device X (n x nvec, ld ldx).  Synchronizes the
+ * context stream.  Returns Q3X_OK, Q3X_ERR_NUMERIC (S not positive definite) or
+ * Q3X_ERR_DEVICE. */
+q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, const void* S, long lds,
+                           int nvec, double* w, void* X, long ldx);
 
 /* ---- block Davidson (PlanewaveDavidsonDiagonalizer::solve,
  *      src/layer_4_computation/diagonalizer/planewave/davidson.hpp:324-841, with the subspace

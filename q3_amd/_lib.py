@@ -61,6 +61,7 @@ SIGNATURES = {
     "q3x_profile_read": (_i, [_p, _i, _p, _p, _i]),
     "q3x_profile_kernel_name": (ctypes.c_char_p, [_i]),
     "q3x_hegv": (_i, [_i, _p, _p, _p, _p, _i]),
+    "q3x_hegv_device": (_i, [_p, _i, _p, _l, _p, _l, _i, _p, _p, _l]),
     "q3x_davidson_default_options": (None, [ctypes.POINTER(DavidsonOptions)]),
     "q3x_davidson_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p, ctypes.POINTER(DavidsonReport)]),
 }

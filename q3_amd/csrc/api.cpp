@@ -9,6 +9,7 @@
 
 #include "aux_kernels.h"
 #include "runtime.h"
+#include "subspace_eig.h"
 #include "zgemm.h"
 
 namespace q3x {
@@ -353,6 +354,28 @@ q3x_status q3x_zgemm(q3x_context ctx, char transa, int m, int n, int k, double a
                   "q3x_zgemm")
                ? Q3X_OK
                : Q3X_ERR_DEVICE;
+}
+
+q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, const void* S, long lds, int nvec,
+                           double* w, void* X, long ldx) {
+    if (!ctx || n < 0 || nvec < 0 || nvec > n ||
+        (n > 0 && (!H || !S || (nvec > 0 && (!w || !X)) || ldh < n || lds < n || ldx < n))) {
+        set_error("q3x_hegv_device: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (n == 0 || nvec == 0) return Q3X_OK;
+    const size_t ws = size_t(64) << 20;
+    if (!ctx->gemm_work.ensure(ws) || !ctx->eig_work.ensure(subspace_eig_workspace(n, nvec))) return Q3X_ERR_DEVICE;
+    hipError_t e = hipSuccess;
+    const int info = subspace_eig(n, static_cast<const double*>(H), ldh, static_cast<const double*>(S), lds, nvec, w,
+                                  static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
+                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->stream, &e, nullptr);
+    if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
+    if (info > 0) {
+        set_error("q3x_hegv_device: S is not positive definite");
+        return Q3X_ERR_NUMERIC;
+    }
+    return Q3X_OK;
 }
 
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {

@@ -15,6 +15,9 @@ hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const
 hipError_t launch_axpby(int npw, int ncol, const double* a, const double* X, long ldx, double beta, double* Y,
                         long ldy, hipStream_t st);
 hipError_t launch_zero(double* p, long ndouble, hipStream_t st);
+// Y[:, c] = s[c] X[:, src[c]] (s null -> 1), complex columns of length rows
+hipError_t launch_gather_columns(int rows, int ncol, const int* src, const double* s, const double* X, long ldx,
+                                 double* Y, long ldy, hipStream_t st);
 // plain copy of n16 16-byte elements (HBM byte-counter reference)
 hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st);

@@ -109,6 +109,27 @@ hipError_t launch_scale_copy(int npw, int ncol, const double* X, long ldx, const
     return hipGetLastError();
 }
 
+// Y[:, c] = s[c] X[:, src[c]] (s null -> 1): picks and scales the subspace eigenvectors of the
+// unconverged bands (serial.hpp:1020-1060).
+__global__ void gather_columns(int rows, const int* __restrict__ src, const double* __restrict__ s,
+                               const double* __restrict__ X, long ldx, double* __restrict__ Y, long ldy) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= rows) return;
+    const double f = s ? s[c] : 1.0;
+    const long ox = 2 * (g + src[c] * ldx), oy = 2 * (g + c * ldy);
+    Y[oy] = f * X[ox];
+    Y[oy + 1] = f * X[ox + 1];
+}
+
+hipError_t launch_gather_columns(int rows, int ncol, const int* src, const double* s, const double* X, long ldx,
+                                 double* Y, long ldy, hipStream_t st) {
+    if (rows == 0 || ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_columns, dim3((rows + 255) / 256, ncol), dim3(256), 0, st, rows, src, s, X, ldx, Y,
+                       ldy);
+    return hipGetLastError();
+}
+
 // Y = a_c X + beta Y column by column (a_c = a[c], or 1 when a is null).  X may alias Y.
 __global__ void axpby_columns(int npw, const double* __restrict__ a, const double* X, long ldx, double beta,
                               double* Y, long ldy) {

@@ -1,11 +1,12 @@
 // Block Davidson with a device-resident basis (PlanewaveDavidsonDiagonalizer::solve,
 // davidson.hpp:324-841, with the subspace manager of subspace_manager/serial.hpp).
 //
-// The basis V, H V, the correction block and the Gram blocks live in HBM; every product over
-// the plane-wave dimension is an MFMA complex GEMM.  What stays on the host is what the
-// reference also decides on the host: the M x M generalized eigenproblem (fp64), the n_c x n_c
-// Cholesky of the Cholesky-QR (fp64), the convergence test on eigenvalue changes and the
-// collapse decision.  Control flow and constants are the reference's:
+// The basis V, H V, the correction block, the subspace matrices H_sub / S_sub and their
+// eigenvectors live in HBM; every product over the plane-wave dimension is an MFMA complex
+// GEMM and the M x M generalized eigenproblem is solved on the device (subspace_eig.hip).
+// The host keeps what the reference also decides on the host: the eigenvalue-change
+// convergence test, the choice of corrections, the n_c x n_c Cholesky of the Cholesky-QR and
+// the collapse decision.  Control flow and constants are the reference's:
 //   M_max = max(min(factor*nb, npw), nb)                           davidson.hpp:399-404
 //   iteration 0 never converges; occupied |de| < tol, empty < max(5 tol, 1e-5)  :607-629
 //   corrections r = (HV - e V) c for unconverged bands, Teter preconditioner  serial.hpp:995-1156
@@ -28,6 +29,7 @@
 #include "aux_kernels.h"
 #include "dense.h"
 #include "runtime.h"
+#include "subspace_eig.h"
 #include "zgemm.h"
 
 namespace q3x {
@@ -44,9 +46,10 @@ struct Solver {
     hipStream_t st;
     int npw, nb, nocc, mmax;
     int precision;
-    DevBuf V, HV, R, C, G, Hs_d, Ss_d, X_d, eps_d, work, norms_d, idx_d;
+    // X_d: subspace eigenvectors (n x nvec, ld n) of the last diag(); Xc_d: their columns for
+    // the corrections; idx_d / eps_d / negeps_d: band indices and energies of the corrections
+    DevBuf V, HV, R, C, G, Hs_d, Ss_d, X_d, Xc_d, eps_d, negeps_d, work, norms_d, idx_d, eigw;
     size_t work_bytes = size_t(256) << 20;
-    std::vector<zc> Hs, Ss;  // host M_max x M_max, ld M_max
     int h_applications = 0;
     double stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     q3x_status status = Q3X_OK;
@@ -84,7 +87,8 @@ struct Solver {
         return true;
     }
 
-    // H_sub[:total, start:start+nnew] = V^H HV_new ; S_sub likewise (serial.hpp:555-678)
+    // H_sub[:total, start:start+nnew] = V^H HV_new ; S_sub likewise (serial.hpp:555-678).
+    // Upper triangles of the leading total x total blocks are what diag() reads.
     bool update(int start, int nnew, int total) {
         if (nnew <= 0) return true;
         const double t0 = now_ms();
@@ -92,22 +96,25 @@ struct Solver {
         double* ss = Ss_d.as<double>() + 2L * start * mmax;
         if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, HVc(start), npw, 0.0, hs, mmax)) return false;
         if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, Vc(start), npw, 0.0, ss, mmax)) return false;
-        const size_t bytes = static_cast<size_t>(nnew) * mmax * 16;
-        if (!ok(hipMemcpyAsync(Hs.data() + static_cast<size_t>(start) * mmax, hs, bytes, hipMemcpyDeviceToHost, st),
-                "D2H H_sub") ||
-            !ok(hipMemcpyAsync(Ss.data() + static_cast<size_t>(start) * mmax, ss, bytes, hipMemcpyDeviceToHost, st),
-                "D2H S_sub") ||
-            !sync())
-            return false;
+        if (!sync()) return false;
         stage[3] += now_ms() - t0;
         return true;
     }
 
-    int diag(int n, std::vector<double>& w, std::vector<zc>& X, int nvec) {
+    // Lowest nvec eigenpairs of the leading n x n pencil; eigenvalues into w (host), vectors
+    // into X_d (n x nvec, ld n).  Returns 0, n + 1 (S not positive definite) or -1.
+    int diag(int n, std::vector<double>& w, int nvec) {
         const double t0 = now_ms();
-        X.assign(static_cast<size_t>(n) * std::max(nvec, 1), zc(0, 0));
+        nvec = std::max(1, std::min(nvec, n));
         w.assign(std::max(n, mmax), 0.0);
-        int info = hegv(n, Hs.data(), mmax, Ss.data(), mmax, w.data(), X.data(), n, nvec);
+        hipError_t e = hipSuccess;
+        const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
+                                      X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
+                                      h->ctx->max_lds, st, &e, nullptr);
+        if (info < 0) {
+            ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
+            return -1;
+        }
         stage[4] += now_ms() - t0;
         return info;
     }
@@ -168,11 +175,8 @@ struct Solver {
         return accepted;
     }
 
-    // V[:, :nb] <- V[:, :n] X[:, :nb], same for HV (serial.hpp:825-945)
-    bool rotate(int n, const std::vector<zc>& X, int ncols) {
-        if (!ok(hipMemcpyAsync(X_d.get(), X.data(), static_cast<size_t>(n) * ncols * 16, hipMemcpyHostToDevice, st),
-                "H2D rotation"))
-            return false;
+    // V[:, :ncols] <- V[:, :n] X[:, :ncols], same for HV (serial.hpp:825-945); X = X_d (ld n)
+    bool rotate(int n, int ncols) {
         for (double* base : {Vc(0), HVc(0)}) {
             if (!gemm(0, npw, ncols, n, 1.0, base, npw, X_d.get(), n, 0.0, R.get(), npw)) return false;
             if (!ok(hipMemcpyAsync(base, R.get(), static_cast<size_t>(npw) * ncols * 16, hipMemcpyDeviceToDevice, st),
@@ -182,29 +186,24 @@ struct Solver {
         return sync();
     }
 
-    bool rebuild_subspace(int n) {
-        std::fill(Hs.begin(), Hs.end(), zc(0, 0));
-        std::fill(Ss.begin(), Ss.end(), zc(0, 0));
-        return update(0, n, n);
-    }
-
     int collapse(int n, bool have_matrices) {
         const double t0 = now_ms();
         std::vector<double> w;
-        std::vector<zc> X;
-        int info = -1;
-        if (have_matrices) info = diag(n, w, X, nb);
+        int info = -2;
+        if (have_matrices) info = diag(n, w, nb);
+        if (info == -1) return -1;
         if (info != 0) {
             if (!update(0, n, n)) return -1;  // build_subspace_matrices_and_diagonalize
-            info = diag(n, w, X, nb);
+            info = diag(n, w, nb);
+            if (info == -1) return -1;
             if (info != 0) {
                 status = Q3X_ERR_NUMERIC;
                 set_error("Davidson: subspace eigensolver failed on a rebuilt subspace");
                 return -1;
             }
         }
-        if (!rotate(n, X, nb)) return -1;
-        if (!rebuild_subspace(nb)) return -1;
+        if (!rotate(n, nb)) return -1;
+        if (!update(0, nb, nb)) return -1;
         stage[5] += now_ms() - t0;
         return nb;
     }
@@ -252,15 +251,14 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
     const int npw = s.npw, mmax = s.mmax;
     const double t_start = s.now_ms();
     const size_t col = static_cast<size_t>(npw) * 16;
+    const size_t mm16 = static_cast<size_t>(mmax) * mmax * 16;
     if (!s.V.ensure(col * mmax) || !s.HV.ensure(col * mmax) || !s.R.ensure(col * std::max(nb, 1)) ||
-        !s.C.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.G.ensure(static_cast<size_t>(mmax) * mmax * 16) ||
-        !s.Hs_d.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.Ss_d.ensure(static_cast<size_t>(mmax) * mmax * 16) ||
-        !s.X_d.ensure(static_cast<size_t>(mmax) * mmax * 16) || !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) ||
-        !s.norms_d.ensure(static_cast<size_t>(mmax) * 8) || !s.idx_d.ensure(static_cast<size_t>(nb) * 4) ||
-        !s.work.ensure(s.work_bytes))
+        !s.C.ensure(mm16) || !s.G.ensure(mm16) || !s.Hs_d.ensure(mm16) || !s.Ss_d.ensure(mm16) ||
+        !s.X_d.ensure(mm16) || !s.Xc_d.ensure(static_cast<size_t>(mmax) * nb * 32) ||
+        !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) || !s.negeps_d.ensure(static_cast<size_t>(mmax) * 8) ||
+        !s.norms_d.ensure(static_cast<size_t>(mmax) * 8) || !s.idx_d.ensure(static_cast<size_t>(mmax) * 4) ||
+        !s.work.ensure(s.work_bytes) || !s.eigw.ensure(subspace_eig_workspace(mmax, mmax)))
         return Q3X_ERR_DEVICE;
-    s.Hs.assign(static_cast<size_t>(mmax) * mmax, zc(0, 0));
-    s.Ss.assign(static_cast<size_t>(mmax) * mmax, zc(0, 0));
     int prec = opt->precision;
 
     // ---- seed (davidson.hpp:1000-1101)
@@ -291,11 +289,18 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
         if (!s.apply(start, basis - start, prec)) return s.status;
     }
     if (!s.update(0, basis, basis)) return s.status;
+    std::vector<double> prev(nb, 0.0), w;
+    {  // starting values: the Rayleigh quotients on the diagonal of H_sub
+        std::vector<zc> diag(basis);
+        if (!s.ok(hipMemcpy2DAsync(diag.data(), 16, s.Hs_d.get(), static_cast<size_t>(mmax + 1) * 16, 16, basis,
+                                   hipMemcpyDeviceToHost, s.st),
+                  "D2H diagonal") ||
+            !s.sync())
+            return s.status;
+        for (int b = 0; b < nb && b < basis; ++b) prev[b] = diag[b].real();
+    }
     s.stage[0] += s.now_ms() - t0;
 
-    std::vector<double> prev(nb, 0.0), w;
-    for (int b = 0; b < nb && b < basis; ++b) prev[b] = s.Hs[static_cast<size_t>(b) * mmax + b].real();
-    std::vector<zc> X;
     const double tol = opt->tolerance_ry;
     const double empty_tol = std::max(kEmptyFactor * tol, kEmptyFloor);
     int it = 0, unconv = nb, iterations_run = 0, refinements = 0;
@@ -304,11 +309,13 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
     for (;;) {
         for (it = 0; it < max_iter; ++it) {
             ++iterations_run;
-            int info = s.diag(basis, w, X, basis);
+            int info = s.diag(basis, w, nb);
+            if (info < 0) return s.status;
             if (info != 0 && basis > nb) {  // davidson.hpp:520-590
                 basis = s.collapse(basis, false);
                 if (basis < 0) return s.status;
-                info = s.diag(basis, w, X, basis);
+                info = s.diag(basis, w, nb);
+                if (info < 0) return s.status;
                 if (info != 0)
                     for (int b = 0; b < nb; ++b) w[b] = prev[b];
             }
@@ -331,24 +338,27 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
             }
             // corrections into V[:, basis:basis+nc]: (HV - e V) X_c, preconditioned
             const int nc = static_cast<int>(src.size());
-            std::vector<zc> Xc(static_cast<size_t>(basis) * nc * 2);
-            std::vector<double> ec(nc);
-            for (int c = 0; c < nc; ++c) {
-                ec[c] = w[src[c]];
-                for (int r = 0; r < basis; ++r) {
-                    const zc x = X[static_cast<size_t>(src[c]) * basis + r];
-                    Xc[static_cast<size_t>(c) * basis + r] = x;                                  // for HV
-                    Xc[static_cast<size_t>(basis) * nc + static_cast<size_t>(c) * basis + r] = -ec[c] * x;  // for V
-                }
-            }
             if (nc > 0) {
+                std::vector<double> ec(nc), nec(nc);
+                for (int c = 0; c < nc; ++c) {
+                    ec[c] = w[src[c]];
+                    nec[c] = -ec[c];
+                }
                 double* corr = s.Vc(basis);
-                double* xd = s.X_d.as<double>();
-                if (!s.ok(hipMemcpyAsync(xd, Xc.data(), Xc.size() * 16, hipMemcpyHostToDevice, s.st), "H2D coeffs") ||
-                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps"))
+                double* xc = s.Xc_d.as<double>();
+                if (!s.ok(hipMemcpyAsync(s.idx_d.get(), src.data(), nc * 4, hipMemcpyHostToDevice, s.st), "H2D idx") ||
+                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps") ||
+                    !s.ok(hipMemcpyAsync(s.negeps_d.get(), nec.data(), nc * 8, hipMemcpyHostToDevice, s.st),
+                          "H2D -eps") ||
+                    !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), nullptr, s.X_d.as<double>(), basis, xc,
+                                                basis, s.st),
+                          "gather X") ||
+                    !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), s.negeps_d.as<double>(),
+                                                s.X_d.as<double>(), basis, xc + 2L * basis * nc, basis, s.st),
+                          "gather -e X"))
                     return s.status;
-                if (!s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xd + 2L * basis * nc, basis, 0.0, corr, npw) ||
-                    !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xd, basis, 1.0, corr, npw) ||
+                if (!s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xc + 2L * basis * nc, basis, 0.0, corr, npw) ||
+                    !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xc, basis, 1.0, corr, npw) ||
                     !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), s.eps_d.as<double>(), corr, npw, s.st),
                           "Teter"))
                     return s.status;
@@ -368,7 +378,9 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
             for (int b = 0; b < nb; ++b) prev[b] = w[b];
         }
         if (!done) {  // davidson.hpp:812-827: final subspace solve after the cap
-            if (s.diag(basis, w, X, basis) != 0) {
+            const int info = s.diag(basis, w, nb);
+            if (info < 0) return s.status;
+            if (info != 0) {
                 set_error("Davidson: final subspace eigensolve failed");
                 return Q3X_ERR_NUMERIC;
             }
@@ -376,11 +388,11 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
         if (prec == Q3X_MIXED && opt->final_fp64_sweep && refinements == 0) {
             // fp64 polish: Ritz vectors of the mixed solve, H re-applied in fp64, iterate on
             ++refinements;
-            if (!s.rotate(basis, X, nb)) return s.status;
+            if (!s.rotate(basis, nb)) return s.status;
             basis = nb;
             prec = Q3X_FP64;
             if (!s.apply(0, nb, prec)) return s.status;
-            if (!s.rebuild_subspace(nb)) return s.status;
+            if (!s.update(0, nb, nb)) return s.status;
             for (int b = 0; b < nb; ++b) prev[b] = w[b];
             done = false;
             max_iter = std::max(opt->max_iterations, 1);
@@ -390,7 +402,7 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
     }
     // extract (serial.hpp:947-993): psi = V X[:, :nb]; then the fp64 residual sweep
     // r_n = H psi_n - e_n psi_n, reported as max_n ||r_n||
-    if (!s.rotate(basis, X, nb)) return s.status;
+    if (!s.rotate(basis, nb)) return s.status;
     double maxres = 0.0;
     {
         std::vector<double> e(nb), n2(nb);

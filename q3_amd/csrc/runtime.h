@@ -66,6 +66,7 @@ struct FftTables {
 struct q3x_context_s {
     q3x::Profiler prof;
     q3x::DevBuf gemm_work;  // split-K workspace of q3x_zgemm
+    q3x::DevBuf eig_work;   // workspace of q3x_hegv_device
     int device = 0;
     hipStream_t stream = nullptr;
     size_t max_lds = 65536;

@@ -1,0 +1,29 @@
+// Generalized Hermitian eigensolver for the Davidson subspace problem, on the device
+// (subspace_eig.hip).  Stands in for GPUEigensolver::generalized_hermitian_eigenvalue_solve
+// (math/linear_algebra/gpu/lapack/init.hpp:77-82) with zhegv itype-1 semantics.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace q3x {
+
+// Device workspace for subspace_eig(n, nvec, ...), in bytes.
+size_t subspace_eig_workspace(int n, int nvec);
+
+struct SubspaceEigStats {
+    int newton_schulz_iterations = 0;  // 0: first-order Loewdin factor was accurate enough
+    double s_deviation = 0.0;          // max |S - I| over the compacted matrix
+};
+
+// Solves H x = w S x for the nvec lowest pairs.  H and S (device, complex128, column-major,
+// leading dimensions ldh / lds) are read from their upper triangles, as zhegv 'U' does.
+// On success w_host holds the eigenvalues (ascending) and X (device, ld ldx) the
+// S-orthonormal eigenvectors; returns 0.  Returns n + 1 when S is not numerically positive
+// definite (the zhegv info > n case) and -1 on a device error (*err set).
+// work: subspace_eig_workspace(n, nvec) bytes; gemm_work: split-K workspace of zgemm.
+int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
+                 long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
+                 hipStream_t st, hipError_t* err, SubspaceEigStats* stats);
+
+}  // namespace q3x

@@ -112,6 +112,16 @@ def zgemm(context: Context, transa: str, m: int, n: int, k: int, alpha: float, A
                           ctypes.c_void_p(B.ptr), ldb, beta, ctypes.c_void_p(C.ptr), ldc, precision))
 
 
+def hegv_device(context: Context, n: int, H: DeviceArray, ldh: int, S: DeviceArray, lds: int, nvec: int,
+                X: DeviceArray, ldx: int) -> np.ndarray:
+    """H x = w S x on device matrices (upper triangles read); returns the nvec lowest eigenvalues
+    (ascending), the S-orthonormal eigenvectors land in X (n x nvec, ld ldx)."""
+    w = np.zeros(max(nvec, 1))
+    check(lib().q3x_hegv_device(context.h, n, ctypes.c_void_p(H.ptr), ldh, ctypes.c_void_p(S.ptr), lds, nvec,
+                                _ptr(w), ctypes.c_void_p(X.ptr), ldx))
+    return w[:nvec]
+
+
 class PlanewaveBasis:
     def __init__(self, crystal: Crystal, ecutwfc_ry: float, ecutrho_ry: float = 0.0,
                  use_exact_wavefunction_grid: bool = True, fft_factors=(1, 1, 1)):

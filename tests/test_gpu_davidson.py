@@ -94,7 +94,13 @@ def test_seed_from_psi_and_capped_iterations(gpu_context):
     # and far fewer H applications than the solve from plane waves
     assert 2 <= d.report_dict()["iterations"] <= 10
     assert d.report_dict()["h_applications"] < fresh
-    assert np.abs(e1[:4] - e0[:4]).max() <= 1e-8
+    # At this k-point band 3 (top occupied) is exactly degenerate with band 4 (first empty).
+    # The empty partner only has to pass the empty-band test (>= 1e-5 Ry), so the pair's Ritz
+    # values may stop ~1e-7 Ry above the exact level; the oracle's own fresh solve stops at
+    # +1.75e-7 Ry there.  Bands 0-2 must agree to 1e-8; band 3 to the reference's
+    # Davidson-vs-direct occupied bound, 1e-6 Ry (test_scf_integration.cpp:504-516).
+    assert np.abs(e1[:3] - e0[:3]).max() <= 1e-8
+    assert abs(e1[3] - e0[3]) <= 1e-6
 
 
 def test_cfg2_davidson_fp64_residuals(gpu_context):
