@@ -38,10 +38,12 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-davidson", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--batch-workers", type=int, default=8)
     p.add_argument("--davidson-iters", type=int, default=60)
-    p.add_argument("--mode", choices=["full", "hpsi", "copy"], default="full",
+    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "copy"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
-                        "copy: a plain 1 GiB device copy, the reference for HBM byte counters")
+                        "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; copy: a plain 1 GiB device copy, the reference for HBM byte counters")
     p.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     return p.parse_args()
 
@@ -93,14 +95,63 @@ def kernel_bytes(name, nvec, npw, nst, n3, npts, cs, sv):
     return None
 
 
+def batched_kpoints(args):
+    """64 random k-points of the 8-atom cubic Si cell (20 Ha, 48^3, 48 bands, seed 3) solved as
+    one batch over worker streams, FP64 and mixed, beside the same batch on a single worker."""
+    from q3_amd import FP64, MIXED, KPointBatch
+    from q3_amd import workloads as W
+    from q3_amd.planewave import PlanewaveBasis
+    try:
+        w = W.cfg4()
+        t0 = time.time()
+        basis = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, True, w.fft_factors)
+        V = W.local_potential(w.crystal, basis, w.species)
+        bs, db = W.d_blocks_for(w.crystal, w.species)
+        kps = [basis.compute_gk(k) for k in w.kpoints]
+        betas = [kp.kb_projectors(w.crystal, w.species) for kp in kps]
+        npws = [kp.npw for kp in kps]
+        log(f"batched setup {time.time() - t0:.1f}s: {len(kps)} k-points, npw {min(npws)}..{max(npws)}")
+        out = {"workload": "cfg4: Si8 cubic a=10.26 bohr, 20 Ha, 48^3, 48 bands (16 occupied), 64 random k (seed 3)",
+               "kpoints": len(kps), "npw_min": min(npws), "npw_max": max(npws), "tolerance_ry": 2e-6}
+        eigs = {}
+        for label, prec, workers in (("fp64", FP64, args.batch_workers), ("mixed", MIXED, args.batch_workers),
+                                     ("fp64_one_worker", FP64, 1)):
+            batch = KPointBatch(workers=workers)
+            for kp, beta in zip(kps, betas):
+                batch.add(basis, kp, V, beta, bs, db)
+            batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec)  # warm the workspaces
+            t = time.perf_counter()
+            e, _ = batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec)
+            dt = time.perf_counter() - t
+            eigs[label] = e
+            its = [r.iterations for r in batch.reports]
+            out[label] = {"workers": workers, "s_all_kpoints": dt, "kpoint_solves_per_s": len(kps) / dt,
+                          "iterations_mean": float(np.mean(its)), "unconverged": int(sum(r.unconverged for r in batch.reports))}
+            log(f"batched {label}: {dt:.3f}s for {len(kps)} k-points ({workers} workers)")
+            batch.close()
+        occ = w.noccupied
+        out["mixed"]["max_abs_err_ha_occupied_vs_fp64"] = float(np.abs(eigs["mixed"] - eigs["fp64"])[:, :occ].max() / 2)
+        out["speedup_workers_vs_one"] = out["fp64_one_worker"]["s_all_kpoints"] / out["fp64"]["s_all_kpoints"]
+        out["speedup_mixed_vs_fp64"] = out["fp64"]["s_all_kpoints"] / out["mixed"]["s_all_kpoints"]
+        return out
+    except Exception as e:  # reported, not hidden
+        log(f"batched failed: {e}")
+        return {"error": str(e)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
+    # one hardware queue per batch worker stream (HIP's default is 4), set before HIP starts
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(16, args.batch_workers))))
     if args.mode == "copy":
         return copy_reference(args.steps)
+    if args.mode == "batch":
+        print(json.dumps({"mode": "batch", **batched_kpoints(args)}))
+        return 0
 
     from q3_amd import FP64, MIXED, Context, PlanewaveDavidsonDiagonalizer, PlanewaveHamiltonianApplicator
     from q3_amd import workloads as W
@@ -125,6 +176,19 @@ def main():
     h.prepare_kpoint(kp)
     h.set_projectors(beta, bs, db)
     L = lib()
+    if args.mode == "davidson":
+        precs = {"fp64": [FP64], "mixed": [MIXED], "both": [FP64, MIXED]}[args.precision]
+        rep = {}
+        for prec in precs:
+            d = PlanewaveDavidsonDiagonalizer(h)
+            d.set_tolerance(2e-6)
+            d.set_maximum_iterations(args.davidson_iters)
+            ctx.synchronize()
+            t = time.perf_counter()
+            d.solve(nb, w.noccupied, psi0, precision=prec)
+            rep["fp64" if prec == FP64 else "mixed"] = {"s_per_solve": time.perf_counter() - t, **d.report_dict()}
+        print(json.dumps({"mode": "davidson", **rep}))
+        return 0
     nbytes = npw * nb * 16
     d_psi, d_hpsi = ctypes.c_void_p(), ctypes.c_void_p()
     check(L.q3x_device_alloc(ctx.h, nbytes, ctypes.byref(d_psi)))
@@ -228,6 +292,8 @@ def main():
             davidson = {"error": str(e)}
             log(f"davidson failed: {e}")
 
+    batched = None if args.no_batch else batched_kpoints(args)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -267,6 +333,7 @@ def main():
         "kernels": per_kernel,
         "roofline": roof,
         "davidson": davidson,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

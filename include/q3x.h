@@ -188,6 +188,26 @@ void q3x_davidson_default_options(q3x_davidson_options* opt);
 q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
                               double* psi_inout, double* eig_out, q3x_davidson_report* report);
 
+/* ---- many k-points on one GPU: replaces the SCF k-loop of
+ *      src/layer_4_computation/iterator/kohn_sham/planewave/pseudopotential/init.hpp:1056-1082
+ *      (and the round-robin KPointDistributor, .../planewave/internal/kpoint_distributor.hpp)
+ *      for independent k-points: k-point i runs on worker i % workers, each worker with its own
+ *      context (stream, workspace) and host thread, so the workers' solves overlap. */
+typedef struct q3x_batch_s* q3x_batch;
+q3x_status q3x_batch_create(int device, int workers, q3x_batch* out);
+q3x_status q3x_batch_destroy(q3x_batch b);
+/* one k-point: a Hamiltonian with V(r) (as q3x_hamiltonian_set_potential), prepared for kp,
+ * with its projector set (as q3x_hamiltonian_set_projectors; nproj may be 0).  *index = its
+ * position in the batch. */
+q3x_status q3x_batch_add(q3x_batch b, q3x_basis basis, q3x_kpoint kp, const double* v_r, long n,
+                         const double* beta, int nproj, int natoms, const int* block_sizes,
+                         const double* d_blocks, int* index);
+int q3x_batch_size(q3x_batch b);
+/* solves every k-point with the same options: psi_inout[i] host npw_i x nb, eig_out nk x nb
+ * (row i = k-point i), reports (nk entries) may be null.  Returns the first failure. */
+q3x_status q3x_batch_solve(q3x_batch b, int nb, const q3x_davidson_options* opt,
+                           double* const* psi_inout, double* eig_out, q3x_davidson_report* reports);
+
 #ifdef __cplusplus
 }
 #endif

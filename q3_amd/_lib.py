@@ -64,6 +64,11 @@ SIGNATURES = {
     "q3x_hegv_device": (_i, [_p, _i, _p, _l, _p, _l, _i, _p, _p, _l]),
     "q3x_davidson_default_options": (None, [ctypes.POINTER(DavidsonOptions)]),
     "q3x_davidson_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p, ctypes.POINTER(DavidsonReport)]),
+    "q3x_batch_create": (_i, [_i, _i, _pp]),
+    "q3x_batch_destroy": (_i, [_p]),
+    "q3x_batch_add": (_i, [_p, _p, _p, _p, _l, _p, _i, _i, _p, _p, ctypes.POINTER(_i)]),
+    "q3x_batch_size": (_i, [_p]),
+    "q3x_batch_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _pp, _p, ctypes.POINTER(DavidsonReport)]),
 }
 
 _lib = None

@@ -120,15 +120,24 @@ bool build_tables(FftTables& t, int n, hipStream_t st) {
         set_error("FFT length is not 2,3,5-smooth");
         return false;
     }
-    std::vector<double> d(2 * n);
-    std::vector<float> f(2 * n);
+    // stage-major twiddles: stage s (radix R, stride ns) holds [r-1][jn] = exp(-2 pi i r jn / (ns R))
+    std::vector<double> d(2 * n, 0.0);
+    std::vector<float> f(2 * n, 0.0f);
     const long double two_pi = 6.283185307179586476925286766559005768L;
-    for (int k = 0; k < n; ++k) {
-        const long double a = -two_pi * k / n;
-        d[2 * k] = static_cast<double>(cosl(a));
-        d[2 * k + 1] = static_cast<double>(sinl(a));
-        f[2 * k] = static_cast<float>(d[2 * k]);
-        f[2 * k + 1] = static_cast<float>(d[2 * k + 1]);
+    int off = 0, ns = 1;
+    for (int s = 0; s < t.plan.nstage; ++s) {
+        const int R = t.plan.radix[s];
+        t.plan.twoff[s] = off;
+        for (int r = 1; r < R; ++r)
+            for (int jn = 0; jn < ns; ++jn, ++off) {
+                const long k = static_cast<long>(r) * jn * (n / (ns * R));
+                const long double a = -two_pi * static_cast<long double>(k % n) / n;
+                d[2 * off] = static_cast<double>(cosl(a));
+                d[2 * off + 1] = static_cast<double>(sinl(a));
+                f[2 * off] = static_cast<float>(d[2 * off]);
+                f[2 * off + 1] = static_cast<float>(d[2 * off + 1]);
+            }
+        ns *= R;
     }
     return t.tw64.upload(d.data(), d.size() * sizeof(double), st) &&
            t.tw32.upload(f.data(), f.size() * sizeof(float), st);
@@ -485,9 +494,11 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
     // plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of 8 waves
     for (int fp32 = 0; fp32 < 2; ++fp32) {
         const size_t cs = fp32 ? 8 : 16;
-        const size_t fixed = cs * (n1 + n2 + static_cast<size_t>(n2) * S.ncol);
+        const int ps = fp32 ? lds_pad_shift<cx<float>>() : lds_pad_shift<cx<double>>();
+        const size_t l1 = padded_len(n1, ps), l2 = padded_len(n2, ps);
+        const size_t fixed = cs * (n1 + n2 + l2 * S.ncol);
         long rp = 0;
-        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * n1 * 8));
+        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * l1 * 8));
         rp = std::min<long>(rp, 4);
         rp = std::min<long>(rp, n2);
         if (rp < 1) {

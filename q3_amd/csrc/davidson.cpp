@@ -42,13 +42,19 @@ constexpr double kEmptyFloor = 1e-5;      // tolerances.hpp:66
 using clk = std::chrono::steady_clock;
 
 struct Solver {
+    explicit Solver(q3x_hamiltonian h_)
+        : h(h_), st(h_->ctx->stream), ws(h_->ctx->dav), V(ws.V), HV(ws.HV), R(ws.R), C(ws.C), G(ws.G),
+          Hs_d(ws.Hs), Ss_d(ws.Ss), X_d(ws.X), Xc_d(ws.Xc), eps_d(ws.eps), negeps_d(ws.negeps), work(ws.work),
+          norms_d(ws.norms), idx_d(ws.idx), eigw(ws.eigw) {}
     q3x_hamiltonian h;
     hipStream_t st;
-    int npw, nb, nocc, mmax;
-    int precision;
-    // X_d: subspace eigenvectors (n x nvec, ld n) of the last diag(); Xc_d: their columns for
-    // the corrections; idx_d / eps_d / negeps_d: band indices and energies of the corrections
-    DevBuf V, HV, R, C, G, Hs_d, Ss_d, X_d, Xc_d, eps_d, negeps_d, work, norms_d, idx_d, eigw;
+    int npw = 0, nb = 0, nocc = 0, mmax = 0;
+    int precision = Q3X_FP64;
+    // buffers of the context's workspace.  X_d: subspace eigenvectors (n x nvec, ld n) of the
+    // last diag(); Xc_d: their columns for the corrections; idx_d / eps_d / negeps_d: band
+    // indices and energies of the corrections
+    DavidsonWorkspace& ws;
+    DevBuf &V, &HV, &R, &C, &G, &Hs_d, &Ss_d, &X_d, &Xc_d, &eps_d, &negeps_d, &work, &norms_d, &idx_d, &eigw;
     size_t work_bytes = size_t(256) << 20;
     int h_applications = 0;
     double stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -240,9 +246,7 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
         set_error("q3x_davidson_solve: need 0 < nb <= npw, basis_factor >= 1, a valid precision");
         return Q3X_ERR_ARGUMENT;
     }
-    Solver s;
-    s.h = h;
-    s.st = h->ctx->stream;
+    Solver s(h);
     s.npw = h->npw;
     s.nb = nb;
     s.nocc = std::max(0, std::min(opt->n_occupied, nb));

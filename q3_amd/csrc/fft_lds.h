@@ -7,7 +7,7 @@
 // itself is an unrolled register DFT (composite radices as two levels of small DFTs with
 // constant inner twiddles).  Every thread loads its butterflies of a round into registers,
 // barrier, writes back, barrier; a round holds whole transforms only.
-// Twiddles come from a table tw[k] = exp(-2 pi i k / n) (host-computed in long double);
+// Twiddles come from a stage-major table (FftPlan::twoff; host-computed in long double);
 // sign = -1 forward (e^{-i}), +1 backward (e^{+i}), no scaling -- the FFTW convention the
 // reference uses (fft_grid_serial.hpp:209-223).
 #pragma once
@@ -37,11 +37,31 @@ __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s
 template <typename T>
 __device__ __forceinline__ cx<T> cmuli(cx<T> a, T s) { return {-a.y * s, a.x * s}; }
 
+// Stages in order with their radices; the twiddles of stage s (stride ns = product of the
+// earlier radices) are stored stage-major at twoff[s] as [r-1][jn] = exp(-2 pi i r jn / (ns R)),
+// so that neighbouring butterflies read neighbouring entries.  The table holds n - 1 entries.
 struct FftPlan {
     int n;
     int nstage;
     int radix[8];
+    int twoff[8];
 };
+
+// LDS padding of a transform's storage: element e lives at e + (e >> PAD).  The first
+// Stockham stages write with a stride of R elements, and transposed accesses walk across
+// transforms; without padding both hit the same banks on every lane.  One pad slot per 128
+// bytes of data: PAD = 3 for complex double (16 B), 4 for complex float; PAD = 31 disables it.
+template <int PAD>
+__device__ __forceinline__ int pidx(int e) {
+    return e + (e >> PAD);
+}
+template <typename T>
+constexpr int lds_pad_shift() {
+    return sizeof(T) == 8 ? 3 : 4;
+}
+// storage length of one padded n-point transform; the extra slot staggers consecutive
+// transforms across banks
+__host__ __device__ constexpr int padded_len(int n, int shift) { return n + (n >> shift) + 1; }
 
 // ---- small prime-size DFTs: y_q = sum_r x_r exp(sgn * 2 pi i q r / R), in place
 template <typename T>
@@ -167,8 +187,9 @@ __device__ __forceinline__ void stage_sync() {
 }
 
 // One Stockham stage of radix R over ntrans transforms of length n stored at
-// buf[t * ts + e * es].  BPT = transforms per thread per round.  Requires m = n/R <= nthr.
-template <typename T, int R, int BPT, bool WAVE = false>
+// buf[t * ts + pidx<PAD>(e * es)].  BPT = transforms per thread per round.  Requires
+// m = n/R <= nthr.
+template <typename T, int R, int BPT, bool WAVE = false, int PAD = 31>
 __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int ntrans, int es, int ts, int tid,
                           int nthr) {
     const int m = n / R;
@@ -178,19 +199,20 @@ __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int
     const bool active = tg < groups;
     const int jn = j % ns;
     const int base = (j - jn) * R + jn;  // (j / ns) * ns * R + jn
-    const int twb = jn * (n / (ns * R));
+    const cx<T>* tws = tw + jn;          // stage table: entry (r - 1) * ns + jn
     for (int t0 = 0; t0 < ntrans; t0 += groups * BPT) {
         cx<T> v[BPT][R];
 #pragma unroll
         for (int b = 0; b < BPT; ++b) {
             const int t = t0 + tg + b * groups;
             if (active && t < ntrans) {
-                const cx<T>* src = buf + t * ts + j * es;
-                v[b][0] = src[0];
+                const cx<T>* src = buf + t * ts;
+                v[b][0] = src[pidx<PAD>(j * es)];
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    const cx<T> w = tw[r * twb];  // LDS twiddle table, fetched per use
-                    v[b][r] = sgn < T(0) ? cmul(src[r * m * es], w) : cmulc(src[r * m * es], w);
+                    const cx<T> w = tws[(r - 1) * ns];  // LDS twiddle table, fetched per use
+                    const cx<T> x = src[pidx<PAD>((j + r * m) * es)];
+                    v[b][r] = sgn < T(0) ? cmul(x, w) : cmulc(x, w);
                 }
                 Dft<T, R>::run(v[b], sgn);
             }
@@ -200,9 +222,9 @@ __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int
         for (int b = 0; b < BPT; ++b) {
             const int t = t0 + tg + b * groups;
             if (active && t < ntrans) {
-                cx<T>* dst = buf + t * ts + base * es;
+                cx<T>* dst = buf + t * ts;
 #pragma unroll
-                for (int q = 0; q < R; ++q) dst[q * ns * es] = v[b][q];
+                for (int q = 0; q < R; ++q) dst[pidx<PAD>((base + q * ns) * es)] = v[b][q];
             }
         }
         stage_sync<WAVE>();
@@ -212,20 +234,21 @@ __device__ void fft_stage(cx<T>* buf, int n, int ns, const cx<T>* tw, T sgn, int
 
 // Runs the plan's stages.  WAVE = true: the calling wavefront alone owns these transforms
 // (tid = lane, nthr = 64) and no block barrier is used.
-template <typename T, int BPT, bool WAVE = false>
+template <typename T, int BPT, bool WAVE = false, int PAD = 31>
 __device__ void fft_lds(cx<T>* buf, const FftPlan& p, const cx<T>* tw, int sign, int ntrans, int es, int ts,
                         int tid, int nthr) {
     const T sgn = sign < 0 ? T(-1) : T(1);
     int ns = 1;
     for (int s = 0; s < p.nstage; ++s) {
         const int R = p.radix[s];
+        const cx<T>* ts_tw = tw + p.twoff[s];
         switch (R) {
-            case 2: fft_stage<T, 2, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 3: fft_stage<T, 3, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 4: fft_stage<T, 4, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 5: fft_stage<T, 5, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 6: fft_stage<T, 6, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
-            case 8: fft_stage<T, 8, BPT, WAVE>(buf, p.n, ns, tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 2: fft_stage<T, 2, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 3: fft_stage<T, 3, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 4: fft_stage<T, 4, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 5: fft_stage<T, 5, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 6: fft_stage<T, 6, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
+            case 8: fft_stage<T, 8, BPT, WAVE, PAD>(buf, p.n, ns, ts_tw, sgn, ntrans, es, ts, tid, nthr); break;
             default: break;
         }
         ns *= R;

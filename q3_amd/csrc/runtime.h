@@ -61,12 +61,20 @@ struct FftTables {
     DevBuf tw64, tw32;
 };
 
+// Device buffers of the block Davidson (the reference's SerialDavidsonWorkspace,
+// davidson.hpp:120-221), kept by the context and grown on demand, so that repeated solves on a
+// context allocate nothing once the largest size has been seen.
+struct DavidsonWorkspace {
+    DevBuf V, HV, R, C, G, Hs, Ss, X, Xc, eps, negeps, work, norms, idx, eigw;
+};
+
 }  // namespace q3x
 
 struct q3x_context_s {
     q3x::Profiler prof;
     q3x::DevBuf gemm_work;  // split-K workspace of q3x_zgemm
     q3x::DevBuf eig_work;   // workspace of q3x_hegv_device
+    q3x::DavidsonWorkspace dav;
     int device = 0;
     hipStream_t stream = nullptr;
     size_t max_lds = 65536;

@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -163,102 +165,179 @@ __global__ void scale_matrix(long nn, double s, z2* __restrict__ A) {
 }
 
 // ---------------------------------------------------------------- 3. tridiagonalisation ----
-// Reflector of column k, rows [k+1, n): H = I - tau v v^H, v_0 = 1, H^H x = beta e_0.
-// v goes to vout (length n-k-1) and, below its leading 1, into column k rows k+2...
-// Returns tau (0: no reflector, x is already of the form beta e_0).
-__device__ double house(z2* M, int n, int k, z2* vout, double* red, int& slot, z2* beta_k1, double* tau_k) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int m = n - k - 1;
-    z2* x = M + (k + 1) + static_cast<long>(k) * n;
+// Reflector k: H_k = I - tau v v^H on rows [k+1, n), v_0 = 1, H_k^H x = beta e_0 for
+// x = column k rows [k+1, n).  Its vector is kept in column k rows k+2.. below the implicit 1.
+//
+// Step k applies the rank-2 update B -= v q^H + q v^H (q = p - (tau/2)(v^H p) v, p = tau B v)
+// to the trailing block.  The update of step k and the product p of step k+1 share one pass
+// over the block; column k+2 (whose reflector step k+2 needs) is finished by wavefront 0 while
+// the others form q.  Per step: one pass and three barriers.
+
+// Column access of the tridiagonalisation.  For the L2-resident matrix, loads and stores go
+// through a buffer resource bounded to the column's live rows: rows past the end read as 0 and
+// their stores are dropped by the hardware, so the row loops need no per-lane branches (with
+// branches the compiler serialises every store behind a full vmcnt wait).  The LDS-resident
+// matrix uses plain accesses with the same guard.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <bool LM>
+struct ColumnView {
+    z2* base;
+    int len;
+    __amdgpu_buffer_rsrc_t rs;
+    __device__ ColumnView(z2* b, int l) : base(b), len(l) {
+        if (!LM) rs = __builtin_amdgcn_make_buffer_rsrc(b, 0, l * 16, 0x00020000);
+    }
+    __device__ z2 load(int i) const {
+        if (LM) return i < len ? base[i] : z2{0.0, 0.0};
+        const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, i * 16, 0, 0);
+        z2 r;
+        __builtin_memcpy(&r, &v, 16);
+        return r;
+    }
+    __device__ void store(int i, z2 a) const {
+        if (LM) {
+            if (i < len) base[i] = a;
+            return;
+        }
+        u32x4 v;
+        __builtin_memcpy(&v, &a, 16);
+        __builtin_amdgcn_raw_buffer_store_b128(v, rs, i * 16, 0, 0);
+    }
+};
+
+// Wavefront 0: finish column j (rows [j, n)) with reflector j-1 (vn, q = p - f vn, if tn != 0),
+// record d[j], then form reflector j into vout (and column j's storage) when j <= n-3.
+// Returns tau_j (0 when there is none).  Lane l holds rows j + l + 64 t.
+template <int RPL, bool LM>
+__device__ double column_step(z2* M, int n, int j, double tn, const z2* vn, const z2* p, double f, z2* vout,
+                              double* d, z2* beta, double* tau) {
+    const int lane = threadIdx.x & 63;
+    const int L = n - j;
+    const ColumnView<LM> col(M + j + static_cast<long>(j) * n, L);
+    z2 a[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) a[t] = col.load(lane + 64 * t);
+    if (tn != 0.0) {
+        const z2 v0 = vn[0];
+        const z2 q0 = {p[0].x - f * v0.x, p[0].y - f * v0.y};
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int i = lane + 64 * t;
+            const int ic = min(i, L - 1);  // LDS vectors hold L entries
+            const z2 vi = vn[ic];
+            const z2 qi = {p[ic].x - f * vi.x, p[ic].y - f * vi.y};
+            const z2 t1 = zmulc(vi, q0), t2 = zmulc(qi, v0);
+            a[t].x -= t1.x + t2.x;
+            a[t].y -= t1.y + t2.y;
+            col.store(i, a[t]);
+            if (i >= L) a[t] = {0.0, 0.0};
+        }
+    }
+    if (lane == 0) d[j] = a[0].x;
+    const z2 x0 = {__shfl(a[0].x, 1, 64), __shfl(a[0].y, 1, 64)};  // row j+1
+    if (j == n - 2) {
+        if (lane == 0) beta[n - 1] = x0;
+        return 0.0;
+    }
+    if (j > n - 3) return 0.0;
     double part = 0.0;
-    for (int i = tid + 1; i < m; i += nt) part += x[i].x * x[i].x + x[i].y * x[i].y;
-    const double tail = block_sum(part, red, slot);
-    const z2 x0 = x[0];
-    double t = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+        const int i = lane + 64 * t;
+        if (i >= 2) part += a[t].x * a[t].x + a[t].y * a[t].y;  // rows past the end hold 0
+    }
+    const double tail = wave_sum(part);
+    double tj = 0.0;
     if (tail == 0.0) {
-        if (tid == 0) *beta_k1 = x0;
+        if (lane == 0) beta[j + 1] = x0;
     } else {
         const double a0 = hypot(x0.x, x0.y);
         const double xnorm = sqrt(tail + a0 * a0);
         const z2 ph = a0 == 0.0 ? z2{1.0, 0.0} : z2{x0.x / a0, x0.y / a0};
         const z2 b = {-ph.x * xnorm, -ph.y * xnorm};
         const z2 u0 = {x0.x - b.x, x0.y - b.y};
-        const double u2 = u0.x * u0.x + u0.y * u0.y;
-        const double iu2 = 1.0 / u2;
-        t = 2.0 / (1.0 + tail * iu2);
-        for (int i = tid; i < m; i += nt) {
-            const z2 xi = x[i];
-            vout[i] = i == 0 ? z2{1.0, 0.0} : z2{(xi.x * u0.x + xi.y * u0.y) * iu2, (xi.y * u0.x - xi.x * u0.y) * iu2};
+        const double iu2 = 1.0 / (u0.x * u0.x + u0.y * u0.y);
+        tj = 2.0 / (1.0 + tail * iu2);
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int i = lane + 64 * t;
+            const z2 w = i == 1 ? z2{1.0, 0.0}
+                                : z2{(a[t].x * u0.x + a[t].y * u0.y) * iu2, (a[t].y * u0.x - a[t].x * u0.y) * iu2};
+            if (i >= 1 && i < L) vout[i - 1] = w;
+            if (i >= 2) col.store(i, w);
         }
-        if (tid == 0) *beta_k1 = b;
+        if (lane == 0) beta[j + 1] = b;
     }
-    if (tid == 0) *tau_k = t;
-    __syncthreads();  // vout complete; every read of x finished
-    if (t != 0.0)
-        for (int i = tid + 1; i < m; i += nt) x[i] = vout[i];
-    return t;
+    if (lane == 0) tau[j] = tj;
+    return tj;
 }
 
-// Fused pass of step k over columns j in [k+2, n), rows [k+2, n):
-//   a -= v_i conj(q_j) + q_i conj(v_j)     (rank-2 update of step k; v, q index rows from k+1)
-//   p[j-k-2] = tn * sum_i conj(a_i) vn_i   (product of the updated block with v_{k+1})
-// Column j is owned by wavefront (j - k - 2) mod nw; lanes run over rows.
-__device__ void fused_pass(z2* M, int n, int k, double tk, const z2* v, const z2* q, double tn, const z2* vn, z2* p) {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int len = n - k - 2;  // rows and columns of the pass
-    for (int c = wv; c < len; c += nw) {
-        z2* col = M + static_cast<long>(k + 2 + c) * n + (k + 2);
-        const z2 qj = tk != 0.0 ? q[c + 1] : z2{0.0, 0.0};
-        const z2 vj = tk != 0.0 ? v[c + 1] : z2{0.0, 0.0};
-        double sr = 0.0, si = 0.0;
-        for (int r = lane; r < len; r += 64) {
-            z2 a = col[r];
-            if (tk != 0.0) {
-                const z2 t1 = zmulc(v[r + 1], qj), t2 = zmulc(q[r + 1], vj);
-                a.x -= t1.x + t2.x;
-                a.y -= t1.y + t2.y;
-                col[r] = a;
+// Pass of step k over columns / rows [k+2, n): rank-2 update with (v, q) when tk != 0, then
+// p[c] = tn * (column)^H vn when tn != 0.  Wavefront w takes CB adjacent columns at a time and
+// issues all their loads before using them.  v, q index rows from k+1; vn and p from k+2.
+template <int RPL, int CB, bool LM>
+__device__ void fused_pass(z2* M, int n, int k, double tk, const z2* v, const z2* q, double tn, const z2* vn,
+                           z2* p) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
+    const int len = n - k - 2, base = k + 2;
+    for (int c0 = wv * CB; c0 < len; c0 += nw * CB) {
+        z2 a[CB][RPL];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = min(c0 + b, len - 1);  // a missing column re-reads the last one (unused)
+            const ColumnView<LM> col(M + static_cast<long>(base + c) * n + base, len);
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) a[b][t] = col.load(lane + 64 * t);
+        }
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            const int c = c0 + b;
+            if (c >= len) break;
+            const ColumnView<LM> col(M + static_cast<long>(base + c) * n + base, len);
+            const z2 qj = tk != 0.0 ? q[c + 1] : z2{0.0, 0.0};
+            const z2 vj = tk != 0.0 ? v[c + 1] : z2{0.0, 0.0};
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = lane + 64 * t;
+                const int rc = min(r, len - 1);  // LDS vectors: stay in range; rows past the end are 0
+                if (tk != 0.0) {
+                    const z2 t1 = zmulc(v[rc + 1], qj), t2 = zmulc(q[rc + 1], vj);
+                    a[b][t].x -= t1.x + t2.x;
+                    a[b][t].y -= t1.y + t2.y;
+                    col.store(r, a[b][t]);
+                }
+                const z2 w = vn[rc];
+                const double msk = r < len ? 1.0 : 0.0;
+                sr += msk * (a[b][t].x * w.x + a[b][t].y * w.y);
+                si += msk * (a[b][t].x * w.y - a[b][t].y * w.x);
             }
             if (tn != 0.0) {
-                const z2 w = vn[r];
-                sr += a.x * w.x + a.y * w.y;
-                si += a.x * w.y - a.y * w.x;
+                sr = wave_sum(sr);
+                si = wave_sum(si);
+                if (lane == 0) p[c] = {tn * sr, tn * si};
             }
         }
-        if (tn != 0.0) {
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            if (lane == 0) p[c] = {tn * sr, tn * si};
-        }
     }
-    __syncthreads();
-}
-
-// q = p - (tn/2) (vn^H p) vn over len entries
-__device__ void form_q(int len, double tn, const z2* vn, const z2* p, z2* q, double* red, int& slot) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    double part = 0.0;
-    for (int i = tid; i < len; i += nt) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
-    const double s = block_sum(part, red, slot);
-    const double f = 0.5 * tn * s;
-    for (int i = tid; i < len; i += nt) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
-    __syncthreads();
 }
 
 // A (n x n, ld n, full storage, Hermitian up to rounding) -> d (diagonal), beta (sub-diagonal,
 // beta[i] couples i-1 and i), tau (reflectors, kept in A below the sub-diagonal), the real
-// tridiagonal |beta| in eabs and the phases D in phase (T = D Tr D^H).
-template <bool LM>
+// tridiagonal |beta| in eabs and the phases D in phase (T = D Tr D^H).  RPL >= ceil(n / 64).
+template <bool LM, int RPL>
 __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double* d, z2* beta, double* tau,
                                                            z2* phase, double* eabs) {
+    constexpr int CB = RPL <= 2 ? 4 : (RPL <= 4 ? 2 : 1);
     extern __shared__ z2 sh[];
     z2* vb0 = sh;
     z2* vb1 = sh + n;
     z2* q = sh + 2 * n;
     z2* p = sh + 3 * n;
     double* red = reinterpret_cast<double*>(sh + 4 * n);  // 32 doubles
-    z2* M = LM ? sh + 4 * n + 16 : A;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    double& tnext = *reinterpret_cast<double*>(sh + 4 * n + 16);
+    z2* M = LM ? sh + 4 * n + 17 : A;
+    const int tid = threadIdx.x, nt = blockDim.x, wv = tid >> 6;
     const long nn = static_cast<long>(n) * n;
     int slot = 0;
     if (LM) {
@@ -277,58 +356,50 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
             M[e].y = 0.0;
         }
     }
-    __syncthreads();
     if (tid == 0) {
-        d[0] = M[0].x;
         beta[0] = {0.0, 0.0};
         for (int k = 0; k < n; ++k) tau[k] = 0.0;
     }
-    if (n == 2 && tid == 0) {
-        d[1] = M[3].x;
-        beta[1] = M[1];
-    }
-    if (n >= 3) {
-        // prologue: reflector 0, p_0 = t0 B v_0, q_0
-        double tk = house(M, n, 0, vb0, red, slot, &beta[1], &tau[0]);
-        if (tk != 0.0) {
-            fused_pass(M, n, -1, 0.0, nullptr, nullptr, tk, vb0, p);
-            form_q(n - 1, tk, vb0, p, q, red, slot);
+    __syncthreads();
+    if (n <= 2) {
+        if (tid == 0) {
+            d[0] = M[0].x;
+            if (n == 2) {
+                d[1] = M[3].x;
+                beta[1] = M[1];
+            }
         }
-        int cur = 0;
-        for (int k = 0; k <= n - 3; ++k) {
-            const int m = n - k - 1;
-            z2* v = cur ? vb1 : vb0;
-            z2* vn = cur ? vb0 : vb1;
-            // column k+1, rows [k+1, n): rank-2 update of step k
-            if (tk != 0.0) {
-                const z2 q0 = q[0], v0 = v[0];
-                z2* col = M + (k + 1) + static_cast<long>(k + 1) * n;
-                for (int i = tid; i < m; i += nt) {
-                    const z2 t1 = zmulc(v[i], q0), t2 = zmulc(q[i], v0);
-                    col[i].x -= t1.x + t2.x;
-                    col[i].y -= t1.y + t2.y;
-                }
-                __syncthreads();
+    } else {
+        // reflector 0 (column 0 needs no update)
+        if (wv == 0) {
+            const double t0 = column_step<RPL, LM>(M, n, 0, 0.0, nullptr, nullptr, 0.0, vb0, d, beta, tau);
+            if ((tid & 63) == 0) tnext = t0;
+        }
+        __syncthreads();
+        double tk = 0.0, tn = tnext;
+        z2* v = vb1;   // reflector k (not yet defined at k = -1)
+        z2* vn = vb0;  // reflector k+1
+        for (int k = -1; k <= n - 3; ++k) {
+            if (tk != 0.0 || tn != 0.0) fused_pass<RPL, CB, LM>(M, n, k, tk, v, q, tn, vn, p);
+            __syncthreads();
+            double f = 0.0;
+            const int len = n - k - 2;
+            if (tn != 0.0) {
+                double part = 0.0;
+                for (int i = tid; i < len; i += nt) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
+                f = 0.5 * tn * block_sum(part, red, slot);
+                for (int i = tid; i < len; i += nt) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
             }
-            if (tid == 0) d[k + 1] = M[(k + 1) + static_cast<long>(k + 1) * n].x;
-            if (k == n - 3) {  // last 2 x 2 block
-                if (tid == 0) {
-                    z2* a = M + (n - 1) + static_cast<long>(n - 1) * n;
-                    if (tk != 0.0) {
-                        const z2 t1 = zmulc(v[1], q[1]), t2 = zmulc(q[1], v[1]);
-                        a->x -= t1.x + t2.x;
-                        a->y -= t1.y + t2.y;
-                    }
-                    d[n - 1] = a->x;
-                    beta[n - 1] = M[(n - 1) + static_cast<long>(n - 2) * n];
-                }
-                break;
+            if (wv == 0) {
+                const double tj = column_step<RPL, LM>(M, n, k + 2, tn, vn, p, f, v, d, beta, tau);
+                if ((tid & 63) == 0) tnext = tj;
             }
-            const double tn = house(M, n, k + 1, vn, red, slot, &beta[k + 2], &tau[k + 1]);
-            if (tk != 0.0 || tn != 0.0) fused_pass(M, n, k, tk, v, q, tn, vn, p);
-            if (tn != 0.0) form_q(m - 1, tn, vn, p, q, red, slot);
+            __syncthreads();
             tk = tn;
-            cur ^= 1;
+            tn = tnext;
+            z2* t = v;
+            v = vn;
+            vn = t;
         }
     }
     __syncthreads();
@@ -465,39 +536,43 @@ __global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __
                 mu = m;
             }
             auto at = [nvec, c](int i) { return static_cast<long>(i) * nvec + c; };
-            // LU with partial pivoting of Tr - mu I: row i keeps (u1 diag, u2, u3 super-diagonals)
-            for (int i = 0; i < n; ++i) {
-                U1[at(i)] = d[i] - mu;
-                U2[at(i)] = i + 1 < n ? e[i + 1] : 0.0;
-                U3[at(i)] = 0.0;
-            }
+            // LU with partial pivoting of Tr - mu I: row i keeps (u1 diag, u2, u3 super-diagonals).
+            // The pending row i+1 is carried in registers, so the loop only stores.
+            double u1 = d[0] - mu, u2 = n > 1 ? e[1] : 0.0;
+#pragma unroll 4
             for (int i = 0; i + 1 < n; ++i) {
                 const double sub = e[i + 1];
-                double u1 = U1[at(i)];
+                const double b2 = d[i + 1] - mu, b3 = i + 2 < n ? e[i + 2] : 0.0;  // row i+1 as it stands
                 if (fabs(u1) >= fabs(sub)) {
-                    SW[at(i)] = 0;
                     if (fabs(u1) < pivmin) u1 = copysign(pivmin, u1 == 0.0 ? 1.0 : u1);
-                    U1[at(i)] = u1;
                     const double f = sub / u1;
+                    SW[at(i)] = 0;
                     LO[at(i)] = f;
-                    U1[at(i + 1)] -= f * U2[at(i)];
+                    U1[at(i)] = u1;
+                    U2[at(i)] = u2;
+                    U3[at(i)] = 0.0;
+                    u1 = b2 - f * u2;
+                    u2 = b3;
                 } else {
-                    SW[at(i)] = 1;
                     const double f = u1 / sub;
+                    SW[at(i)] = 1;
                     LO[at(i)] = f;
-                    const double a2 = U2[at(i)], b2 = U1[at(i + 1)], b3 = U2[at(i + 1)];
                     U1[at(i)] = sub;
                     U2[at(i)] = b2;
                     U3[at(i)] = b3;
-                    U1[at(i + 1)] = a2 - f * b2;
-                    U2[at(i + 1)] = -f * b3;
+                    u1 = u2 - f * b2;
+                    u2 = -f * b3;
                 }
             }
-            if (fabs(U1[at(n - 1)]) < pivmin) U1[at(n - 1)] = copysign(pivmin, U1[at(n - 1)] == 0.0 ? 1.0 : U1[at(n - 1)]);
+            if (fabs(u1) < pivmin) u1 = copysign(pivmin, u1 == 0.0 ? 1.0 : u1);
+            U1[at(n - 1)] = u1;
+            U2[at(n - 1)] = 0.0;
+            U3[at(n - 1)] = 0.0;
             for (int i = 0; i < n; ++i) Zt[at(i)] = 1.0 + 0.1 * sin(1.3 * i + 0.7 * c);
             for (int it = 0; it < 3; ++it) {
                 // forward: L^-1 with the recorded interchanges
                 double xi = Zt[at(0)];
+#pragma unroll 8
                 for (int i = 0; i + 1 < n; ++i) {
                     double xn = Zt[at(i + 1)];
                     if (SW[at(i)]) {
@@ -511,6 +586,7 @@ __global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __
                 Zt[at(n - 1)] = xi;
                 // backward: U x = y
                 double x1 = 0.0, x2 = 0.0;  // x[i+1], x[i+2]
+#pragma unroll 8
                 for (int i = n - 1; i >= 0; --i) {
                     const double s = Zt[at(i)] - U2[at(i)] * x1 - U3[at(i)] * x2;
                     const double xv = s / U1[at(i)];
@@ -616,7 +692,7 @@ Layout layout(int n, int nvec) {
     return L;
 }
 
-size_t tridiag_lds(int n, bool lm) { return (4 * static_cast<size_t>(n) + 16 + (lm ? static_cast<size_t>(n) * n : 0)) * 16; }
+size_t tridiag_lds(int n, bool lm) { return (4 * static_cast<size_t>(n) + 17 + (lm ? static_cast<size_t>(n) * n : 0)) * 16; }
 
 }  // namespace
 
@@ -693,23 +769,35 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     // tridiagonalisation
     double *d = D(B_D), *e = D(B_E), *tau = D(B_TAU), *w = D(B_W);
     z2 *beta = P(B_BETA), *ph = P(B_PH);
-    const bool lm = tridiag_lds(n, true) <= max_lds;
+    const int rows_lane = (n + 63) / 64;
+    const bool lm = rows_lane <= 2 && tridiag_lds(n, true) <= max_lds;
     const size_t tl = tridiag_lds(n, lm);
-    if (tl > max_lds) {
+    if (tl > max_lds || rows_lane > 16) {
         *err = hipErrorInvalidValue;
         return -1;
     }
-    static bool lds_raised = false;
-    if (!lds_raised && tl > 65536) {
-        if (!ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true>),
+    static std::atomic<bool> lds_raised{false};  // set once per process; the attribute call is idempotent
+    if (!lds_raised && max_lds > 65536) {
+        if (!ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 1>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))) ||
+            !ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 2>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))))
             return -1;
         lds_raised = true;
     }
-    if (lm)
-        hipLaunchKernelGGL(tridiag_kernel<true>, dim3(1), dim3(kThreads), tl, st, C, n, d, beta, tau, ph, e);
-    else
-        hipLaunchKernelGGL(tridiag_kernel<false>, dim3(1), dim3(kThreads), tl, st, C, n, d, beta, tau, ph, e);
+#define Q3X_TRD(LMV, R) \
+    hipLaunchKernelGGL((tridiag_kernel<LMV, R>), dim3(1), dim3(kThreads), tl, st, C, n, d, beta, tau, ph, e)
+    if (lm) {
+        if (rows_lane <= 1) Q3X_TRD(true, 1);
+        else Q3X_TRD(true, 2);
+    } else if (rows_lane <= 1) Q3X_TRD(false, 1);
+    else if (rows_lane <= 2) Q3X_TRD(false, 2);
+    else if (rows_lane <= 4) Q3X_TRD(false, 4);
+    else if (rows_lane <= 6) Q3X_TRD(false, 6);
+    else if (rows_lane <= 8) Q3X_TRD(false, 8);
+    else if (rows_lane <= 12) Q3X_TRD(false, 12);
+    else Q3X_TRD(false, 16);
+#undef Q3X_TRD
     if (!ok(hipGetLastError())) return -1;
     hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(64), static_cast<size_t>(n) * 16, st, n, d, e, w);
     if (!ok(hipGetLastError())) return -1;

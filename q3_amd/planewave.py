@@ -303,3 +303,63 @@ class PlanewaveDavidsonDiagonalizer:
         return {"iterations": r.iterations, "unconverged": r.unconverged, "h_applications": r.h_applications,
                 "fp64_refinements": r.fp64_refinements, "max_residual_norm": r.max_residual_norm,
                 "stage_ms": {n: r.stage_ms[i] for i, n in enumerate(names)}}
+
+
+class KPointBatch:
+    """Independent k-points solved together on one GPU (replaces the SCF k-loop of
+    iterator/kohn_sham/planewave/pseudopotential/init.hpp:1056-1082 for independent k-points):
+    k-point i runs on worker i % workers, each worker with its own stream and host thread."""
+
+    def __init__(self, workers: int = 8, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().q3x_batch_create(device, workers, ctypes.byref(h)))
+        self.h = h
+        self.workers = workers
+        self.npw = []
+        self._keep = []  # the batch's Hamiltonians refer to these bases and k-point lists
+        self.options = _lib.DavidsonOptions()
+        lib().q3x_davidson_default_options(ctypes.byref(self.options))
+        self.reports = []
+
+    def add(self, basis: PlanewaveBasis, kpoint: KPoint, v_r: np.ndarray, beta: np.ndarray, block_sizes,
+            d_blocks) -> int:
+        v = np.ascontiguousarray(v_r, dtype=np.float64).ravel()
+        b = np.asfortranarray(beta, dtype=np.complex128)
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        db = np.ascontiguousarray(d_blocks, dtype=np.float64)
+        idx = ctypes.c_int()
+        check(lib().q3x_batch_add(self.h, basis.h, kpoint.h, _ptr(v), v.size, _ptr(b), b.shape[1], len(bs), _ptr(bs),
+                                  _ptr(db), ctypes.byref(idx)))
+        self.npw.append(kpoint.npw)
+        self._keep.append((basis, kpoint))
+        return idx.value
+
+    def solve(self, number_of_bands: int, number_of_occupied: int, tolerance_ry: float, max_iterations: int = 20,
+              precision: int = FP64, psi0=None, final_fp64_sweep: bool = True):
+        """Returns (eigenvalues nk x nb [Ry], list of psi_k (npw_k x nb))."""
+        nb = number_of_bands
+        o = self.options
+        o.tolerance_ry = tolerance_ry
+        o.max_iterations = max_iterations
+        o.n_occupied = number_of_occupied
+        o.precision = precision
+        o.final_fp64_sweep = int(final_fp64_sweep)
+        o.seed_from_psi = int(psi0 is not None)
+        if psi0 is None:
+            psis = [np.zeros((n, nb), np.complex128, order="F") for n in self.npw]
+        else:
+            psis = [np.asfortranarray(p, dtype=np.complex128).copy(order="F") for p in psi0]
+        ptrs = (ctypes.c_void_p * len(psis))(*[p.ctypes.data for p in psis])
+        eig = np.zeros((len(psis), nb))
+        reps = (_lib.DavidsonReport * len(psis))()
+        check(lib().q3x_batch_solve(self.h, nb, ctypes.byref(o), ptrs, _ptr(eig), reps))
+        self.reports = list(reps)
+        return eig, psis
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().q3x_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -53,7 +53,8 @@ def _check(H, S, w, X, nvec, tol=1e-12):
 
 
 @pytest.mark.parametrize("n,nvec", [(1, 1), (2, 2), (3, 3), (5, 2), (17, 17), (64, 32), (96, 48), (99, 99),
-                                    (100, 50), (160, 160), (250, 125), (320, 160)])
+                                    (100, 50), (160, 160), (250, 125), (320, 160), (450, 100),
+                                    (700, 64)])
 def test_hegv_device_near_identity_overlap(gpu_context, n, nvec):
     rng = np.random.default_rng(n)
     H = _herm(n, rng)
