@@ -146,7 +146,7 @@ def main():
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
     # one hardware queue per batch worker stream (HIP's default is 4), set before HIP starts
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(16, args.batch_workers))))
+    os.environ["GPU_MAX_HW_QUEUES"] = str(max(4, min(16, args.batch_workers)))
     if args.mode == "copy":
         return copy_reference(args.steps)
     if args.mode == "batch":
@@ -177,6 +177,9 @@ def main():
     h.set_projectors(beta, bs, db)
     L = lib()
     if args.mode == "davidson":
+        # profiling on: every Davidson stage ends with a stream synchronisation, so the reported
+        # stage times are device times (the timed solves of the full run leave it off)
+        check(L.q3x_profile_enable(ctx.h, 1))
         precs = {"fp64": [FP64], "mixed": [MIXED], "both": [FP64, MIXED]}[args.precision]
         rep = {}
         for prec in precs:

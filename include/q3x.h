@@ -179,7 +179,10 @@ typedef struct {
     /* stage times in ms, named like DavidsonTimingAccumulator
      * (src/layer_0_core/profiling/counters.hpp:36-52):
      * 0 seed, 1 orthogonalize, 2 hamiltonian_application, 3 subspace_update,
-     * 4 diagonalize, 5 collapse, 6 residual, 7 total */
+     * 4 diagonalize, 5 collapse, 6 residual, 7 total.  Device times when profiling is enabled
+     * on the context (q3x_profile_enable: each stage then ends with a stream synchronisation);
+     * otherwise the solve synchronises only where the host needs device data and the stage
+     * times are host-side. */
     double stage_ms[8];
 } q3x_davidson_report;
 
@@ -187,6 +190,10 @@ void q3x_davidson_default_options(q3x_davidson_options* opt);
 /* psi_inout: host, npw x nb complex; eig_out: nb (Ry, ascending). */
 q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
                               double* psi_inout, double* eig_out, q3x_davidson_report* report);
+/* the same with psi in HBM: d_psi_inout is a device buffer (npw x nb complex128, ld npw) on the
+ * context's device; it seeds the solve when opt->seed_from_psi and receives the eigenvectors. */
+q3x_status q3x_davidson_solve_device(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
+                                     void* d_psi_inout, double* eig_out, q3x_davidson_report* report);
 
 /* ---- many k-points on one GPU: replaces the SCF k-loop of
  *      src/layer_4_computation/iterator/kohn_sham/planewave/pseudopotential/init.hpp:1056-1082
@@ -207,6 +214,55 @@ int q3x_batch_size(q3x_batch b);
  * (row i = k-point i), reports (nk entries) may be null.  Returns the first failure. */
 q3x_status q3x_batch_solve(q3x_batch b, int nb, const q3x_davidson_options* opt,
                            double* const* psi_inout, double* eig_out, q3x_davidson_report* reports);
+
+/* ---- band-structure request server (continuous batching).  No counterpart in the reference,
+ *      which runs one calculation per process; it serves non-self-consistent band solves of the
+ *      reference's kind (flavors/non_self_consistent_field.hpp:78-112) for many cells at once.
+ *      Requests are admitted in arrival order while their estimated HBM footprint (the terms of
+ *      PlanewaveMemoryEstimate::compute, iterator/kohn_sham/planewave/internal/
+ *      memory_estimate.hpp:75-215) fits the budget; the k-points of admitted requests share the
+ *      workers (one context, stream and host thread each), so requests that arrive while others
+ *      run overlap with them on the device.  Wavefunctions stay in HBM until the result is
+ *      collected. */
+typedef struct q3x_server_s* q3x_server;
+typedef struct {
+    q3x_basis basis;          /* cell and cutoff; caller-owned, alive until q3x_server_wait returns */
+    const double* v_r;        /* V(r) on the basis' wave grid, as q3x_hamiltonian_set_potential */
+    long nv;
+    int nk;                   /* k-points, crystal coordinates, nk x 3 */
+    const double* kpoints;
+    int natoms;               /* atoms and species, as q3x_kb_projectors */
+    const double* atom_pos;
+    const int* atom_species;
+    int nspecies;
+    const int* nmesh;
+    const double* r_all;
+    const double* rab_all;
+    const int* nbeta;
+    const int* l_all;
+    const int* cutoff_all;
+    const double* beta_r_all;
+    const int* block_sizes;   /* D blocks per atom, as q3x_hamiltonian_set_projectors */
+    const double* d_blocks;
+    int nbands;
+    int noccupied;
+    q3x_davidson_options options;  /* tolerance, iterations, precision (seed_from_psi ignored) */
+} q3x_band_request;
+typedef struct {
+    double submit_s, admit_s, done_s;  /* seconds since the server started */
+    double footprint_bytes;            /* the admission estimate */
+    int iterations;                    /* Davidson iterations summed over the k-points */
+} q3x_request_stats;
+/* hbm_budget_bytes <= 0: 80% of the device memory free at creation */
+q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out);
+/* waits for every submitted request, then stops the workers */
+q3x_status q3x_server_destroy(q3x_server s);
+q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* req, double* bytes);
+/* copies the request's arrays and returns at once with a ticket */
+q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* req, long* ticket);
+/* blocks until the request is done: eig_out nk x nb (Ry, row k = k-point k); frees its state */
+q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_request_stats* stats);
+double q3x_server_in_flight_bytes(q3x_server s);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,18 @@ class DavidsonReport(ctypes.Structure):
                 ("max_residual_norm", _d), ("stage_ms", _d * 8)]
 
 
+class BandRequest(ctypes.Structure):
+    _fields_ = [("basis", _p), ("v_r", _p), ("nv", _l), ("nk", _i), ("kpoints", _p), ("natoms", _i),
+                ("atom_pos", _p), ("atom_species", _p), ("nspecies", _i), ("nmesh", _p), ("r_all", _p),
+                ("rab_all", _p), ("nbeta", _p), ("l_all", _p), ("cutoff_all", _p), ("beta_r_all", _p),
+                ("block_sizes", _p), ("d_blocks", _p), ("nbands", _i), ("noccupied", _i),
+                ("options", DavidsonOptions)]
+
+
+class RequestStats(ctypes.Structure):
+    _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/q3x.h
 SIGNATURES = {
     "q3x_last_error": (ctypes.c_char_p, []),
@@ -64,6 +76,14 @@ SIGNATURES = {
     "q3x_hegv_device": (_i, [_p, _i, _p, _l, _p, _l, _i, _p, _p, _l]),
     "q3x_davidson_default_options": (None, [ctypes.POINTER(DavidsonOptions)]),
     "q3x_davidson_solve": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p, ctypes.POINTER(DavidsonReport)]),
+    "q3x_davidson_solve_device": (_i, [_p, _i, ctypes.POINTER(DavidsonOptions), _p, _p,
+                                       ctypes.POINTER(DavidsonReport)]),
+    "q3x_server_create": (_i, [_i, _i, _d, _pp]),
+    "q3x_server_destroy": (_i, [_p]),
+    "q3x_server_estimate": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_d)]),
+    "q3x_server_submit": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_l)]),
+    "q3x_server_wait": (_i, [_p, _l, _p, ctypes.POINTER(RequestStats)]),
+    "q3x_server_in_flight_bytes": (_d, [_p]),
     "q3x_batch_create": (_i, [_i, _i, _pp]),
     "q3x_batch_destroy": (_i, [_p]),
     "q3x_batch_add": (_i, [_p, _p, _p, _p, _l, _p, _i, _i, _p, _p, ctypes.POINTER(_i)]),

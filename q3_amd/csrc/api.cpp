@@ -378,7 +378,7 @@ q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, cons
     hipError_t e = hipSuccess;
     const int info = subspace_eig(n, static_cast<const double*>(H), ldh, static_cast<const double*>(S), lds, nvec, w,
                                   static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
-                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->stream, &e, nullptr);
+                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr);
     if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
     if (info > 0) {
         set_error("q3x_hegv_device: S is not positive definite");

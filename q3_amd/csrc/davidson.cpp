@@ -58,6 +58,12 @@ struct Solver {
     size_t work_bytes = size_t(256) << 20;
     int h_applications = 0;
     double stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // With profiling on, every stage ends with a stream synchronisation so that stage_ms are
+    // device times; otherwise the solve synchronises only where the host needs device data
+    // (the Gram block of the Cholesky-QR, the subspace eigenvalues) and stage_ms measure the
+    // host side.
+    bool timing = false;
+    bool stage_sync() { return !timing || sync(); }
     q3x_status status = Q3X_OK;
 
     double* Vc(int c) { return V.as<double>() + 2L * c * npw; }
@@ -88,7 +94,7 @@ struct Solver {
             return false;
         }
         h_applications += n;
-        if (!sync()) return false;
+        if (!stage_sync()) return false;
         stage[2] += now_ms() - t0;
         return true;
     }
@@ -102,7 +108,7 @@ struct Solver {
         double* ss = Ss_d.as<double>() + 2L * start * mmax;
         if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, HVc(start), npw, 0.0, hs, mmax)) return false;
         if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, Vc(start), npw, 0.0, ss, mmax)) return false;
-        if (!sync()) return false;
+        if (!stage_sync()) return false;
         stage[3] += now_ms() - t0;
         return true;
     }
@@ -116,7 +122,7 @@ struct Solver {
         hipError_t e = hipSuccess;
         const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
                                       X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
-                                      h->ctx->max_lds, st, &e, nullptr);
+                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr);
         if (info < 0) {
             ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
             return -1;
@@ -176,7 +182,7 @@ struct Solver {
                 }
             }
         }
-        if (!sync()) return -1;
+        if (!stage_sync()) return -1;
         stage[1] += now_ms() - t0;
         return accepted;
     }
@@ -189,7 +195,7 @@ struct Solver {
                     "D2D rotated"))
                 return false;
         }
-        return sync();
+        return stage_sync();
     }
 
     int collapse(int n, bool have_matrices) {
@@ -231,8 +237,10 @@ extern "C" void q3x_davidson_default_options(q3x_davidson_options* o) {
     o->final_fp64_sweep = 1;
 }
 
-extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
-                                         double* psi_inout, double* eig_out, q3x_davidson_report* report) {
+namespace {
+// psi_on_device: psi_inout is a device buffer (npw x nb, ld npw) that stays in HBM
+q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt, double* psi_inout,
+                          bool psi_on_device, double* eig_out, q3x_davidson_report* report) {
     if (!h || !opt || !psi_inout || !eig_out) {
         set_error("q3x_davidson_solve: invalid argument");
         return Q3X_ERR_ARGUMENT;
@@ -247,6 +255,7 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
         return Q3X_ERR_ARGUMENT;
     }
     Solver s(h);
+    s.timing = h->ctx->prof.on;
     s.npw = h->npw;
     s.nb = nb;
     s.nocc = std::max(0, std::min(opt->n_occupied, nb));
@@ -269,7 +278,10 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
     double t0 = s.now_ms();
     int basis = 0;
     if (opt->seed_from_psi) {
-        if (!s.ok(hipMemcpyAsync(s.Vc(0), psi_inout, col * nb, hipMemcpyHostToDevice, s.st), "H2D psi")) return s.status;
+        if (!s.ok(hipMemcpyAsync(s.Vc(0), psi_inout, col * nb,
+                                 psi_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s.st),
+                  "copy psi in"))
+            return s.status;
         const int acc = s.orthogonalize(0, nb);
         if (acc < 0) return s.status;
         basis = acc;
@@ -367,7 +379,7 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
                           "Teter"))
                     return s.status;
             }
-            if (!s.sync()) return s.status;
+            if (!s.stage_sync()) return s.status;
             s.stage[6] += s.now_ms() - t0;
             const int accepted = s.orthogonalize(basis, nc);
             if (accepted < 0) return s.status;
@@ -421,7 +433,10 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
             return s.status;
         for (int b = 0; b < nb; ++b) maxres = std::max(maxres, std::sqrt(n2[b]));
     }
-    if (!s.ok(hipMemcpyAsync(psi_inout, s.Vc(0), col * nb, hipMemcpyDeviceToHost, s.st), "D2H psi") || !s.sync())
+    if (!s.ok(hipMemcpyAsync(psi_inout, s.Vc(0), col * nb,
+                             psi_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s.st),
+              "copy psi out") ||
+        !s.sync())
         return s.status;
     for (int b = 0; b < nb; ++b) eig_out[b] = w[b];
     s.stage[7] = s.now_ms() - t_start;
@@ -434,4 +449,15 @@ extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_da
         for (int i = 0; i < 8; ++i) report->stage_ms[i] = s.stage[i];
     }
     return Q3X_OK;
+}
+}  // namespace
+
+extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
+                                         double* psi_inout, double* eig_out, q3x_davidson_report* report) {
+    return davidson_solve(h, nb, opt, psi_inout, false, eig_out, report);
+}
+
+extern "C" q3x_status q3x_davidson_solve_device(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
+                                                void* d_psi_inout, double* eig_out, q3x_davidson_report* report) {
+    return davidson_solve(h, nb, opt, static_cast<double*>(d_psi_inout), true, eig_out, report);
 }

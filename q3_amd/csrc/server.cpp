@@ -1,0 +1,349 @@
+// Band-structure request server: continuous batching of non-self-consistent solves on one GPU.
+//
+// A request is one cell (basis, V(r), atoms and projector data) with a list of k-points; its
+// result is nk x nb eigenvalues, its wavefunctions stay in HBM until the client collects the
+// result.  Requests are admitted in arrival order while their estimated HBM footprint fits the
+// budget (the reference's PlanewaveMemoryEstimate::compute terms,
+// iterator/kohn_sham/planewave/internal/memory_estimate.hpp:75-215); an admitted request's
+// k-points join one shared task queue, and W worker threads -- each with its own context,
+// stream and Davidson workspace -- take k-point tasks from it in order.  A request that arrives
+// while others are running is admitted as soon as the budget allows and its k-points interleave
+// with theirs on the device; when its last k-point finishes, its footprint is released and the
+// next waiting requests are admitted.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "runtime.h"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Request {
+    long ticket = 0;
+    q3x_basis basis = nullptr;
+    std::vector<double> v, kpts, atom_pos, r_all, rab_all, beta_r_all, d_blocks;
+    std::vector<int> atom_species, nmesh, nbeta, l_all, cutoff_all, block_sizes;
+    int nk = 0, natoms = 0, nspecies = 0, nb = 0, nocc = 0;
+    q3x_davidson_options opt{};
+    double bytes = 0.0;
+    // progress
+    int remaining = 0;
+    int next_task = 0;
+    bool admitted = false, done = false;
+    q3x_status status = Q3X_OK;
+    std::string error;
+    std::vector<double> eig;
+    std::vector<std::pair<q3x_context, void*>> psi;  // device wavefunctions, one per k-point
+    int iterations = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+};
+
+double now_s(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
+
+}  // namespace
+
+struct q3x_server_s {
+    int device = 0;
+    double budget = 0.0, in_flight = 0.0;
+    clk::time_point t0;
+    std::vector<q3x_context> workers;
+    std::vector<std::thread> threads;
+    std::mutex mu;
+    std::condition_variable cv_task, cv_done;
+    std::deque<std::shared_ptr<Request>> waiting;   // submitted, not admitted
+    std::deque<std::shared_ptr<Request>> running;   // admitted, tasks left to hand out
+    std::map<long, std::shared_ptr<Request>> all;
+    long next_ticket = 1;
+    bool stop = false;
+
+    // admission in arrival order: the head waits until it fits (or nothing else is in flight)
+    void admit_locked() {
+        while (!waiting.empty()) {
+            auto r = waiting.front();
+            if (in_flight > 0.0 && in_flight + r->bytes > budget) break;
+            waiting.pop_front();
+            r->admitted = true;
+            r->t_admit = now_s(t0);
+            in_flight += r->bytes;
+            running.push_back(r);
+        }
+        cv_task.notify_all();
+    }
+
+    void worker(int w) {
+        q3x_context ctx = workers[w];
+        if (hipSetDevice(device) != hipSuccess) return;
+        for (;;) {
+            std::shared_ptr<Request> r;
+            int k = -1;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv_task.wait(g, [&] { return stop || !running.empty(); });
+                if (stop && running.empty()) return;
+                r = running.front();
+                k = r->next_task++;
+                if (r->next_task >= r->nk) running.pop_front();
+            }
+            std::string err;
+            const q3x_status s = solve_kpoint(ctx, *r, k, err);
+            std::lock_guard<std::mutex> g(mu);
+            if (s != Q3X_OK && r->status == Q3X_OK) {
+                r->status = s;
+                r->error = "k-point " + std::to_string(k) + ": " + err;
+            }
+            if (--r->remaining == 0) {
+                r->done = true;
+                r->t_done = now_s(t0);
+                in_flight -= r->bytes;
+                if (in_flight < 1.0) in_flight = 0.0;
+                admit_locked();
+                cv_done.notify_all();
+            }
+        }
+    }
+
+    q3x_status solve_kpoint(q3x_context ctx, Request& r, int k, std::string& err) {
+        q3x_kpoint kp = nullptr;
+        q3x_hamiltonian h = nullptr;
+        int npw = 0, nproj = 0;
+        q3x_status s = q3x_kpoint_create(r.basis, &r.kpts[3 * k], &kp, &npw);
+        auto fail = [&](q3x_status st) {
+            err = q3x_last_error();
+            if (h) q3x_hamiltonian_destroy(h);
+            if (kp) q3x_kpoint_destroy(kp);
+            return st;
+        };
+        if (s != Q3X_OK) return fail(s);
+        auto kb = [&](double* out) {
+            return q3x_kb_projectors(r.basis, kp, r.natoms, r.atom_pos.data(), r.atom_species.data(), r.nspecies,
+                                     r.nmesh.data(), r.r_all.data(), r.rab_all.data(), r.nbeta.data(), r.l_all.data(),
+                                     r.cutoff_all.data(), r.beta_r_all.data(), out, &nproj);
+        };
+        if ((s = kb(nullptr)) != Q3X_OK) return fail(s);
+        std::vector<double> beta(static_cast<size_t>(npw) * std::max(nproj, 1) * 2);
+        if ((s = kb(beta.data())) != Q3X_OK) return fail(s);
+        // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
+        // column-major layout the Hamiltonian takes
+        if ((s = q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
+            (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK ||
+            (s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
+            (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, beta.data(), nproj, r.natoms, r.block_sizes.data(),
+                                                               r.d_blocks.data())) != Q3X_OK))
+            return fail(s);
+        void* psi = nullptr;
+        if ((s = q3x_device_alloc(ctx, static_cast<unsigned long>(npw) * r.nb * 16, &psi)) != Q3X_OK) return fail(s);
+        {
+            std::lock_guard<std::mutex> g(mu);
+            r.psi[k] = {ctx, psi};
+        }
+        q3x_davidson_options o = r.opt;
+        o.seed_from_psi = 0;
+        q3x_davidson_report rep{};
+        s = q3x_davidson_solve_device(h, r.nb, &o, psi, &r.eig[static_cast<size_t>(k) * r.nb], &rep);
+        if (s != Q3X_OK) return fail(s);
+        {
+            std::lock_guard<std::mutex> g(mu);
+            r.iterations += rep.iterations;
+        }
+        q3x_hamiltonian_destroy(h);
+        q3x_kpoint_destroy(kp);
+        return Q3X_OK;
+    }
+};
+
+using namespace q3x;
+
+namespace {
+
+template <typename T>
+std::vector<T> copy_of(const T* p, size_t n) {
+    return p ? std::vector<T>(p, p + n) : std::vector<T>(n, T{});
+}
+
+// Footprint of one request in bytes: its wavefunctions (kept for all k-points) plus, for each
+// k-point that may run at once, the Davidson basis V and HV, the rotation buffer, the subspace
+// matrices and eigensolver workspace, the projectors (complex128 + complex64), the z-stick
+// buffer and the potential -- the terms of PlanewaveMemoryEstimate::compute
+// (memory_estimate.hpp:75-215) for this path.
+double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int nproj, int workers) {
+    const double c = 16.0, nb = q->nbands, m = std::max(std::min(2.0 * nb, static_cast<double>(npw)), nb);
+    const double psi = static_cast<double>(q->nk) * nb * npw * c;
+    const double sticks = std::max(1.0, static_cast<double>(npw) / std::max(1, n3)) * 1.6;  // z-sticks
+    const double per_k = 2 * m * npw * c + nb * npw * c + 30.0 * m * m * c + 1.5 * nproj * npw * c +
+                         nb * n3 * sticks * c + 12.0 * npoints;
+    return psi + per_k * std::min(q->nk, workers);
+}
+
+}  // namespace
+
+extern "C" {
+
+q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out) {
+    if (!out || workers < 1 || workers > 64) {
+        set_error("q3x_server_create: need 1 <= workers <= 64");
+        return Q3X_ERR_ARGUMENT;
+    }
+    auto* s = new q3x_server_s;
+    s->device = device;
+    s->t0 = clk::now();
+    for (int w = 0; w < workers; ++w) {
+        q3x_context c = nullptr;
+        const q3x_status st = q3x_context_create(device, &c);
+        if (st != Q3X_OK) {
+            for (q3x_context x : s->workers) q3x_context_destroy(x);
+            delete s;
+            return st;
+        }
+        s->workers.push_back(c);
+    }
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
+    for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
+    *out = s;
+    return Q3X_OK;
+}
+
+q3x_status q3x_server_destroy(q3x_server s) {
+    if (!s) return Q3X_OK;
+    {
+        std::unique_lock<std::mutex> g(s->mu);
+        s->cv_done.wait(g, [&] {
+            for (auto& kv : s->all)
+                if (!kv.second->done) return false;
+            return true;
+        });
+        s->stop = true;
+        s->cv_task.notify_all();
+    }
+    for (auto& t : s->threads) t.join();
+    for (auto& kv : s->all)
+        for (auto& cp : kv.second->psi)
+            if (cp.second) q3x_device_free(cp.first, cp.second);
+    for (q3x_context c : s->workers) q3x_context_destroy(c);
+    delete s;
+    return Q3X_OK;
+}
+
+q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* bytes) {
+    if (!s || !q || !q->basis || !bytes || q->nk < 1) {
+        set_error("q3x_server_estimate: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    q3x_kpoint kp = nullptr;
+    int npw = 0;
+    const q3x_status st = q3x_kpoint_create(q->basis, q->kpoints, &kp, &npw);
+    if (st != Q3X_OK) return st;
+    q3x_kpoint_destroy(kp);
+    int rho[3], wave[3], ng = 0;
+    double vol = 0, tpa2 = 0;
+    q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
+    int nproj = 0;
+    for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
+    *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2], static_cast<long>(wave[0]) * wave[1] * wave[2],
+                       nproj, static_cast<int>(s->workers.size()));
+    return Q3X_OK;
+}
+
+q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* ticket) {
+    if (!s || !q || !q->basis || !q->v_r || !q->kpoints || q->nk < 1 || q->nbands < 1 || !ticket) {
+        set_error("q3x_server_submit: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    auto r = std::make_shared<Request>();
+    const q3x_status st = q3x_server_estimate(s, q, &r->bytes);
+    if (st != Q3X_OK) return st;
+    r->basis = q->basis;
+    r->v = copy_of(q->v_r, static_cast<size_t>(q->nv));
+    r->kpts = copy_of(q->kpoints, static_cast<size_t>(q->nk) * 3);
+    r->nk = q->nk;
+    r->natoms = q->natoms;
+    r->nspecies = q->nspecies;
+    r->atom_pos = copy_of(q->atom_pos, static_cast<size_t>(q->natoms) * 3);
+    r->atom_species = copy_of(q->atom_species, static_cast<size_t>(q->natoms));
+    r->nmesh = copy_of(q->nmesh, static_cast<size_t>(q->nspecies));
+    r->nbeta = copy_of(q->nbeta, static_cast<size_t>(q->nspecies));
+    const size_t mesh_total = std::accumulate(r->nmesh.begin(), r->nmesh.end(), size_t(0));
+    size_t beta_total = 0, beta_values = 0;
+    for (int sp = 0; sp < q->nspecies; ++sp) {
+        beta_total += r->nbeta[sp];
+        beta_values += static_cast<size_t>(r->nbeta[sp]) * r->nmesh[sp];
+    }
+    r->r_all = copy_of(q->r_all, mesh_total);
+    r->rab_all = copy_of(q->rab_all, mesh_total);
+    r->l_all = copy_of(q->l_all, beta_total);
+    r->cutoff_all = copy_of(q->cutoff_all, beta_total);
+    r->beta_r_all = copy_of(q->beta_r_all, beta_values);
+    r->block_sizes = copy_of(q->block_sizes, static_cast<size_t>(q->natoms));
+    size_t dvals = 0;
+    for (int b : r->block_sizes) dvals += static_cast<size_t>(b) * b;
+    r->d_blocks = copy_of(q->d_blocks, dvals);
+    r->nb = q->nbands;
+    r->nocc = q->noccupied;
+    r->opt = q->options;
+    r->opt.n_occupied = q->noccupied;
+    r->remaining = q->nk;
+    r->eig.assign(static_cast<size_t>(q->nk) * q->nbands, 0.0);
+    r->psi.assign(q->nk, {nullptr, nullptr});
+    std::lock_guard<std::mutex> g(s->mu);
+    r->ticket = s->next_ticket++;
+    r->t_submit = now_s(s->t0);
+    s->all[r->ticket] = r;
+    s->waiting.push_back(r);
+    s->admit_locked();
+    *ticket = r->ticket;
+    return Q3X_OK;
+}
+
+q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_request_stats* stats) {
+    if (!s) {
+        set_error("q3x_server_wait: null server");
+        return Q3X_ERR_ARGUMENT;
+    }
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> g(s->mu);
+        auto it = s->all.find(ticket);
+        if (it == s->all.end()) {
+            set_error("q3x_server_wait: unknown ticket");
+            return Q3X_ERR_ARGUMENT;
+        }
+        r = it->second;
+        s->cv_done.wait(g, [&] { return r->done; });
+        s->all.erase(it);
+    }
+    if (eig_out) std::memcpy(eig_out, r->eig.data(), r->eig.size() * sizeof(double));
+    if (stats) {
+        stats->submit_s = r->t_submit;
+        stats->admit_s = r->t_admit;
+        stats->done_s = r->t_done;
+        stats->footprint_bytes = r->bytes;
+        stats->iterations = r->iterations;
+    }
+    for (auto& cp : r->psi)
+        if (cp.second) q3x_device_free(cp.first, cp.second);
+    r->psi.clear();
+    if (r->status != Q3X_OK) set_error(r->error);
+    return r->status;
+}
+
+double q3x_server_in_flight_bytes(q3x_server s) {
+    if (!s) return 0.0;
+    std::lock_guard<std::mutex> g(s->mu);
+    return s->in_flight;
+}
+
+}  // extern "C"

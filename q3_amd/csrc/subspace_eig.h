@@ -22,8 +22,9 @@ struct SubspaceEigStats {
 // S-orthonormal eigenvectors; returns 0.  Returns n + 1 when S is not numerically positive
 // definite (the zhegv info > n case) and -1 on a device error (*err set).
 // work: subspace_eig_workspace(n, nvec) bytes; gemm_work: split-K workspace of zgemm.
+// num_cus: compute units of the device (bounds the workgroups of the cooperative reduction).
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 hipStream_t st, hipError_t* err, SubspaceEigStats* stats);
+                 int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats);
 
 }  // namespace q3x

@@ -419,6 +419,199 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
         for (long e = tid; e < nn; e += nt) A[e] = M[e];
 }
 
+// ---------------------------------------------------------------- 3b. tridiagonalisation over several CUs ----
+// The same reduction with the matrix spread over the LDS of W workgroups: column j lives in
+// workgroup j mod W (all rows), so the passes run at LDS speed on W CUs instead of at one CU's
+// L2 bandwidth.  Per step: every workgroup updates its own columns and forms its entries of p;
+// the owner of the next pivot column also publishes that column.  One grid barrier; then every
+// workgroup reads p and the pivot column and forms q and the next reflector itself -- the same
+// operations in the same order in each workgroup, so all hold identical values and nothing else
+// has to be broadcast.  p and the pivot column are double-buffered by step parity: a workgroup
+// can be at most one step ahead of another.  Workgroup 0 writes the outputs (d, beta, tau and the
+// reflectors into R, n x n, column k rows k+2..).  Launched cooperatively (all workgroups
+// resident); the barrier's wait is bounded and sets *fail instead of waiting forever.
+
+// Once any workgroup has given up (about a second of polling), every later barrier returns at
+// once, so the kernel always drains quickly; the host turns *fail into an error.
+__device__ void grid_barrier(unsigned* bar, unsigned target, int* fail) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();  // release this workgroup's global writes
+        atomicAdd(bar, 1u);
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1u << 20)) {
+                atomicExch(fail, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();  // acquire the other workgroups' writes
+    }
+    __syncthreads();
+}
+
+// Block-level reflector of x = col[1..L) (col: LDS, length L = n - j, rows j..n-1 of column j).
+// Writes v (length L-1) to vout, returns tau; workgroup 0 stores beta[j+1], tau[j] and v[1..]
+// into R.  Barriers inside; every thread returns the same tau.
+__device__ double block_house(const z2* col, int L, int n, int j, z2* vout, double* red, int& slot, bool writer,
+                              z2* beta, double* tau, z2* R) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double part = 0.0;
+    for (int i = tid + 2; i < L; i += nt) part += col[i].x * col[i].x + col[i].y * col[i].y;
+    const double tail = block_sum(part, red, slot);
+    const z2 x0 = col[1];
+    double t = 0.0;
+    if (tail == 0.0) {
+        if (writer && tid == 0) beta[j + 1] = x0;
+    } else {
+        const double a0 = hypot(x0.x, x0.y);
+        const double xnorm = sqrt(tail + a0 * a0);
+        const z2 ph = a0 == 0.0 ? z2{1.0, 0.0} : z2{x0.x / a0, x0.y / a0};
+        const z2 b = {-ph.x * xnorm, -ph.y * xnorm};
+        const z2 u0 = {x0.x - b.x, x0.y - b.y};
+        const double iu2 = 1.0 / (u0.x * u0.x + u0.y * u0.y);
+        t = 2.0 / (1.0 + tail * iu2);
+        for (int i = tid + 1; i < L; i += nt) {
+            const z2 a = col[i];
+            const z2 w = i == 1 ? z2{1.0, 0.0} : z2{(a.x * u0.x + a.y * u0.y) * iu2, (a.y * u0.x - a.x * u0.y) * iu2};
+            vout[i - 1] = w;
+            if (writer && i >= 2) R[(j + i) + static_cast<long>(j) * n] = w;
+        }
+        if (writer && tid == 0) beta[j + 1] = b;
+    }
+    if (writer && tid == 0) tau[j] = t;
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(kThreads) void tridiag_coop(const z2* __restrict__ A, int n, z2* R, double* d,
+                                                         z2* beta, double* tau, z2* phase, double* eabs,
+                                                         z2* xP, z2* xC, unsigned* bar, int* fail) {
+    extern __shared__ z2 sh[];
+    const int W = gridDim.x, wg = blockIdx.x;
+    const int ncl = (n + W - 1) / W;
+    z2* vb0 = sh;
+    z2* vb1 = sh + n;
+    z2* q = sh + 2 * n;
+    z2* p = sh + 3 * n;
+    z2* colb = sh + 4 * n;
+    double* red = reinterpret_cast<double*>(sh + 5 * n);  // 32 doubles
+    z2* Mloc = sh + 5 * n + 16;                           // ncl columns of n
+    const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nt >> 6;
+    const bool writer = wg == 0;
+    int slot = 0;
+    unsigned nbar = 0;
+    // own columns, Hermitian part (A + A^H) / 2 with a real diagonal
+    for (long e = tid; e < static_cast<long>(ncl) * n; e += nt) {
+        const int c = static_cast<int>(e / n), i = static_cast<int>(e - static_cast<long>(c) * n);
+        const int j = wg + W * c;
+        if (j >= n) continue;
+        const z2 a = A[i + static_cast<long>(j) * n], b = A[j + static_cast<long>(i) * n];
+        Mloc[e] = i == j ? z2{a.x, 0.0} : z2{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+    }
+    // column 0 for the first reflector
+    for (int i = tid; i < n; i += nt) {
+        const z2 a = A[i], b = A[static_cast<long>(i) * n];
+        colb[i] = i == 0 ? z2{a.x, 0.0} : z2{0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+    }
+    if (writer && tid == 0) {
+        beta[0] = {0.0, 0.0};
+        for (int k = 0; k < n; ++k) tau[k] = 0.0;
+    }
+    __syncthreads();
+    if (writer && tid == 0) d[0] = colb[0].x;
+    double tk = 0.0;
+    double tn = block_house(colb, n, n, 0, vb0, red, slot, writer, beta, tau, R);
+    z2* v = vb1;
+    z2* vn = vb0;
+    for (int k = -1; k <= n - 3; ++k) {
+        const int len = n - k - 2, base = k + 2, par = (k + 1) & 1;
+        z2* P = xP + static_cast<long>(par) * n;
+        z2* Cc = xC + static_cast<long>(par) * n;
+        // own columns j >= base: local c with wg + W c >= base
+        const int c_first = base <= wg ? 0 : (base - wg + W - 1) / W;
+        for (int c = c_first + wv; c < ncl; c += nw) {
+            const int j = wg + W * c;
+            if (j >= n) break;
+            z2* col = Mloc + static_cast<long>(c) * n + base;
+            const int jb = j - base;
+            const z2 qj = tk != 0.0 ? q[jb + 1] : z2{0.0, 0.0};
+            const z2 vj = tk != 0.0 ? v[jb + 1] : z2{0.0, 0.0};
+            double sr = 0.0, si = 0.0;
+            for (int r = lane; r < len; r += 64) {
+                z2 a = col[r];
+                if (tk != 0.0) {
+                    const z2 t1 = zmulc(v[r + 1], qj), t2 = zmulc(q[r + 1], vj);
+                    a.x -= t1.x + t2.x;
+                    a.y -= t1.y + t2.y;
+                    col[r] = a;
+                }
+                if (jb == 0) Cc[r] = a;
+                const z2 w = vn[r];
+                sr += a.x * w.x + a.y * w.y;
+                si += a.x * w.y - a.y * w.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            if (lane == 0) P[jb] = {tn * sr, tn * si};
+        }
+        nbar += static_cast<unsigned>(W);
+        grid_barrier(bar, nbar, fail);
+        for (int i = tid; i < len; i += nt) {
+            p[i] = P[i];
+            colb[i] = Cc[i];
+        }
+        __syncthreads();
+        double f = 0.0;
+        if (tn != 0.0) {
+            double part = 0.0;
+            for (int i = tid; i < len; i += nt) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
+            f = 0.5 * tn * block_sum(part, red, slot);
+            for (int i = tid; i < len; i += nt) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
+            __syncthreads();
+            const z2 q0 = q[0], v0 = vn[0];
+            for (int i = tid; i < len; i += nt) {  // pivot column with reflector k+1
+                const z2 t1 = zmulc(vn[i], q0), t2 = zmulc(q[i], v0);
+                colb[i].x -= t1.x + t2.x;
+                colb[i].y -= t1.y + t2.y;
+            }
+            __syncthreads();
+        }
+        if (writer && tid == 0) d[base] = colb[0].x;
+        double tj = 0.0;
+        if (base <= n - 3) {
+            tj = block_house(colb, len, n, base, v, red, slot, writer, beta, tau, R);
+        } else if (base == n - 2) {
+            if (writer && tid == 0) beta[n - 1] = colb[1];
+        }
+        tk = tn;
+        tn = tj;
+        z2* t = v;
+        v = vn;
+        vn = t;
+        __syncthreads();
+    }
+    if (writer && tid == 0) {
+        z2 ph = {1.0, 0.0};
+        phase[0] = ph;
+        eabs[0] = 0.0;
+        for (int i = 1; i < n; ++i) {
+            const z2 b = beta[i];
+            const double r = hypot(b.x, b.y);
+            eabs[i] = r;
+            if (r != 0.0) ph = zmul(ph, z2{b.x / r, b.y / r});
+            phase[i] = ph;
+        }
+    }
+}
+
+__global__ void zero_words(unsigned* a, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0u;
+}
+
 // ---------------------------------------------------------------- 4. eigenvalues ----
 // Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x (Sturm count of the LDL^T
 // pivots; pivots smaller than pivmin are replaced by -pivmin, as LAPACK's dstebz does).
@@ -670,18 +863,20 @@ __global__ void ortho_factor(int m, z2* __restrict__ G) {
 }
 
 struct Layout {
-    size_t off[24];
+    size_t off[32];  // >= B_COUNT
     size_t total;
 };
 enum Buf { B_HF, B_SF, B_ZL, B_C, B_T1, B_Y, B_Z2, B_ZY, B_YB, B_G, B_Y2, B_D, B_E, B_TAU, B_W, B_ZT, B_U1, B_U2,
-           B_U3, B_LO, B_SW, B_BETA, B_PH, B_ERR, B_COUNT };
+           B_U3, B_LO, B_SW, B_BETA, B_PH, B_ERR, B_XP, B_XC, B_BAR, B_COUNT };
+static_assert(B_COUNT <= 32, "Layout::off holds 32 buffers");
 Layout layout(int n, int nvec) {
     const size_t nn = static_cast<size_t>(n) * n * 16, nv = static_cast<size_t>(n) * nvec;
     const size_t sz[B_COUNT] = {nn, nn, nn, nn, nn, nn, nn, nn,                   // Hf Sf Zl C T1 Y Z2 ZY
                                 nv * 16, static_cast<size_t>(nvec) * nvec * 16, nv * 16,  // Yb G Y2
                                 size_t(n) * 8, size_t(n) * 8, size_t(n) * 8, size_t(nvec) * 8,  // d e tau w
                                 nv * 8, nv * 8, nv * 8, nv * 8, nv * 8, nv * 4,  // Zt U1 U2 U3 LO SW
-                                size_t(n) * 16, size_t(n) * 16, 64};              // beta phase err
+                                size_t(n) * 16, size_t(n) * 16, 64,               // beta phase err
+                                size_t(n) * 32, size_t(n) * 32, 64};              // exchange p, pivot, barrier
     Layout L{};
     size_t o = 0;
     for (int i = 0; i < B_COUNT; ++i) {
@@ -700,7 +895,7 @@ size_t subspace_eig_workspace(int n, int nvec) { return layout(std::max(n, 1), s
 
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 hipStream_t st, hipError_t* err, SubspaceEigStats* stats) {
+                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats) {
     *err = hipSuccess;
     if (n <= 0 || nvec <= 0) return 0;
     nvec = std::min(nvec, n);
@@ -729,16 +924,11 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     hipLaunchKernelGGL(herm_prep, dim3(1), dim3(kThreads), 0, st, n, reinterpret_cast<const z2*>(H), ldh,
                        reinterpret_cast<const z2*>(S), lds, Hf, Sf, Zl, errd);
     if (!ok(hipGetLastError())) return -1;
-    double sdev = 0.0;
-    if (!ok(hipMemcpyAsync(&sdev, errd, 8, hipMemcpyDeviceToHost, st)) || !ok(hipStreamSynchronize(st))) return -1;
-    if (stats) {
-        stats->s_deviation = sdev;
-        stats->newton_schulz_iterations = 0;
-    }
-    // first-order Loewdin factor error ~ (3/8) |E|_2^2 <= (3/8) (n max|E|)^2
-    if (!(sdev * n <= 1e-7)) {
+    // Newton-Schulz on S / c with c >= lambda_max(S): Y -> (S/c)^(1/2), Z -> (S/c)^(-1/2);
+    // replaces the first-order factor in Zl.  0, n + 1 (no convergence: S not positive
+    // definite) or -1.
+    auto newton_schulz = [&](double sdev) -> int {
         if (!(sdev == sdev) || sdev > 1e6) return n + 1;
-        // Newton-Schulz on S / c with c >= lambda_max(S): Y -> (S/c)^(1/2), Z -> (S/c)^(-1/2)
         const double c = 1.0 + n * sdev;
         z2 *Y = P(B_Y), *Z = Zl, *Z2 = P(B_Z2), *ZY = P(B_ZY), *Tm = T1, *Yn = C;
         hipLaunchKernelGGL(ns_init, dim3(eg), dim3(256), 0, st, n, Sf, 1.0 / c, Y, Z);
@@ -763,73 +953,137 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
         hipLaunchKernelGGL(scale_matrix, dim3(eg), dim3(256), 0, st, nn, 1.0 / std::sqrt(c), Zl);
         if (!ok(hipGetLastError())) return -1;
         if (stats) stats->newton_schulz_iterations = it;
-    }
-    // C = Zl Hf Zl
-    if (!gemm(0, n, n, n, Hf, n, Zl, n, T1, n) || !gemm(0, n, n, n, Zl, n, T1, n, C, n)) return -1;
-    // tridiagonalisation
-    double *d = D(B_D), *e = D(B_E), *tau = D(B_TAU), *w = D(B_W);
-    z2 *beta = P(B_BETA), *ph = P(B_PH);
-    const int rows_lane = (n + 63) / 64;
-    const bool lm = rows_lane <= 2 && tridiag_lds(n, true) <= max_lds;
-    const size_t tl = tridiag_lds(n, lm);
-    if (tl > max_lds || rows_lane > 16) {
-        *err = hipErrorInvalidValue;
-        return -1;
-    }
-    static std::atomic<bool> lds_raised{false};  // set once per process; the attribute call is idempotent
-    if (!lds_raised && max_lds > 65536) {
-        if (!ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 1>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))) ||
-            !ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 2>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))))
+        return 0;
+    };
+    // C = Zl Hf Zl, then steps 3-6; one synchronisation at the end, which also brings back
+    // max |S - I| (errd, written by herm_prep) into *sdev_out.  0 or -1.
+    auto solve_from_zl = [&](double* sdev_out) -> int {
+        // C = Zl Hf Zl
+        if (!gemm(0, n, n, n, Hf, n, Zl, n, T1, n) || !gemm(0, n, n, n, Zl, n, T1, n, C, n)) return -1;
+        // tridiagonalisation
+        double *d = D(B_D), *e = D(B_E), *tau = D(B_TAU), *w = D(B_W);
+        z2 *beta = P(B_BETA), *ph = P(B_PH);
+        const int rows_lane = (n + 63) / 64;
+        const bool lm = rows_lane <= 2 && tridiag_lds(n, true) <= max_lds;
+        const size_t tl = tridiag_lds(n, lm);
+        if (tl > max_lds || rows_lane > 16) {
+            *err = hipErrorInvalidValue;
             return -1;
-        lds_raised = true;
-    }
+        }
+        static std::atomic<bool> lds_raised{false};  // set once per process; the attribute call is idempotent
+        if (!lds_raised && max_lds > 65536) {
+            if (!ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 1>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))) ||
+                !ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_kernel<true, 2>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(max_lds))))
+                return -1;
+            lds_raised = true;
+        }
+            // several CUs when the matrix does not fit one CU's LDS: column j on workgroup j mod W,
+        // about 16 columns (one per wavefront) each, fewer when LDS is short
+        z2* refl = C;  // where the reflectors end up
+        bool coop_done = false;
+        if (!lm && n >= 100) {
+            const long room = static_cast<long>(max_lds / 16) - 5L * n - 16;
+            const int ncl = static_cast<int>(std::min<long>(16, room / n));
+            if (ncl >= 1) {
+                const int W = (n + ncl - 1) / ncl;
+                const size_t lds = (5 * static_cast<size_t>(n) + 16 + static_cast<size_t>((n + W - 1) / W) * n) * 16;
+                if (W <= num_cus_hint && lds <= max_lds) {
+                    static std::atomic<bool> coop_raised{false};
+                    if (!coop_raised) {
+                        if (!ok(hipFuncSetAttribute(reinterpret_cast<const void*>(&tridiag_coop),
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    static_cast<int>(max_lds))))
+                            return -1;
+                        coop_raised = true;
+                    }
+                    unsigned* bar = reinterpret_cast<unsigned*>(base + L.off[B_BAR]);
+                    hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, bar, 2);
+                    if (!ok(hipGetLastError())) return -1;
+                    const z2* Cc = C;
+                    z2* R = T1;
+                    z2* xp = P(B_XP);
+                    z2* xc = P(B_XC);
+                    int* fail = reinterpret_cast<int*>(bar + 1);
+                    int nn_ = n;
+                    void* args[] = {&Cc, &nn_, &R, &d, &beta, &tau, &ph, &e, &xp, &xc, &bar, &fail};
+                    const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tridiag_coop),
+                                                                     dim3(W), dim3(kThreads), args, lds, st);
+                    if (le == hipSuccess) {
+                        coop_done = true;
+                        refl = T1;
+                    } else {
+                        (void)hipGetLastError();  // not co-resident: the one-workgroup kernel below
+                    }
+                }
+            }
+        }
 #define Q3X_TRD(LMV, R) \
-    hipLaunchKernelGGL((tridiag_kernel<LMV, R>), dim3(1), dim3(kThreads), tl, st, C, n, d, beta, tau, ph, e)
-    if (lm) {
-        if (rows_lane <= 1) Q3X_TRD(true, 1);
-        else Q3X_TRD(true, 2);
-    } else if (rows_lane <= 1) Q3X_TRD(false, 1);
-    else if (rows_lane <= 2) Q3X_TRD(false, 2);
-    else if (rows_lane <= 4) Q3X_TRD(false, 4);
-    else if (rows_lane <= 6) Q3X_TRD(false, 6);
-    else if (rows_lane <= 8) Q3X_TRD(false, 8);
-    else if (rows_lane <= 12) Q3X_TRD(false, 12);
-    else Q3X_TRD(false, 16);
-#undef Q3X_TRD
-    if (!ok(hipGetLastError())) return -1;
-    hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(64), static_cast<size_t>(n) * 16, st, n, d, e, w);
-    if (!ok(hipGetLastError())) return -1;
-    hipLaunchKernelGGL(invit_kernel, dim3(1), dim3(kThreads), 0, st, n, d, e, w, nvec, D(B_ZT), D(B_U1), D(B_U2),
-                       D(B_U3), D(B_LO), reinterpret_cast<int*>(base + L.off[B_SW]));
-    if (!ok(hipGetLastError())) return -1;
-    z2* Yb = P(B_YB);
-    const int rpl = (n + 63) / 64;
-#define Q3X_BT(R)                                                                                              \
-    hipLaunchKernelGGL(backtransform_kernel<R>, dim3(nvec), dim3(64), 0, st, n, C, tau, ph, D(B_ZT), nvec, Yb)
-    if (rpl <= 1) Q3X_BT(1);
-    else if (rpl <= 2) Q3X_BT(2);
-    else if (rpl <= 4) Q3X_BT(4);
-    else if (rpl <= 8) Q3X_BT(8);
-    else if (rpl <= 16) Q3X_BT(16);
-    else {
-        *err = hipErrorInvalidValue;
-        return -1;
+        hipLaunchKernelGGL((tridiag_kernel<LMV, R>), dim3(1), dim3(kThreads), tl, st, C, n, d, beta, tau, ph, e)
+        if (coop_done) {
+        } else if (lm) {
+            if (rows_lane <= 1) Q3X_TRD(true, 1);
+            else Q3X_TRD(true, 2);
+        } else if (rows_lane <= 1) Q3X_TRD(false, 1);
+        else if (rows_lane <= 2) Q3X_TRD(false, 2);
+        else if (rows_lane <= 4) Q3X_TRD(false, 4);
+        else if (rows_lane <= 6) Q3X_TRD(false, 6);
+        else if (rows_lane <= 8) Q3X_TRD(false, 8);
+        else if (rows_lane <= 12) Q3X_TRD(false, 12);
+        else Q3X_TRD(false, 16);
+    #undef Q3X_TRD
+        if (!ok(hipGetLastError())) return -1;
+        hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(64), static_cast<size_t>(n) * 16, st, n, d, e, w);
+        if (!ok(hipGetLastError())) return -1;
+        hipLaunchKernelGGL(invit_kernel, dim3(1), dim3(kThreads), 0, st, n, d, e, w, nvec, D(B_ZT), D(B_U1), D(B_U2),
+                           D(B_U3), D(B_LO), reinterpret_cast<int*>(base + L.off[B_SW]));
+        if (!ok(hipGetLastError())) return -1;
+        z2* Yb = P(B_YB);
+        const int rpl = (n + 63) / 64;
+    #define Q3X_BT(R)                                                                                              \
+        hipLaunchKernelGGL(backtransform_kernel<R>, dim3(nvec), dim3(64), 0, st, n, refl, tau, ph, D(B_ZT), nvec, Yb)
+        if (rpl <= 1) Q3X_BT(1);
+        else if (rpl <= 2) Q3X_BT(2);
+        else if (rpl <= 4) Q3X_BT(4);
+        else if (rpl <= 8) Q3X_BT(8);
+        else if (rpl <= 16) Q3X_BT(16);
+        else {
+            *err = hipErrorInvalidValue;
+            return -1;
+        }
+    #undef Q3X_BT
+        if (!ok(hipGetLastError())) return -1;
+        // Y <- Y (1.5 I - 0.5 Y^H Y); X = Zl Y
+        z2 *G = P(B_G), *Y2 = P(B_Y2);
+        if (!gemm(1, nvec, nvec, n, Yb, n, Yb, n, G, nvec)) return -1;
+        const long mm = static_cast<long>(nvec) * nvec;
+        hipLaunchKernelGGL(ortho_factor, dim3(static_cast<unsigned>(std::min<long>((mm + 255) / 256, 1024))), dim3(256),
+                           0, st, nvec, G);
+        if (!ok(hipGetLastError())) return -1;
+        if (!gemm(0, n, nvec, nvec, Yb, n, G, nvec, Y2, n) || !gemm(0, n, nvec, n, Zl, n, Y2, n, X, ldx)) return -1;
+        int coop_fail = 0;
+        if (!ok(hipMemcpyAsync(w_host, w, static_cast<size_t>(nvec) * 8, hipMemcpyDeviceToHost, st)) ||
+            !ok(hipMemcpyAsync(sdev_out, errd, 8, hipMemcpyDeviceToHost, st)) ||
+            (coop_done && !ok(hipMemcpyAsync(&coop_fail, base + L.off[B_BAR] + 4, 4, hipMemcpyDeviceToHost, st))) ||
+            !ok(hipStreamSynchronize(st)))
+            return -1;
+        if (coop_fail) {  // a grid barrier gave up waiting
+            *err = hipErrorLaunchTimeOut;
+            return -1;
+        }
+        return 0;
+    };
+    double sdev = 0.0, sdev2 = 0.0;
+    if (stats) stats->newton_schulz_iterations = 0;
+    int r = solve_from_zl(&sdev);
+    if (r != 0) return r;
+    if (stats) stats->s_deviation = sdev;
+    // first-order Loewdin factor error ~ (3/8) |E|_2^2 <= (3/8) (n max|E|)^2
+    if (!(sdev * n <= 1e-7)) {
+        if ((r = newton_schulz(sdev)) != 0) return r;
+        if ((r = solve_from_zl(&sdev2)) != 0) return r;
     }
-#undef Q3X_BT
-    if (!ok(hipGetLastError())) return -1;
-    // Y <- Y (1.5 I - 0.5 Y^H Y); X = Zl Y
-    z2 *G = P(B_G), *Y2 = P(B_Y2);
-    if (!gemm(1, nvec, nvec, n, Yb, n, Yb, n, G, nvec)) return -1;
-    const long mm = static_cast<long>(nvec) * nvec;
-    hipLaunchKernelGGL(ortho_factor, dim3(static_cast<unsigned>(std::min<long>((mm + 255) / 256, 1024))), dim3(256),
-                       0, st, nvec, G);
-    if (!ok(hipGetLastError())) return -1;
-    if (!gemm(0, n, nvec, nvec, Yb, n, G, nvec, Y2, n) || !gemm(0, n, nvec, n, Zl, n, Y2, n, X, ldx)) return -1;
-    if (!ok(hipMemcpyAsync(w_host, w, static_cast<size_t>(nvec) * 8, hipMemcpyDeviceToHost, st)) ||
-        !ok(hipStreamSynchronize(st)))
-        return -1;
     for (int i = 0; i < nvec; ++i)
         if (!(w_host[i] == w_host[i])) return n + 1;
     return 0;
