@@ -40,10 +40,13 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-workers", type=int, default=8)
+    p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--serve-requests", type=int, default=24)
+    p.add_argument("--serve-max-atoms", type=int, default=16)
     p.add_argument("--davidson-iters", type=int, default=60)
-    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "copy"], default="full",
+    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "copy"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
-                        "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; copy: a plain 1 GiB device copy, the reference for HBM byte counters")
+                        "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; serve: the request server only; copy: a plain 1 GiB device copy, the reference for HBM byte counters")
     p.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     return p.parse_args()
 
@@ -139,6 +142,67 @@ def batched_kpoints(args):
         return {"error": str(e)}
 
 
+def serving(args):
+    """A Poisson stream (20 req/s, seed 4) of band-structure requests through the continuous-
+    batching server.  The sample: the first --serve-requests requests of the stream with the cell
+    drawn among supercells of at most --serve-max-atoms atoms (the full stream reaches 216 atoms
+    and 2,000 requests, hours of GPU time).  Requests are prepared (basis, V) before the clock
+    and submitted at their arrival times; latency = completion - submission on the server's
+    clock."""
+    from q3_amd import BandRequestSpec, BandStructureServer, FP64
+    from q3_amd import workloads as W
+    from q3_amd.planewave import PlanewaveBasis
+    try:
+        reqs = W.cfg5_requests(args.serve_requests, rate_per_s=20.0, seed=4, max_atoms=args.serve_max_atoms)
+        sp = W.serving_species()
+        specs = []
+        t0 = time.time()
+        for r in reqs:
+            basis = PlanewaveBasis(r.crystal, r.ecutwfc_ry, 0.0, True, (1, 1, 1))
+            V = W.local_potential(r.crystal, basis, sp)
+            bs, db = W.d_blocks_for(r.crystal, sp)
+            specs.append(BandRequestSpec(basis, V, r.kpoints, sp, bs, db, r.nbands, r.noccupied,
+                                         tolerance_ry=2e-6, max_iterations=args.davidson_iters, precision=FP64))
+        log(f"serving setup {time.time() - t0:.1f}s: {len(specs)} requests")
+        srv = BandStructureServer(workers=args.batch_workers)
+        srv.wait(srv.submit(specs[0]))  # warm the workers' workspaces
+        start = time.perf_counter()
+        tickets = []
+        for r, spec in zip(reqs, specs):
+            delay = start + r.arrival_s - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(spec))
+        results = [srv.wait(t) for t in tickets]
+        srv.close()
+        lat = np.array([st["done_s"] - st["submit_s"] for _, st in results])
+        span = max(st["done_s"] for _, st in results) - min(st["submit_s"] for _, st in results)
+        nks = sum(len(r.kpoints) for r in reqs)
+        nat = [len(r.crystal.species) for r in reqs]
+        # accuracy sample: the first two requests re-solved alone to 1e-10 Ry
+        ref_err = 0.0
+        srv = BandStructureServer(workers=args.batch_workers)
+        for i in range(min(2, len(specs))):
+            s = specs[i]
+            tight = BandRequestSpec(s.basis, s.v_r, s.kpoints, s.species, s.block_sizes, s.d_blocks, s.nbands,
+                                    s.noccupied, tolerance_ry=1e-10, max_iterations=4 * args.davidson_iters)
+            e_ref, _ = srv.wait(srv.submit(tight))
+            ref_err = max(ref_err, float(np.abs(results[i][0] - e_ref)[:, :s.noccupied].max() / 2))
+        srv.close()
+        out = {"sample": f"first {len(reqs)} requests of the seed-4 stream at 20 req/s, cells of <= "
+                         f"{args.serve_max_atoms} atoms ({min(nat)}..{max(nat)}), {nks} k-point solves",
+               "workers": args.batch_workers, "requests": len(reqs), "p50_latency_s": float(np.percentile(lat, 50)),
+               "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
+               "solves_per_s": len(reqs) / span, "kpoint_solves_per_s": nks / span, "span_s": span,
+               "max_abs_err_ha_occupied_sampled": ref_err, "tolerance_ry": 2e-6}
+        log(f"serving: p50 {out['p50_latency_s']:.2f}s p99 {out['p99_latency_s']:.2f}s, "
+            f"{out['solves_per_s']:.2f} req/s, {out['kpoint_solves_per_s']:.1f} k-solves/s")
+        return out
+    except Exception as e:  # reported, not hidden
+        log(f"serving failed: {e}")
+        return {"error": str(e)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -151,6 +215,9 @@ def main():
         return copy_reference(args.steps)
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", **batched_kpoints(args)}))
+        return 0
+    if args.mode == "serve":
+        print(json.dumps({"mode": "serve", **serving(args)}))
         return 0
 
     from q3_amd import FP64, MIXED, Context, PlanewaveDavidsonDiagonalizer, PlanewaveHamiltonianApplicator
@@ -189,7 +256,13 @@ def main():
             ctx.synchronize()
             t = time.perf_counter()
             d.solve(nb, w.noccupied, psi0, precision=prec)
-            rep["fp64" if prec == FP64 else "mixed"] = {"s_per_solve": time.perf_counter() - t, **d.report_dict()}
+            dt = time.perf_counter() - t
+            tot = (ctypes.c_double * 12)()
+            cnt = (ctypes.c_long * 12)()
+            check(L.q3x_profile_read(ctx.h, 12, tot, cnt, 1))
+            kern = {L.q3x_profile_kernel_name(i).decode(): {"total_ms": tot[i], "calls": int(cnt[i])}
+                    for i in range(12) if cnt[i]}
+            rep["fp64" if prec == FP64 else "mixed"] = {"s_per_solve": dt, **d.report_dict(), "kernels": kern}
         print(json.dumps({"mode": "davidson", **rep}))
         return 0
     nbytes = npw * nb * 16
@@ -217,7 +290,7 @@ def main():
         dt = time.perf_counter() - t
         tot = (ctypes.c_double * 6)()
         cnt = (ctypes.c_long * 6)()
-        check(L.q3x_profile_read(ctx.h, 6, tot, cnt, 1))
+        check(L.q3x_profile_read(ctx.h, 6, tot, cnt, 1))  # the six H|psi> kernels
         check(L.q3x_profile_enable(ctx.h, 0))
         kern = {}
         for i in range(6):
@@ -296,6 +369,7 @@ def main():
             log(f"davidson failed: {e}")
 
     batched = None if args.no_batch else batched_kpoints(args)
+    serve = None if args.no_serve else serving(args)
 
     cpu = None
     if not args.no_cpu:
@@ -337,6 +411,7 @@ def main():
         "roofline": roof,
         "davidson": davidson,
         "batched": batched,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -134,7 +134,8 @@ q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigne
 /* ---- per-kernel timing on the context stream (HIP events around each launch; the
  *      reference's ProfilingCounters / StageTimer, src/layer_0_core/profiling/counters.hpp).
  *      Kernel ids: 0 zsticks_inverse, 1 plane_sandwich, 2 zsticks_forward, 3 kb_overlap_gemm,
- *      4 kb_dblock, 5 kb_update_gemm. */
+ *      4 kb_dblock, 5 kb_update_gemm; subspace eigensolver stages: 6 eig_prepare, 7 eig_gemms,
+ *      8 eig_tridiagonal, 9 eig_bisection, 10 eig_inverse_iteration, 11 eig_back_transform. */
 q3x_status q3x_profile_enable(q3x_context ctx, int enable);
 /* synchronizes the stream; fills up to n totals (ms) and launch counts; reset != 0 zeroes them */
 q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* count, int reset);

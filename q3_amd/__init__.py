@@ -4,8 +4,10 @@ The compute path is libq3x.so (hand-written gfx950 HIP kernels + host C++17, C A
 include/q3x.h).  Importing this package does not touch the GPU; the library is loaded on
 first use and its absence raises ImportError (there is no CPU fallback).
 """
-from .planewave import (FP64, MIXED, Context, Crystal, KPoint, KPointBatch, PlanewaveBasis,  # noqa: F401
-                        PlanewaveDavidsonDiagonalizer, PlanewaveHamiltonianApplicator)
+from .planewave import (FP64, MIXED, BandRequestSpec, BandStructureServer, Context, Crystal,  # noqa: F401
+                        KPoint, KPointBatch, PlanewaveBasis, PlanewaveDavidsonDiagonalizer,
+                        PlanewaveHamiltonianApplicator)
 
 __all__ = ["FP64", "MIXED", "Context", "Crystal", "KPoint", "PlanewaveBasis", "PlanewaveHamiltonianApplicator",
-           "PlanewaveDavidsonDiagonalizer", "KPointBatch"]
+           "PlanewaveDavidsonDiagonalizer", "KPointBatch",
+           "BandRequestSpec", "BandStructureServer"]

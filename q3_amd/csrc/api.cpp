@@ -338,8 +338,10 @@ q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* coun
 }
 
 const char* q3x_profile_kernel_name(int id) {
-    static const char* names[K_COUNT] = {"zsticks_inverse", "plane_sandwich", "zsticks_forward",
-                                         "kb_overlap_gemm", "kb_dblock",      "kb_update_gemm"};
+    static const char* names[K_COUNT] = {"zsticks_inverse", "plane_sandwich", "zsticks_forward", "kb_overlap_gemm",
+                                         "kb_dblock",       "kb_update_gemm", "eig_prepare",     "eig_gemms",
+                                         "eig_tridiagonal", "eig_bisection",  "eig_inverse_iteration",
+                                         "eig_back_transform"};
     return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 

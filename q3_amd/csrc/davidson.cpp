@@ -122,7 +122,7 @@ struct Solver {
         hipError_t e = hipSuccess;
         const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
                                       X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
-                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr);
+                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr, &h->ctx->prof);
         if (info < 0) {
             ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
             return -1;

@@ -37,7 +37,11 @@ class DevBuf {
 };
 
 // Per-kernel-family timing with HIP events recorded on the context stream around each launch.
-enum KernelId { K_ZINV = 0, K_PLANE, K_ZFWD, K_KB_OVERLAP, K_KB_DBLOCK, K_KB_UPDATE, K_COUNT };
+enum KernelId {
+    K_ZINV = 0, K_PLANE, K_ZFWD, K_KB_OVERLAP, K_KB_DBLOCK, K_KB_UPDATE,
+    // stages of the subspace eigensolver
+    K_EIG_PREP, K_EIG_GEMM, K_EIG_TRIDIAG, K_EIG_BISECT, K_EIG_INVIT, K_EIG_BACK, K_COUNT
+};
 struct Profiler {
     bool on = false;
     std::vector<hipEvent_t> pool;

@@ -6,6 +6,8 @@
 
 #include <cstddef>
 
+#include "runtime.h"
+
 namespace q3x {
 
 // Device workspace for subspace_eig(n, nvec, ...), in bytes.
@@ -25,6 +27,6 @@ struct SubspaceEigStats {
 // num_cus: compute units of the device (bounds the workgroups of the cooperative reduction).
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats);
+                 int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof = nullptr);
 
 }  // namespace q3x

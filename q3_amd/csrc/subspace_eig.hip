@@ -45,10 +45,27 @@ __device__ __forceinline__ z2 zmulc(z2 a, z2 b) { return {a.x * b.x + a.y * b.y,
 constexpr int kThreads = 1024;  // single-workgroup kernels: 16 wavefronts
 constexpr double kEps = 2.220446049250313e-16;
 
+// Cross-lane move of a double by a DPP pattern (each 32-bit half separately).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes (all active), the same value in every lane: within each row of 16
+// lanes by DPP rotations (row_ror 8, row_ror 4, quad_perm [2,3,0,1], quad_perm [1,0,3,2]),
+// then the four row sums read from lanes 0, 16, 32, 48 in a fixed order.  DPP moves stay in the
+// VALU; a shuffle through ds_bpermute costs an LDS round trip per step.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_mov<0x128>(v);
+    v += dpp_mov<0x124>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0xB1>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -90,15 +107,16 @@ __device__ double block_max(double v, double* red, int& slot) {
 
 // ---------------------------------------------------------------- 1. preparation ----
 // Hf, Sf: full Hermitian n x n (ld n) from the upper triangles; Zl = 1.5 I - 0.5 Sf;
-// err[0] = max |S - I| (componentwise).  One workgroup.
-__global__ __launch_bounds__(kThreads) void herm_prep(int n, const z2* __restrict__ H, long ldh,
-                                                      const z2* __restrict__ S, long lds, z2* __restrict__ Hf,
-                                                      z2* __restrict__ Sf, z2* __restrict__ Zl, double* err) {
-    __shared__ double red[32];
-    int slot = 0;
+// *err_bits = bits of max |S - I| (componentwise; non-negative doubles order like their bit
+// patterns, so the maximum over workgroups is an integer atomic max; *err_bits starts at 0).
+__global__ __launch_bounds__(256) void herm_prep(int n, const z2* __restrict__ H, long ldh,
+                                                 const z2* __restrict__ S, long lds, z2* __restrict__ Hf,
+                                                 z2* __restrict__ Sf, z2* __restrict__ Zl,
+                                                 unsigned long long* err_bits) {
+    __shared__ double red[4];
     double dev = 0.0;
     const long nn = static_cast<long>(n) * n;
-    for (long e = threadIdx.x; e < nn; e += blockDim.x) {
+    for (long e = blockIdx.x * 256L + threadIdx.x; e < nn; e += 256L * gridDim.x) {
         const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
         z2 h, s;
         if (i <= j) {
@@ -121,8 +139,14 @@ __global__ __launch_bounds__(kThreads) void herm_prep(int n, const z2* __restric
         Hf[e] = h;
         Sf[e] = s;
     }
-    const double m = block_max(dev, red, slot);
-    if (threadIdx.x == 0) err[0] = m;
+    if (dev != dev) dev = INFINITY;
+    dev = wave_max(dev);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dev;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(err_bits, static_cast<unsigned long long>(__double_as_longlong(m)));
+    }
 }
 
 // Y = Sf / c, Z = I
@@ -895,7 +919,15 @@ size_t subspace_eig_workspace(int n, int nvec) { return layout(std::max(n, 1), s
 
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats) {
+                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof) {
+    // stage timing on the caller's profiler (events on this stream; nothing when it is off)
+    int mark_prev = prof ? prof->mark(st) : -1;
+    auto stage_done = [&](int id) {
+        if (!prof || !prof->on) return;
+        const int m = prof->mark(st);
+        prof->span(id, mark_prev, m);
+        mark_prev = m;
+    };
     *err = hipSuccess;
     if (n <= 0 || nvec <= 0) return 0;
     nvec = std::min(nvec, n);
@@ -921,9 +953,12 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     const long nn = static_cast<long>(n) * n;
     const unsigned eg = static_cast<unsigned>(std::min<long>((nn + 255) / 256, 1024));
 
-    hipLaunchKernelGGL(herm_prep, dim3(1), dim3(kThreads), 0, st, n, reinterpret_cast<const z2*>(H), ldh,
-                       reinterpret_cast<const z2*>(S), lds, Hf, Sf, Zl, errd);
+    hipLaunchKernelGGL(zero_words, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned*>(errd), 2);
+    hipLaunchKernelGGL(herm_prep, dim3(static_cast<unsigned>(std::min<long>((nn + 255) / 256, 512))), dim3(256), 0,
+                       st, n, reinterpret_cast<const z2*>(H), ldh, reinterpret_cast<const z2*>(S), lds, Hf, Sf, Zl,
+                       reinterpret_cast<unsigned long long*>(errd));
     if (!ok(hipGetLastError())) return -1;
+    stage_done(K_EIG_PREP);
     // Newton-Schulz on S / c with c >= lambda_max(S): Y -> (S/c)^(1/2), Z -> (S/c)^(-1/2);
     // replaces the first-order factor in Zl.  0, n + 1 (no convergence: S not positive
     // definite) or -1.
@@ -960,6 +995,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     auto solve_from_zl = [&](double* sdev_out) -> int {
         // C = Zl Hf Zl
         if (!gemm(0, n, n, n, Hf, n, Zl, n, T1, n) || !gemm(0, n, n, n, Zl, n, T1, n, C, n)) return -1;
+        stage_done(K_EIG_GEMM);
         // tridiagonalisation
         double *d = D(B_D), *e = D(B_E), *tau = D(B_TAU), *w = D(B_W);
         z2 *beta = P(B_BETA), *ph = P(B_PH);
@@ -1034,11 +1070,14 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
         else Q3X_TRD(false, 16);
     #undef Q3X_TRD
         if (!ok(hipGetLastError())) return -1;
+        stage_done(K_EIG_TRIDIAG);
         hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(64), static_cast<size_t>(n) * 16, st, n, d, e, w);
         if (!ok(hipGetLastError())) return -1;
+        stage_done(K_EIG_BISECT);
         hipLaunchKernelGGL(invit_kernel, dim3(1), dim3(kThreads), 0, st, n, d, e, w, nvec, D(B_ZT), D(B_U1), D(B_U2),
                            D(B_U3), D(B_LO), reinterpret_cast<int*>(base + L.off[B_SW]));
         if (!ok(hipGetLastError())) return -1;
+        stage_done(K_EIG_INVIT);
         z2* Yb = P(B_YB);
         const int rpl = (n + 63) / 64;
     #define Q3X_BT(R)                                                                                              \
@@ -1062,6 +1101,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
                            0, st, nvec, G);
         if (!ok(hipGetLastError())) return -1;
         if (!gemm(0, n, nvec, nvec, Yb, n, G, nvec, Y2, n) || !gemm(0, n, nvec, n, Zl, n, Y2, n, X, ldx)) return -1;
+        stage_done(K_EIG_BACK);
         int coop_fail = 0;
         if (!ok(hipMemcpyAsync(w_host, w, static_cast<size_t>(nvec) * 8, hipMemcpyDeviceToHost, st)) ||
             !ok(hipMemcpyAsync(sdev_out, errd, 8, hipMemcpyDeviceToHost, st)) ||

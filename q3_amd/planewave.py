@@ -363,3 +363,115 @@ class KPointBatch:
 
     def __del__(self):
         self.close()
+
+
+def _species_arrays(species):
+    nmesh = np.ascontiguousarray([len(s["r"]) for s in species], dtype=np.int32)
+    return {
+        "nmesh": nmesh,
+        "r_all": np.ascontiguousarray(np.concatenate([s["r"] for s in species]), dtype=np.float64),
+        "rab_all": np.ascontiguousarray(np.concatenate([s["rab"] for s in species]), dtype=np.float64),
+        "nbeta": np.ascontiguousarray([len(s["l"]) for s in species], dtype=np.int32),
+        "l_all": np.ascontiguousarray(np.concatenate([s["l"] for s in species]), dtype=np.int32),
+        "cutoff_all": np.ascontiguousarray(np.concatenate([s["cutoff"] for s in species]), dtype=np.int32),
+        "beta_r_all": np.ascontiguousarray(np.concatenate([np.asarray(s["beta_r"]).ravel() for s in species]),
+                                           dtype=np.float64),
+    }
+
+
+@dataclass
+class BandRequestSpec:
+    """One band-structure request: a cell (basis), its V(r), the k-points and the band count."""
+    basis: PlanewaveBasis
+    v_r: np.ndarray
+    kpoints: np.ndarray
+    species: list
+    block_sizes: np.ndarray
+    d_blocks: np.ndarray
+    nbands: int
+    noccupied: int
+    tolerance_ry: float = 2e-6
+    max_iterations: int = 60
+    precision: int = FP64
+
+
+class BandStructureServer:
+    """Continuous-batching server of band-structure requests (q3x_server_*): requests are
+    admitted in arrival order while their estimated HBM footprint fits the budget, and the
+    k-points of all admitted requests share the worker streams."""
+
+    def __init__(self, workers: int = 8, hbm_budget_bytes: float = 0.0, device: int = 0):
+        h = ctypes.c_void_p()
+        check(lib().q3x_server_create(device, workers, float(hbm_budget_bytes), ctypes.byref(h)))
+        self.h = h
+        self._live = {}  # ticket -> (spec, arrays) kept alive until the result is collected
+
+    def _request(self, spec: BandRequestSpec):
+        c = spec.basis.crystal
+        arrays = {
+            "v": np.ascontiguousarray(spec.v_r, dtype=np.float64).ravel(),
+            "k": np.ascontiguousarray(spec.kpoints, dtype=np.float64).reshape(-1, 3),
+            "pos": np.ascontiguousarray(c.positions, dtype=np.float64),
+            "sp": np.ascontiguousarray(c.species, dtype=np.int32),
+            "bs": np.ascontiguousarray(spec.block_sizes, dtype=np.int32),
+            "db": np.ascontiguousarray(spec.d_blocks, dtype=np.float64),
+            **_species_arrays(spec.species),
+        }
+        r = _lib.BandRequest()
+        r.basis = spec.basis.h
+        r.v_r = arrays["v"].ctypes.data
+        r.nv = arrays["v"].size
+        r.nk = arrays["k"].shape[0]
+        r.kpoints = arrays["k"].ctypes.data
+        r.natoms = len(arrays["sp"])
+        r.atom_pos = arrays["pos"].ctypes.data
+        r.atom_species = arrays["sp"].ctypes.data
+        r.nspecies = len(spec.species)
+        for name in ("nmesh", "r_all", "rab_all", "nbeta", "l_all", "cutoff_all", "beta_r_all"):
+            setattr(r, name, arrays[name].ctypes.data)
+        r.block_sizes = arrays["bs"].ctypes.data
+        r.d_blocks = arrays["db"].ctypes.data
+        r.nbands = spec.nbands
+        r.noccupied = spec.noccupied
+        lib().q3x_davidson_default_options(ctypes.byref(r.options))
+        r.options.tolerance_ry = spec.tolerance_ry
+        r.options.max_iterations = spec.max_iterations
+        r.options.precision = spec.precision
+        return r, arrays
+
+    def estimate(self, spec: BandRequestSpec) -> float:
+        r, _arrays = self._request(spec)
+        out = ctypes.c_double()
+        check(lib().q3x_server_estimate(self.h, ctypes.byref(r), ctypes.byref(out)))
+        return out.value
+
+    def submit(self, spec: BandRequestSpec) -> int:
+        r, arrays = self._request(spec)
+        t = ctypes.c_long()
+        check(lib().q3x_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        self._live[t.value] = (spec, arrays)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Returns (eigenvalues nk x nb [Ry], stats dict) and frees the request's device state."""
+        spec, arrays = self._live[ticket]
+        eig = np.zeros((arrays["k"].shape[0], spec.nbands))
+        st = _lib.RequestStats()
+        try:
+            check(lib().q3x_server_wait(self.h, ticket, _ptr(eig), ctypes.byref(st)))
+        finally:
+            del self._live[ticket]
+        return eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
+                     "footprint_bytes": st.footprint_bytes, "iterations": st.iterations}
+
+    def in_flight_bytes(self) -> float:
+        return lib().q3x_server_in_flight_bytes(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().q3x_server_destroy(self.h)
+            self.h = None
+            self._live.clear()
+
+    def __del__(self):
+        self.close()

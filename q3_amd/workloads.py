@@ -159,3 +159,71 @@ def small_test_cell(ecutwfc_ry: float = 12.0) -> Workload:
     w.ecutwfc_ry = ecutwfc_ry
     w.name = "si2_fcc_small"
     return w
+
+
+# ---------------------------------------------------------------- cfg5: serving stream ----
+CARBON_R_LOC = 0.35
+# (lattice kind, supercell repetitions of the conventional cube, atoms): primitive FCC cell to
+# the 3x3x3 conventional supercell
+SUPERCELLS = [("fcc", (1, 1, 1), 2), ("sc", (1, 1, 1), 8), ("sc", (2, 1, 1), 16), ("sc", (2, 2, 1), 32),
+              ("sc", (2, 2, 2), 64), ("sc", (3, 2, 2), 96), ("sc", (3, 3, 2), 144), ("sc", (3, 3, 3), 216)]
+A_SI, A_C = 10.26, 6.74
+
+
+def carbon_like_species():
+    """Second synthetic species of the Si/C alloy: same form, D from seed 1, r_loc 0.35 bohr."""
+    sp = synthetic_species(1)
+    sp["r_loc"] = CARBON_R_LOC
+    return sp
+
+
+@dataclass
+class ServingRequest:
+    arrival_s: float
+    crystal: Crystal
+    ecutwfc_ry: float
+    kpoints: np.ndarray
+    nbands: int
+    noccupied: int
+    carbon_fraction: float
+
+
+def alloy_cell(reps, kind: str, x: float, rng) -> Crystal:
+    """Si_(1-x) C_x diamond cell: each site is C with probability x; a interpolated linearly."""
+    a = A_SI + (A_C - A_SI) * x
+    if kind == "fcc":
+        lattice, frac = FCC.copy(), np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]])
+    else:
+        r = np.array(reps, dtype=float)
+        cells = np.array([[i, j, k] for i in range(reps[0]) for j in range(reps[1]) for k in range(reps[2])])
+        frac = np.array([(c + b) / r for c in cells for b in DIAMOND_CONVENTIONAL])
+        lattice = np.diag(r)
+    species = (rng.uniform(size=len(frac)) < x).astype(np.int32)
+    return Crystal(lattice, a, frac, species)
+
+
+def cfg5_requests(n_requests: int = 2000, rate_per_s: float = 20.0, seed: int = 4,
+                  max_atoms: int = 216) -> list:
+    """Poisson stream of band-structure requests (seed 4): Si/C alloy cells from the primitive
+    cell to 3x3x3 conventional (C fraction U[0, 0.5], a from 10.26 to 6.74 bohr), ecutwfc
+    U[15, 25] Ha, 8-40 k-points along a random straight line, bands = 2 N_atoms + 8.
+    max_atoms < 216 draws the cell among the smaller supercells only."""
+    rng = np.random.default_rng(seed)
+    cells = [c for c in SUPERCELLS if c[2] <= max_atoms]
+    t = 0.0
+    out = []
+    for _ in range(n_requests):
+        t += rng.exponential(1.0 / rate_per_s)
+        kind, reps, nat = cells[int(rng.integers(len(cells)))]
+        x = float(rng.uniform(0.0, 0.5))
+        crystal = alloy_cell(reps, kind, x, rng)
+        ecut = float(rng.uniform(15.0, 25.0)) * HA
+        nk = int(rng.integers(8, 41))
+        k0, k1 = rng.uniform(-0.5, 0.5, 3), rng.uniform(-0.5, 0.5, 3)
+        ks = k0[None, :] + np.linspace(0.0, 1.0, nk)[:, None] * (k1 - k0)[None, :]
+        out.append(ServingRequest(t, crystal, ecut, ks, 2 * nat + 8, 2 * nat, x))
+    return out
+
+
+def serving_species():
+    return [synthetic_species(0), carbon_like_species()]
