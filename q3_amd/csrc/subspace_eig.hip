@@ -15,8 +15,9 @@
 //      the matrix sits in LDS.
 //   4. T = D Tr D^H with a real tridiagonal Tr (phases D); eigenvalues of Tr by multisection:
 //      one wavefront per eigenvalue, 63 Sturm counts per round, 6 bits per round.
-//   5. Eigenvectors of Tr by inverse iteration, one thread per vector, in parallel; vectors
-//      of (nearly) equal eigenvalues are orthogonalised against each other in rounds.
+//   5. Eigenvectors of Tr, one thread per vector, in parallel: an isolated eigenvalue by one
+//      twisted factorisation (the core step of MRRR); eigenvalues closer than 1e-6 |T| by
+//      inverse iteration, orthogonalised against each other in rounds.
 //   6. Y = Q D Z_tr (reflectors applied per column, one wavefront each), Y <- Y (1.5 I - 0.5
 //      Y^H Y) to restore orthonormality to rounding, X = Z Y.
 #include "subspace_eig.h"
@@ -753,6 +754,59 @@ __global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __
                 mu = m;
             }
             auto at = [nvec, c](int i) { return static_cast<long>(i) * nvec + c; };
+            const bool isolated = c == c0 && (c + 1 >= nvec || w[c + 1] - w[c] > ctol);
+            if (isolated && n >= 2) {
+                // Twisted factorisation of Tr - mu I (the core step of the MRRR algorithm,
+                // Dhillon & Parlett, LAPACK's dlar1v): L+ D+ L+^T from the top, U- D- U-^T from
+                // the bottom, twist index r where |gamma_r| = |D+_r + D-_r - (d_r - mu)| is
+                // smallest, then z_r = 1 and the two one-term recurrences outwards.  One pass
+                // each way instead of repeated LU solves; for an eigenvalue apart from the others
+                // the vector is accurate to about eps |T| / gap.  U1 holds L+, U2 D+, U3 U-.
+                double dp = d[0] - mu;
+#pragma unroll 4
+                for (int i = 0; i + 1 < n; ++i) {
+                    if (fabs(dp) < pivmin) dp = dp >= 0.0 ? pivmin : -pivmin;
+                    const double l = e[i + 1] / dp;
+                    U1[at(i)] = l;
+                    U2[at(i)] = dp;
+                    dp = (d[i + 1] - mu) - l * e[i + 1];
+                }
+                if (fabs(dp) < pivmin) dp = dp >= 0.0 ? pivmin : -pivmin;
+                U2[at(n - 1)] = dp;
+                double dm = d[n - 1] - mu;
+                double best = fabs(dp + dm - (d[n - 1] - mu));
+                int rt = n - 1;
+#pragma unroll 4
+                for (int i = n - 1; i >= 1; --i) {
+                    if (fabs(dm) < pivmin) dm = dm >= 0.0 ? pivmin : -pivmin;
+                    const double u = e[i] / dm;
+                    U3[at(i)] = u;
+                    dm = (d[i - 1] - mu) - u * e[i];
+                    const double g = fabs(U2[at(i - 1)] + dm - (d[i - 1] - mu));
+                    if (g < best) {
+                        best = g;
+                        rt = i - 1;
+                    }
+                }
+                double z = 1.0, nrm = 1.0;
+                Zt[at(rt)] = 1.0;
+#pragma unroll 4
+                for (int i = rt - 1; i >= 0; --i) {
+                    z = -U1[at(i)] * z;
+                    Zt[at(i)] = z;
+                    nrm += z * z;
+                }
+                z = 1.0;
+#pragma unroll 4
+                for (int i = rt + 1; i < n; ++i) {
+                    z = -U3[at(i)] * z;
+                    Zt[at(i)] = z;
+                    nrm += z * z;
+                }
+                const double inv = 1.0 / sqrt(nrm);
+                for (int i = 0; i < n; ++i) Zt[at(i)] *= inv;
+                continue;
+            }
             // LU with partial pivoting of Tr - mu I: row i keeps (u1 diag, u2, u3 super-diagonals).
             // The pending row i+1 is carried in registers, so the loop only stores.
             double u1 = d[0] - mu, u2 = n > 1 ? e[1] : 0.0;
