@@ -29,7 +29,7 @@ EXTRA = {
                   "-fno-math-errno"],
     "davidson.cpp": ["-fopenmp"],
 }
-SOURCES = ["setup.cpp", "kb_projectors.cpp", "api.cpp", "hlocal.hip", "zgemm.hip", "aux_kernels.hip",
+SOURCES = ["setup.cpp", "kb_projectors.cpp", "api.cpp", "hlocal.hip", "plane_tl.hip", "zgemm.hip", "aux_kernels.hip",
            "subspace_eig.hip", "dense.cpp", "davidson.cpp", "batch.cpp", "server.cpp"]
 
 

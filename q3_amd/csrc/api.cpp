@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -139,8 +140,19 @@ bool build_tables(FftTables& t, int n, hipStream_t st) {
             }
         ns *= R;
     }
+    std::vector<double> pd(2 * n);
+    std::vector<float> pf(2 * n);
+    for (int j = 0; j < n; ++j) {
+        const long double a = -two_pi * static_cast<long double>(j) / n;
+        pd[2 * j] = static_cast<double>(cosl(a));
+        pd[2 * j + 1] = static_cast<double>(sinl(a));
+        pf[2 * j] = static_cast<float>(pd[2 * j]);
+        pf[2 * j + 1] = static_cast<float>(pd[2 * j + 1]);
+    }
     return t.tw64.upload(d.data(), d.size() * sizeof(double), st) &&
-           t.tw32.upload(f.data(), f.size() * sizeof(float), st);
+           t.tw32.upload(f.data(), f.size() * sizeof(float), st) &&
+           t.plain64.upload(pd.data(), pd.size() * sizeof(double), st) &&
+           t.plain32.upload(pf.data(), pf.size() * sizeof(float), st);
 }
 
 HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
@@ -151,6 +163,7 @@ HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
     a.tw1 = fp32 ? h->fft[0].tw32.get() : h->fft[0].tw64.get();
     a.tw2 = fp32 ? h->fft[1].tw32.get() : h->fft[1].tw64.get();
     a.tw3 = fp32 ? h->fft[2].tw32.get() : h->fft[2].tw64.get();
+    a.twp = fp32 ? h->fft[0].plain32.get() : h->fft[0].plain64.get();
     a.nst = h->nst;
     a.ncol = h->ncol;
     a.sticks_per_block = h->sticks_per_block;
@@ -167,9 +180,10 @@ HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
     return a;
 }
 
-// Largest band chunk whose stick buffer stays under 2 GiB.
-int band_chunk(const q3x_hamiltonian_s* h, int nvec) {
-    const size_t per = static_cast<size_t>(h->dims[2]) * h->nst * 16;
+// Largest band chunk whose stick buffer (and plane slots, for planes kept outside LDS) stays under 2 GiB.
+int band_chunk(const q3x_hamiltonian_s* h, const PlaneTlConfig& tl, int nvec) {
+    size_t per = static_cast<size_t>(h->dims[2]) * h->nst * 16;
+    if (tl.ok && tl.plane_global) per += static_cast<size_t>(h->dims[2]) * tl.plane_bytes;
     const size_t cap = size_t(2) << 30;
     int c = static_cast<int>(std::max<size_t>(1, cap / std::max<size_t>(per, 1)));
     return std::min(c, nvec);
@@ -195,9 +209,12 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
     const bool fp32 = precision == Q3X_MIXED;
     hipStream_t st = h->ctx->stream;
     const HLocalArgs args = make_args(h, fp32);
-    const int chunk = band_chunk(h, nvec);
+    const PlaneTlConfig& tl = fp32 ? h->tl32 : h->tl64;
+    const int chunk = band_chunk(h, tl, nvec);
     const size_t cs = fp32 ? 8 : 16;
     if (!h->zbuf.ensure(static_cast<size_t>(chunk) * h->dims[2] * h->nst * cs)) return Q3X_ERR_DEVICE;
+    if (tl.ok && tl.plane_global && !h->plane_slots.ensure(static_cast<size_t>(chunk) * h->dims[2] * tl.plane_bytes))
+        return Q3X_ERR_DEVICE;
     const void* vplane = fp32 ? h->v32.get() : h->v64.get();
     Profiler& pr = h->ctx->prof;
     for (int b0 = 0; b0 < nvec; b0 += chunk) {
@@ -216,7 +233,8 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
             }
         }
         if (!hip_ok(launch_hlocal(args, fp32, vplane, h->zbuf.get(), d_psi + 2L * b0 * ldpsi, ldpsi,
-                                  d_hpsi + 2L * b0 * ldh, ldh, nb, st, pr.on ? marks : nullptr),
+                                  d_hpsi + 2L * b0 * ldh, ldh, nb, st, pr.on ? marks : nullptr,
+                                  &tl, h->plane_slots.get()),
                     "H_local kernels"))
             return Q3X_ERR_DEVICE;
         pr.span(K_ZINV, mi[0], mi[1]);
@@ -493,7 +511,14 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
     h->n_outside = static_cast<int>(out.size());
     const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
     h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
-    // plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of 8 waves
+    // two-level plane kernel where the plane is square and of a size it is built for;
+    // Q3X_PLANE_KERNEL=generic keeps the Stockham kernel (used to compare the two)
+    const char* pk = std::getenv("Q3X_PLANE_KERNEL");
+    const bool use_tl = !(pk && std::strcmp(pk, "generic") == 0) && plane_tl_supported(n1, n2);
+    h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds) : PlaneTlConfig{};
+    h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds) : PlaneTlConfig{};
+    // generic plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of
+    // 8 waves
     for (int fp32 = 0; fp32 < 2; ++fp32) {
         const size_t cs = fp32 ? 8 : 16;
         const int ps = fp32 ? lds_pad_shift<cx<float>>() : lds_pad_shift<cx<double>>();
@@ -503,11 +528,11 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
         if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * l1 * 8));
         rp = std::min<long>(rp, 4);
         rp = std::min<long>(rp, n2);
-        if (rp < 1) {
+        if (rp < 1 && !(fp32 ? h->tl32.ok : h->tl64.ok)) {
             set_error("prepare_kpoint: the occupied plane does not fit in LDS for this box");
             return Q3X_ERR_ARGUMENT;
         }
-        (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(rp);
+        (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(std::max<long>(rp, 1));
     }
     bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st) &&
               h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st) &&

@@ -11,6 +11,7 @@ struct HLocalArgs {
     const void* tw1;              // twiddle tables exp(-2 pi i k/n), element type = compute type
     const void* tw2;
     const void* tw3;
+    const void* twp;              // exp(-2 pi i j/n1), j < n1, for the two-level plane kernel
     int nst;                      // occupied z-sticks
     int ncol;                     // occupied i1 columns
     int sticks_per_block;
@@ -27,12 +28,30 @@ struct HLocalArgs {
 };
 
 size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which);
+
+// Two-level plane kernel (plane_tl.hip) for square planes n1 = n2 of the sizes it is built for.
+struct PlaneTlConfig {
+    bool ok = false;
+    int lc = 0, lr = 0;  // LDS strides (complex elements) of a column and of a row in flight
+    int rb = 0;          // rows in flight per chunk
+    int threads = 0;
+    size_t lds = 0;
+    bool plane_global = false;  // plane too large for LDS: kept in a global slot per workgroup
+    size_t plane_bytes = 0;     // bytes of one plane slot
+};
+bool plane_tl_supported(int n1, int n2);
+PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds);
+// pglob: nvec * n3 plane slots of c.plane_bytes when c.plane_global, else unused
+template <typename T>
+hipError_t launch_plane_tl(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
+                           void* pglob, hipStream_t st);
 // psi / hpsi: complex128, column per band with leading dimension ld (in complex elements).
 // marks (optional): 4 events recorded before/between/after the three kernels.
 // vplane: V(r)/N in plane-major layout [i3][i2][i1], double or float to match fp32.
 // zbuf: nvec * n3 * nst complex elements of the compute type.
+// tl: configuration of the two-level plane kernel, or null (or !ok) for the generic one.
 hipError_t launch_hlocal(const HLocalArgs& a, bool fp32, const void* vplane, void* zbuf, const double* psi,
                          long ldpsi, double* hpsi, long ldh, int nvec, hipStream_t st,
-                         hipEvent_t* marks = nullptr);
+                         hipEvent_t* marks = nullptr, const PlaneTlConfig* tl = nullptr, void* pglob = nullptr);
 
 }  // namespace q3x

@@ -207,7 +207,8 @@ size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which) {
 
 template <typename T>
 hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, const double* psi, long ldpsi,
-                           double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks) {
+                           double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks,
+                           const PlaneTlConfig* tl, void* pglob) {
     if (marks) (void)hipEventRecord(marks[0], st);
     const bool fp32 = sizeof(T) == 4;
     const int nsb_blocks = (a.nst + a.sticks_per_block - 1) / a.sticks_per_block;
@@ -232,9 +233,13 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (marks) (void)hipEventRecord(marks[1], st);
-    hipLaunchKernelGGL(plane_sandwich<T>, dim3(a.p3.n, nvec), dim3(512), lds_p, st, a, vplane,
-                       static_cast<cx<T>*>(zbuf));
-    e = hipGetLastError();
+    if (tl && tl->ok) {
+        e = launch_plane_tl<T>(a, vplane, static_cast<cx<T>*>(zbuf), *tl, nvec, pglob, st);
+    } else {
+        hipLaunchKernelGGL(plane_sandwich<T>, dim3(a.p3.n, nvec), dim3(512), lds_p, st, a, vplane,
+                           static_cast<cx<T>*>(zbuf));
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) return e;
     if (marks) (void)hipEventRecord(marks[2], st);
     hipLaunchKernelGGL(zsticks_forward<T>, g1, dim3(256), lds_z, st, a, static_cast<const cx<T>*>(zbuf), psi, ldpsi,
@@ -251,12 +256,13 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
 }
 
 hipError_t launch_hlocal(const HLocalArgs& a, bool fp32, const void* vplane, void* zbuf, const double* psi,
-                         long ldpsi, double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks) {
+                         long ldpsi, double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks,
+                         const PlaneTlConfig* tl, void* pglob) {
     if (fp32)
         return launch_hlocal_t<float>(a, static_cast<const float*>(vplane), zbuf, psi, ldpsi, hpsi, ldh, nvec, st,
-                                      marks);
+                                      marks, tl, pglob);
     return launch_hlocal_t<double>(a, static_cast<const double*>(vplane), zbuf, psi, ldpsi, hpsi, ldh, nvec, st,
-                                   marks);
+                                   marks, tl, pglob);
 }
 
 }  // namespace q3x

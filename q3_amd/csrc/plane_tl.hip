@@ -1,0 +1,308 @@
+// Plane stage of the local Hamiltonian (the yz/xy part of serial.hpp:456-478) with two-level
+// register transforms, for boxes whose plane is square (n1 = n2 = N = A * B).
+//
+// Per (band, i3) plane, as plane_sandwich in hlocal.hip: scatter the plane's z-stick values
+// into the occupied i1 columns, backward FFT along i2, per i2 row backward FFT along i1,
+// times V(r)/N, forward FFT along i1 keeping the occupied columns, forward FFT along i2,
+// write the sticks back.  The difference is how a length-N transform runs: as two register
+// stages (the "four-step" split N = A * B) instead of one LDS round trip per radix.
+//
+//   backward, natural -> digit-permuted:  n = B n1 + n2,  k = k1 + A k2
+//     stage 1 (lane n2):  y[k1] = sum_n1 x[B n1 + n2] w_A^(n1 k1),   y[k1] *= w_N^(n2 k1)
+//     stage 2 (lane k1):  X[k1 + A k2] = sum_n2 y_n2[k1] w_B^(n2 k2)
+//   forward, digit-permuted -> natural:   n = n1 + A n2,  m = m2 + B m1
+//     stage A (lane n1):  u[m2] = sum_n2 z[n1 + A n2] w_B^(-n2 m2),  u[m2] *= w_N^(-n1 m2)
+//     stage B (lane m2):  Y[m2 + B m1] = sum_n1 u_n1[m2] w_A^(-n1 m1)
+//
+// Logical slot q of a transform is stored at q + q / B (one pad slot per block of B).  Stage 1
+// and stage B touch slots {B j + lane}, stage 2 and stage A the block {B lane + j}, each lane
+// writing back the slots it read, so every stage runs in place and only the hand-over between
+// stages needs a barrier.  The element k = k1 + A k2 of a backward column sits at slot
+// B k1 + k2, which is where the forward stage A of the same column expects it.  A row's
+// backward stage 2, the V(r) multiply and the forward stage A use the same values of the same
+// lane, so they run on registers with no LDS traffic between them.  Transform strides in LDS
+// are odd, and lanes run across transforms fastest, so that the lanes of a wave hit distinct
+// banks.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+
+#include "fft_lds.h"
+#include "hlocal.h"
+
+namespace q3x {
+
+// PG = false: the plane P lives in LDS.  PG = true (planes too large for LDS): P lives in this
+// workgroup's slot of pglob, [gridDim.y][gridDim.x][ncol][lc], which stays in L2 while the
+// workgroup runs; the rows in flight stay in LDS.
+template <typename T, int A, int B, bool PG>
+__global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
+                                                cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg, int lc,
+                                                int lr, int rb, cx<T>* __restrict__ pglob) {
+    constexpr int N = A * B;
+    constexpr int S = B + 1;  // stored length of a block of B slots
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // exp(-2 pi i j / N), j < N
+    int* colmap = reinterpret_cast<int*>(tw + N);    // i1 -> occupied column or -1
+    cx<T>* lds_next = reinterpret_cast<cx<T>*>(colmap + ((N + 3) & ~3));
+    const int ncol = a.ncol;
+    cx<T>* P = PG ? pglob + (static_cast<long>(blockIdx.y) * gridDim.x + blockIdx.x) * ncol * lc
+                  : lds_next;                // [ncol][lc]
+    cx<T>* R = PG ? lds_next : P + ncol * lc;  // [rb][lr] rows in flight
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int z = blockIdx.x, band = blockIdx.y;
+    const cx<T> zero = {T(0), T(0)};
+
+    for (int i = tid; i < N; i += nt) {
+        tw[i] = twg[i];
+        colmap[i] = -1;
+    }
+    for (int i = tid; i < ncol * lc; i += nt) P[i] = zero;
+    __syncthreads();
+    for (int c = tid; c < ncol; c += nt) colmap[a.col_i1[c]] = c;
+    cx<T>* zp = zbuf + (static_cast<long>(band) * a.p3.n + z) * a.nst;
+    for (int s = tid; s < a.nst; s += nt) {
+        const int q = a.stick_i2[s];
+        P[a.stick_col[s] * lc + q + q / B] = zp[s];
+    }
+    __syncthreads();
+
+    // columns, backward stage 1: task t -> column t % ncol, lane n2 = t / ncol
+    for (int t = tid; t < ncol * B; t += nt) {
+        const int n2 = t / ncol, c = t - n2 * ncol;
+        cx<T>* col = P + c * lc + n2;
+        cx<T> v[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) v[j] = col[j * S];
+        Dft<T, A>::run(v, T(1));
+#pragma unroll
+        for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+#pragma unroll
+        for (int k = 0; k < A; ++k) col[k * S] = v[k];
+    }
+    __syncthreads();
+    // columns, backward stage 2: lane k1 owns the block at k1 * S
+    for (int t = tid; t < ncol * A; t += nt) {
+        const int k1 = t / ncol, c = t - k1 * ncol;
+        cx<T>* blk = P + c * lc + k1 * S;
+        cx<T> u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = blk[j];
+        Dft<T, B>::run(u, T(1));
+#pragma unroll
+        for (int j = 0; j < B; ++j) blk[j] = u[j];
+    }
+    __syncthreads();
+
+    // rows in chunks of rb; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
+    for (int r0 = 0; r0 < N; r0 += rb) {
+        const int nr = min(rb, N - r0);
+        for (int t = tid; t < nr * B; t += nt) {
+            const int n2 = t / nr, r = t - n2 * nr;
+            const int i2 = r0 + r;
+            const int pp = (i2 % A) * S + i2 / A;
+            cx<T> v[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                const int c = colmap[B * j + n2];
+                v[j] = zero;
+                if (c >= 0) v[j] = P[c * lc + pp];
+            }
+            Dft<T, A>::run(v, T(1));
+#pragma unroll
+            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+            cx<T>* row = R + r * lr + n2;
+#pragma unroll
+            for (int k = 0; k < A; ++k) row[k * S] = v[k];
+        }
+        __syncthreads();
+        // backward stage 2, x V(r)/N, forward stage A: lane k1, values at i1 = k1 + A k2
+        for (int t = tid; t < nr * A; t += nt) {
+            const int r = t / A, k1 = t - r * A;
+            const int i2 = r0 + r;
+            cx<T>* blk = R + r * lr + k1 * S;
+            const T* vrow = vplane + (static_cast<long>(z) * N + i2) * N + k1;
+            T vv[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
+            cx<T> u[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) u[j] = blk[j];
+            Dft<T, B>::run(u, T(1));
+#pragma unroll
+            for (int j = 0; j < B; ++j) u[j] = cscale(u[j], vv[j]);
+            Dft<T, B>::run(u, T(-1));
+#pragma unroll
+            for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[k1 * m]);
+#pragma unroll
+            for (int j = 0; j < B; ++j) blk[j] = u[j];
+        }
+        __syncthreads();
+        // forward stage B: lane m2, outputs i1 = m2 + B m1, occupied columns only
+        for (int t = tid; t < nr * B; t += nt) {
+            const int m2 = t / nr, r = t - m2 * nr;
+            const int i2 = r0 + r;
+            const int pp = (i2 % A) * S + i2 / A;
+            const cx<T>* row = R + r * lr + m2;
+            cx<T> v[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) v[j] = row[j * S];
+            Dft<T, A>::run(v, T(-1));
+#pragma unroll
+            for (int m = 0; m < A; ++m) {
+                const int c = colmap[m2 + B * m];
+                if (c >= 0) P[c * lc + pp] = v[m];
+            }
+        }
+        __syncthreads();
+    }
+
+    // columns, forward stage A: lane n1 owns the block at n1 * S (elements n1 + A n2)
+    for (int t = tid; t < ncol * A; t += nt) {
+        const int n1 = t / ncol, c = t - n1 * ncol;
+        cx<T>* blk = P + c * lc + n1 * S;
+        cx<T> u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = blk[j];
+        Dft<T, B>::run(u, T(-1));
+#pragma unroll
+        for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+#pragma unroll
+        for (int j = 0; j < B; ++j) blk[j] = u[j];
+    }
+    __syncthreads();
+    // columns, forward stage B: natural order out
+    for (int t = tid; t < ncol * B; t += nt) {
+        const int m2 = t / ncol, c = t - m2 * ncol;
+        cx<T>* col = P + c * lc + m2;
+        cx<T> v[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) v[j] = col[j * S];
+        Dft<T, A>::run(v, T(-1));
+#pragma unroll
+        for (int m = 0; m < A; ++m) col[m * S] = v[m];
+    }
+    __syncthreads();
+    for (int s = tid; s < a.nst; s += nt) {
+        const int q = a.stick_i2[s];
+        zp[s] = P[a.stick_col[s] * lc + q + q / B];
+    }
+}
+
+namespace {
+
+// first-stage radix A and second-stage radix B of the square plane sizes with a kernel
+struct TlShape {
+    int n, a, b;
+};
+constexpr TlShape kTlShapes[] = {{24, 6, 4},  {30, 6, 5},   {32, 8, 4},   {36, 6, 6},   {40, 8, 5},
+                                 {45, 9, 5},  {48, 8, 6},   {54, 9, 6},   {60, 10, 6},  {64, 8, 8},
+                                 {72, 9, 8},  {75, 15, 5},  {80, 10, 8},  {81, 9, 9},   {90, 10, 9},
+                                 {96, 12, 8}, {100, 10, 10}, {108, 12, 9}, {120, 12, 10}, {128, 16, 8},
+                                 {135, 15, 9}, {144, 12, 12}};
+
+int odd_at_least(int x) { return x | 1; }
+
+}  // namespace
+
+bool plane_tl_supported(int n1, int n2) {
+    if (n1 != n2) return false;
+    for (const TlShape& s : kTlShapes)
+        if (s.n == n1) return true;
+    return false;
+}
+
+PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
+    PlaneTlConfig cfg{};
+    int A = 0, B = 0;
+    for (const TlShape& s : kTlShapes)
+        if (s.n == n) A = s.a, B = s.b;
+    if (A == 0) return cfg;
+    const size_t cs = fp32 ? 8 : 16;
+    cfg.lc = odd_at_least(n + A);
+    cfg.lr = cfg.lc;
+    const size_t base = cs * n + 4 * static_cast<size_t>((n + 3) & ~3);
+    const size_t plane = cs * static_cast<size_t>(ncol) * cfg.lc;
+    // the plane goes to LDS when at least min(n, 8) rows fit beside it
+    const long min_rows = std::min(n, 8);
+    cfg.plane_global = base + plane + cs * min_rows * cfg.lr > max_lds;
+    const size_t fixed = base + (cfg.plane_global ? 0 : plane);
+    if (fixed >= max_lds) return cfg;
+    long rb = static_cast<long>((max_lds - fixed) / (cs * cfg.lr));
+    if (rb < 1) return cfg;
+    rb = std::min<long>(rb, n);
+    const long chunks = (n + rb - 1) / rb;
+    rb = (n + chunks - 1) / chunks;
+    cfg.rb = static_cast<int>(rb);
+    cfg.lds = fixed + cs * static_cast<size_t>(rb) * cfg.lr;
+    cfg.plane_bytes = plane;
+    const int tasks = std::max(std::max(ncol * B, ncol * A), std::max(cfg.rb * B, cfg.rb * A));
+    cfg.threads = std::min(768, std::max(64, (tasks + 63) / 64 * 64));
+    cfg.ok = true;
+    return cfg;
+}
+
+template <typename T, int A, int B, bool PG>
+static hipError_t launch_one(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
+                             void* pglob, hipStream_t st) {
+    static std::atomic<bool> raised{false};
+    if (!raised) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plane_tl<T, A, B, PG>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    hipLaunchKernelGGL((plane_tl<T, A, B, PG>), dim3(a.p3.n, nvec), dim3(c.threads), c.lds, st, a, vplane, zbuf,
+                       static_cast<const cx<T>*>(a.twp), c.lc, c.lr, c.rb, static_cast<cx<T>*>(pglob));
+    return hipGetLastError();
+}
+
+template <typename T, int A, int B>
+static hipError_t launch_shape(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
+                               void* pglob, hipStream_t st) {
+    if (c.plane_global) {
+        if (!pglob) return hipErrorInvalidValue;
+        return launch_one<T, A, B, true>(a, vplane, zbuf, c, nvec, pglob, st);
+    }
+    return launch_one<T, A, B, false>(a, vplane, zbuf, c, nvec, nullptr, st);
+}
+
+template <typename T>
+hipError_t launch_plane_tl(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
+                           void* pglob, hipStream_t st) {
+    switch (a.p1.n) {
+#define Q3X_TL(NN, AA, BB) \
+    case NN: return launch_shape<T, AA, BB>(a, vplane, zbuf, c, nvec, pglob, st);
+        Q3X_TL(24, 6, 4)
+        Q3X_TL(30, 6, 5)
+        Q3X_TL(32, 8, 4)
+        Q3X_TL(36, 6, 6)
+        Q3X_TL(40, 8, 5)
+        Q3X_TL(45, 9, 5)
+        Q3X_TL(48, 8, 6)
+        Q3X_TL(54, 9, 6)
+        Q3X_TL(60, 10, 6)
+        Q3X_TL(64, 8, 8)
+        Q3X_TL(72, 9, 8)
+        Q3X_TL(75, 15, 5)
+        Q3X_TL(80, 10, 8)
+        Q3X_TL(81, 9, 9)
+        Q3X_TL(90, 10, 9)
+        Q3X_TL(96, 12, 8)
+        Q3X_TL(100, 10, 10)
+        Q3X_TL(108, 12, 9)
+        Q3X_TL(120, 12, 10)
+        Q3X_TL(128, 16, 8)
+        Q3X_TL(135, 15, 9)
+        Q3X_TL(144, 12, 12)
+#undef Q3X_TL
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_plane_tl<double>(const HLocalArgs&, const double*, cx<double>*, const PlaneTlConfig&, int,
+                                            void*, hipStream_t);
+template hipError_t launch_plane_tl<float>(const HLocalArgs&, const float*, cx<float>*, const PlaneTlConfig&, int,
+                                           void*, hipStream_t);
+
+}  // namespace q3x

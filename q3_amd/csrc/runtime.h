@@ -62,7 +62,8 @@ struct Profiler {
 
 struct FftTables {
     FftPlan plan;
-    DevBuf tw64, tw32;
+    DevBuf tw64, tw32;        // stage-major Stockham twiddles
+    DevBuf plain64, plain32;  // exp(-2 pi i j/n), j < n
 };
 
 // Device buffers of the block Davidson (the reference's SerialDavidsonWorkspace,
@@ -98,6 +99,7 @@ struct q3x_hamiltonian_s {
     const q3x_kpoint_s* kp = nullptr;
     int npw = 0, nst = 0, ncol = 0, n_outside = 0;
     int sticks_per_block = 16, rows_per_pass64 = 8, rows_per_pass32 = 8;
+    q3x::PlaneTlConfig tl64, tl32;  // two-level plane kernel (ok = false: generic kernel)
     q3x::DevBuf stick_off, stick_i2, stick_col, col_i1, ent_pw, ent_pos, ekin, outside;
     // projectors of the prepared k-point
     int nproj = 0;
@@ -105,6 +107,7 @@ struct q3x_hamiltonian_s {
     q3x::DevBuf beta64, beta32, dblocks, pstart, psize, poff;
     // scratch
     q3x::DevBuf zbuf, P, W, gemm_work, io_psi, io_hpsi;
+    q3x::DevBuf plane_slots;  // per-workgroup planes of the two-level kernel when a plane exceeds LDS
 };
 
 namespace q3x {

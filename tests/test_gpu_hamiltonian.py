@@ -134,3 +134,46 @@ def test_cfg2_full_block_hermitian_and_linear(gpu_context):
     HY = h.apply_batch(Y, FP64)
     assert rel_err(HY, a * HX[:, :80] + b * HX[:, 80:]) <= 1e-12
     assert HX.shape == (npw, 160)
+
+
+# square plane sizes of the two-level plane kernel (plane_tl.hip), with the first-stage radix
+TL_SIZES = [24, 30, 32, 36, 40, 45, 48, 54, 60, 64, 72, 75, 80, 81, 90, 96, 100, 108, 120, 128, 135, 144]
+
+
+def _cubic_case(n):
+    """Simple-cubic one-atom cell whose wave grid is n^3: extent e = (n - 1) // 2 along each
+    axis, ecutwfc chosen so that the 4 x ecutwfc sphere reaches e + 1/2 (in 2 pi / a units)."""
+    from q3_amd.planewave import Crystal
+    alat = 10.0
+    e = (n - 1) // 2
+    ecut = ((e + 0.5) * 2.0 * np.pi / alat) ** 2 / 4.0
+    c = Crystal(np.eye(3), alat, np.array([[0.1, 0.2, 0.3]]), np.zeros(1, np.int32))
+    w = W.Workload(f"sc{n}", c, ecut, (1, 1, 1), np.array([[0.15, -0.1, 0.05]]), 2, 1, [W.synthetic_species(0)])
+    case = kcase(w, 0)
+    assert case.basis.wave_dims == (n, n, n)
+    return case
+
+
+@pytest.mark.parametrize("n", TL_SIZES)
+def test_two_level_plane_sizes(gpu_context, n):
+    """Every square plane size with a two-level kernel, FP64 (1e-10) and mixed (2e-6),
+    local part against the oracle."""
+    case = _cubic_case(n)
+    psi = W.random_psi(case.kp.kinetic_energy_squared, 2, seed=n)
+    h = _applicator(gpu_context, case, projectors=False)
+    ref = _oracle(case, psi, projectors=False)
+    assert rel_err(h.apply_batch(psi, FP64), ref) <= 1e-10
+    assert rel_err(h.apply_batch(psi, MIXED), ref) <= 2e-6
+
+
+def test_plane_kernels_agree_cfg2(gpu_context, monkeypatch):
+    """The generic Stockham plane kernel (Q3X_PLANE_KERNEL=generic) and the two-level one give
+    the same H|psi> at cfg2 to rounding, in FP64 and in mixed precision."""
+    case = kcase(W.cfg2(), 0)
+    psi = W.random_psi(case.kp.kinetic_energy_squared, 8, seed=5)
+    h_tl = _applicator(gpu_context, case)
+    monkeypatch.setenv("Q3X_PLANE_KERNEL", "generic")
+    h_gen = _applicator(gpu_context, case)
+    monkeypatch.delenv("Q3X_PLANE_KERNEL")
+    for prec, tol in ((FP64, 1e-13), (MIXED, 2e-6)):
+        assert rel_err(h_tl.apply_batch(psi, prec), h_gen.apply_batch(psi, prec)) <= tol
