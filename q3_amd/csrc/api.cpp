@@ -164,6 +164,7 @@ HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
     a.tw2 = fp32 ? h->fft[1].tw32.get() : h->fft[1].tw64.get();
     a.tw3 = fp32 ? h->fft[2].tw32.get() : h->fft[2].tw64.get();
     a.twp = fp32 ? h->fft[0].plain32.get() : h->fft[0].plain64.get();
+    a.twp3 = fp32 ? h->fft[2].plain32.get() : h->fft[2].plain64.get();
     a.nst = h->nst;
     a.ncol = h->ncol;
     a.sticks_per_block = h->sticks_per_block;
@@ -234,7 +235,7 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
         }
         if (!hip_ok(launch_hlocal(args, fp32, vplane, h->zbuf.get(), d_psi + 2L * b0 * ldpsi, ldpsi,
                                   d_hpsi + 2L * b0 * ldh, ldh, nb, st, pr.on ? marks : nullptr,
-                                  &tl, h->plane_slots.get()),
+                                  &tl, h->plane_slots.get(), fp32 ? &h->tz32 : &h->tz64),
                     "H_local kernels"))
             return Q3X_ERR_DEVICE;
         pr.span(K_ZINV, mi[0], mi[1]);
@@ -512,11 +513,14 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
     const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
     h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
     // two-level plane kernel where the plane is square and of a size it is built for;
-    // Q3X_PLANE_KERNEL=generic keeps the Stockham kernel (used to compare the two)
-    const char* pk = std::getenv("Q3X_PLANE_KERNEL");
+    // Q3X_FFT_KERNELS=generic keeps the Stockham kernels (used to compare the two)
+    const char* pk = std::getenv("Q3X_FFT_KERNELS");
     const bool use_tl = !(pk && std::strcmp(pk, "generic") == 0) && plane_tl_supported(n1, n2);
     h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds) : PlaneTlConfig{};
     h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds) : PlaneTlConfig{};
+    const bool use_tz = !(pk && std::strcmp(pk, "generic") == 0) && stick_tl_supported(n3);
+    h->tz64 = use_tz ? stick_tl_config(n3, false) : StickTlConfig{};
+    h->tz32 = use_tz ? stick_tl_config(n3, true) : StickTlConfig{};
     // generic plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of
     // 8 waves
     for (int fp32 = 0; fp32 < 2; ++fp32) {

@@ -12,6 +12,7 @@ struct HLocalArgs {
     const void* tw2;
     const void* tw3;
     const void* twp;              // exp(-2 pi i j/n1), j < n1, for the two-level plane kernel
+    const void* twp3;             // exp(-2 pi i j/n3), j < n3, for the two-level z-stick kernels
     int nst;                      // occupied z-sticks
     int ncol;                     // occupied i1 columns
     int sticks_per_block;
@@ -41,6 +42,20 @@ struct PlaneTlConfig {
 };
 bool plane_tl_supported(int n1, int n2);
 PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds);
+// Two-level z-stick kernels (plane_tl.hip) for the n3 the plane sizes list holds.
+struct StickTlConfig {
+    bool ok = false;
+    int spb = 0;  // sticks per workgroup
+    int lz = 0;   // LDS stride of a stick (complex elements)
+    int threads = 0;
+    size_t lds = 0;
+};
+bool stick_tl_supported(int n3);
+StickTlConfig stick_tl_config(int n3, bool fp32);
+// dir > 0: zsticks inverse (psi -> Z); dir < 0: forward + gather + kinetic (Z -> hpsi)
+template <typename T>
+hipError_t launch_sticks_tl(const HLocalArgs& a, int dir, const double* psi, long ldpsi, cx<T>* zbuf, double* hpsi,
+                            long ldh, const StickTlConfig& c, int nvec, hipStream_t st);
 // pglob: nvec * n3 plane slots of c.plane_bytes when c.plane_global, else unused
 template <typename T>
 hipError_t launch_plane_tl(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
@@ -52,6 +67,7 @@ hipError_t launch_plane_tl(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, co
 // tl: configuration of the two-level plane kernel, or null (or !ok) for the generic one.
 hipError_t launch_hlocal(const HLocalArgs& a, bool fp32, const void* vplane, void* zbuf, const double* psi,
                          long ldpsi, double* hpsi, long ldh, int nvec, hipStream_t st,
-                         hipEvent_t* marks = nullptr, const PlaneTlConfig* tl = nullptr, void* pglob = nullptr);
+                         hipEvent_t* marks = nullptr, const PlaneTlConfig* tl = nullptr, void* pglob = nullptr,
+                         const StickTlConfig* tz = nullptr);
 
 }  // namespace q3x

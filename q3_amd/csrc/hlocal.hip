@@ -208,7 +208,7 @@ size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which) {
 template <typename T>
 hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, const double* psi, long ldpsi,
                            double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks,
-                           const PlaneTlConfig* tl, void* pglob) {
+                           const PlaneTlConfig* tl, void* pglob, const StickTlConfig* tz) {
     if (marks) (void)hipEventRecord(marks[0], st);
     const bool fp32 = sizeof(T) == 4;
     const int nsb_blocks = (a.nst + a.sticks_per_block - 1) / a.sticks_per_block;
@@ -229,8 +229,13 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
         if (e != hipSuccess) return e;
         lds_raised = true;
     }
-    hipLaunchKernelGGL(zsticks_inverse<T>, g1, dim3(256), lds_z, st, a, psi, ldpsi, static_cast<cx<T>*>(zbuf));
-    e = hipGetLastError();
+    const bool ztl = tz && tz->ok;
+    if (ztl) {
+        e = launch_sticks_tl<T>(a, +1, psi, ldpsi, static_cast<cx<T>*>(zbuf), nullptr, 0, *tz, nvec, st);
+    } else {
+        hipLaunchKernelGGL(zsticks_inverse<T>, g1, dim3(256), lds_z, st, a, psi, ldpsi, static_cast<cx<T>*>(zbuf));
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) return e;
     if (marks) (void)hipEventRecord(marks[1], st);
     if (tl && tl->ok) {
@@ -242,9 +247,13 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
     }
     if (e != hipSuccess) return e;
     if (marks) (void)hipEventRecord(marks[2], st);
-    hipLaunchKernelGGL(zsticks_forward<T>, g1, dim3(256), lds_z, st, a, static_cast<const cx<T>*>(zbuf), psi, ldpsi,
-                       hpsi, ldh);
-    e = hipGetLastError();
+    if (ztl) {
+        e = launch_sticks_tl<T>(a, -1, psi, ldpsi, static_cast<cx<T>*>(zbuf), hpsi, ldh, *tz, nvec, st);
+    } else {
+        hipLaunchKernelGGL(zsticks_forward<T>, g1, dim3(256), lds_z, st, a, static_cast<const cx<T>*>(zbuf), psi,
+                           ldpsi, hpsi, ldh);
+        e = hipGetLastError();
+    }
     if (e != hipSuccess) return e;
     if (marks) (void)hipEventRecord(marks[3], st);
     if (a.n_outside > 0) {
@@ -257,12 +266,12 @@ hipError_t launch_hlocal_t(const HLocalArgs& a, const T* vplane, void* zbuf, con
 
 hipError_t launch_hlocal(const HLocalArgs& a, bool fp32, const void* vplane, void* zbuf, const double* psi,
                          long ldpsi, double* hpsi, long ldh, int nvec, hipStream_t st, hipEvent_t* marks,
-                         const PlaneTlConfig* tl, void* pglob) {
+                         const PlaneTlConfig* tl, void* pglob, const StickTlConfig* tz) {
     if (fp32)
         return launch_hlocal_t<float>(a, static_cast<const float*>(vplane), zbuf, psi, ldpsi, hpsi, ldh, nvec, st,
-                                      marks, tl, pglob);
+                                      marks, tl, pglob, tz);
     return launch_hlocal_t<double>(a, static_cast<const double*>(vplane), zbuf, psi, ldpsi, hpsi, ldh, nvec, st,
-                                   marks, tl, pglob);
+                                   marks, tl, pglob, tz);
 }
 
 }  // namespace q3x

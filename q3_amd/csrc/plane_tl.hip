@@ -1,5 +1,6 @@
-// Plane stage of the local Hamiltonian (the yz/xy part of serial.hpp:456-478) with two-level
-// register transforms, for boxes whose plane is square (n1 = n2 = N = A * B).
+// Two-level register transforms for the local Hamiltonian (serial.hpp:456-478): the plane
+// stage for boxes whose plane is square (n1 = n2 = N = A * B), and the z-stick stages for
+// n3 = A * B, for the box sizes listed in kTlShapes.
 //
 // Per (band, i3) plane, as plane_sandwich in hlocal.hip: scatter the plane's z-stick values
 // into the occupied i1 columns, backward FFT along i2, per i2 row backward FFT along i1,
@@ -189,6 +190,120 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
     }
 }
 
+// z-stick transforms with the same two-level split, N = n3 = A * B.  The sticks of a block
+// sit in LDS at sl * lz + slot(q), slot(q) = q + q / B, lanes running across sticks fastest.
+//
+// Inverse: scatter the sphere entries of psi, backward stage 1 in LDS, then backward stage 2
+// in registers, written straight to Z[band][i3][stick] (element i3 = k1 + A k2 of lane k1).
+template <typename T, int A, int B>
+__global__ void __launch_bounds__(256) zsticks_inverse_tl(HLocalArgs a, const double* __restrict__ psi, long ldpsi,
+                                                          cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg,
+                                                          int spb, int lz) {
+    constexpr int N = A * B;
+    constexpr int S = B + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* buf = tw + N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int s0 = blockIdx.x * spb;
+    const int nsb = min(spb, a.nst - s0);
+    const int band = blockIdx.y;
+    for (int i = tid; i < N; i += nt) tw[i] = twg[i];
+    for (int i = tid; i < nsb * lz; i += nt) buf[i] = cx<T>{T(0), T(0)};
+    __syncthreads();
+    const double* src = psi + band * ldpsi * 2;
+    const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
+    const int base = s0 * N;
+    for (int j = j0 + tid; j < j1; j += nt) {
+        const int p = a.ent_pw[j];
+        const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
+        buf[sl * lz + z + z / B] = cx<T>{T(src[2 * p]), T(src[2 * p + 1])};
+    }
+    __syncthreads();
+    for (int t = tid; t < nsb * B; t += nt) {
+        const int n2 = t / nsb, sl = t - n2 * nsb;
+        cx<T>* x = buf + sl * lz + n2;
+        cx<T> v[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) v[j] = x[j * S];
+        Dft<T, A>::run(v, T(1));
+#pragma unroll
+        for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+#pragma unroll
+        for (int k = 0; k < A; ++k) x[k * S] = v[k];
+    }
+    __syncthreads();
+    cx<T>* out = zbuf + static_cast<long>(band) * N * a.nst + s0;
+    for (int t = tid; t < nsb * A; t += nt) {
+        const int k1 = t / nsb, sl = t - k1 * nsb;
+        const cx<T>* x = buf + sl * lz + k1 * S;
+        cx<T> u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = x[j];
+        Dft<T, B>::run(u, T(1));
+#pragma unroll
+        for (int j = 0; j < B; ++j) out[static_cast<long>(k1 + A * j) * a.nst + sl] = u[j];
+    }
+}
+
+// Forward: stage A reads Z[band][n1 + A n2][stick] straight into registers (lane n1), stage B
+// in LDS puts the transform in natural order, then the sphere gather fused with the kinetic
+// term, hpsi[g] = T[g] psi[g] + box[g] (serial.hpp:451-478).
+template <typename T, int A, int B>
+__global__ void __launch_bounds__(256) zsticks_forward_tl(HLocalArgs a, const cx<T>* __restrict__ zbuf,
+                                                          const double* __restrict__ psi, long ldpsi,
+                                                          double* __restrict__ hpsi, long ldh,
+                                                          const cx<T>* __restrict__ twg, int spb, int lz) {
+    constexpr int N = A * B;
+    constexpr int S = B + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* buf = tw + N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int s0 = blockIdx.x * spb;
+    const int nsb = min(spb, a.nst - s0);
+    const int band = blockIdx.y;
+    for (int i = tid; i < N; i += nt) tw[i] = twg[i];
+    __syncthreads();
+    const cx<T>* in = zbuf + static_cast<long>(band) * N * a.nst + s0;
+    for (int t = tid; t < nsb * A; t += nt) {
+        const int n1 = t / nsb, sl = t - n1 * nsb;
+        cx<T> u[B];
+#pragma unroll
+        for (int j = 0; j < B; ++j) u[j] = in[static_cast<long>(n1 + A * j) * a.nst + sl];
+        Dft<T, B>::run(u, T(-1));
+#pragma unroll
+        for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+        cx<T>* x = buf + sl * lz + n1 * S;
+#pragma unroll
+        for (int m = 0; m < B; ++m) x[m] = u[m];
+    }
+    __syncthreads();
+    for (int t = tid; t < nsb * B; t += nt) {
+        const int m2 = t / nsb, sl = t - m2 * nsb;
+        cx<T>* x = buf + sl * lz + m2;
+        cx<T> v[A];
+#pragma unroll
+        for (int j = 0; j < A; ++j) v[j] = x[j * S];
+        Dft<T, A>::run(v, T(-1));
+#pragma unroll
+        for (int m = 0; m < A; ++m) x[m * S] = v[m];
+    }
+    __syncthreads();
+    const double* src = psi + band * ldpsi * 2;
+    double* dst = hpsi + band * ldh * 2;
+    const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
+    const int base = s0 * N;
+    for (int j = j0 + tid; j < j1; j += nt) {
+        const int p = a.ent_pw[j];
+        const double t = a.ekin[p];
+        const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
+        const cx<T> v = buf[sl * lz + z + z / B];
+        dst[2 * p] = t * src[2 * p] + static_cast<double>(v.x);
+        dst[2 * p + 1] = t * src[2 * p + 1] + static_cast<double>(v.y);
+    }
+}
+
 namespace {
 
 // first-stage radix A and second-stage radix B of the square plane sizes with a kernel
@@ -241,6 +356,89 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
     cfg.ok = true;
     return cfg;
 }
+
+bool stick_tl_supported(int n3) {
+    for (const TlShape& s : kTlShapes)
+        if (s.n == n3) return true;
+    return false;
+}
+
+StickTlConfig stick_tl_config(int n3, bool fp32) {
+    StickTlConfig cfg{};
+    int A = 0;
+    for (const TlShape& s : kTlShapes)
+        if (s.n == n3) A = s.a;
+    if (A == 0) return cfg;
+    const size_t cs = fp32 ? 8 : 16;
+    cfg.lz = odd_at_least(n3 + A);
+    cfg.spb = 32;
+    cfg.threads = 256;
+    cfg.lds = cs * (n3 + static_cast<size_t>(cfg.spb) * cfg.lz);
+    cfg.ok = true;
+    return cfg;
+}
+
+template <typename T, int A, int B>
+static hipError_t launch_sticks_shape(const HLocalArgs& a, int dir, const double* psi, long ldpsi, cx<T>* zbuf,
+                                      double* hpsi, long ldh, const StickTlConfig& c, int nvec, hipStream_t st) {
+    static std::atomic<bool> raised{false};
+    if (!raised) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zsticks_inverse_tl<T, A, B>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&zsticks_forward_tl<T, A, B>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    const dim3 grid((a.nst + c.spb - 1) / c.spb, nvec);
+    const cx<T>* tw = static_cast<const cx<T>*>(a.twp3);
+    if (dir > 0)
+        hipLaunchKernelGGL((zsticks_inverse_tl<T, A, B>), grid, dim3(c.threads), c.lds, st, a, psi, ldpsi, zbuf, tw,
+                           c.spb, c.lz);
+    else
+        hipLaunchKernelGGL((zsticks_forward_tl<T, A, B>), grid, dim3(c.threads), c.lds, st, a, zbuf, psi, ldpsi,
+                           hpsi, ldh, tw, c.spb, c.lz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sticks_tl(const HLocalArgs& a, int dir, const double* psi, long ldpsi, cx<T>* zbuf, double* hpsi,
+                            long ldh, const StickTlConfig& c, int nvec, hipStream_t st) {
+    switch (a.p3.n) {
+#define Q3X_TL(NN, AA, BB) \
+    case NN: return launch_sticks_shape<T, AA, BB>(a, dir, psi, ldpsi, zbuf, hpsi, ldh, c, nvec, st);
+        Q3X_TL(24, 6, 4)
+        Q3X_TL(30, 6, 5)
+        Q3X_TL(32, 8, 4)
+        Q3X_TL(36, 6, 6)
+        Q3X_TL(40, 8, 5)
+        Q3X_TL(45, 9, 5)
+        Q3X_TL(48, 8, 6)
+        Q3X_TL(54, 9, 6)
+        Q3X_TL(60, 10, 6)
+        Q3X_TL(64, 8, 8)
+        Q3X_TL(72, 9, 8)
+        Q3X_TL(75, 15, 5)
+        Q3X_TL(80, 10, 8)
+        Q3X_TL(81, 9, 9)
+        Q3X_TL(90, 10, 9)
+        Q3X_TL(96, 12, 8)
+        Q3X_TL(100, 10, 10)
+        Q3X_TL(108, 12, 9)
+        Q3X_TL(120, 12, 10)
+        Q3X_TL(128, 16, 8)
+        Q3X_TL(135, 15, 9)
+        Q3X_TL(144, 12, 12)
+#undef Q3X_TL
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t launch_sticks_tl<double>(const HLocalArgs&, int, const double*, long, cx<double>*, double*, long,
+                                             const StickTlConfig&, int, hipStream_t);
+template hipError_t launch_sticks_tl<float>(const HLocalArgs&, int, const double*, long, cx<float>*, double*, long,
+                                            const StickTlConfig&, int, hipStream_t);
 
 template <typename T, int A, int B, bool PG>
 static hipError_t launch_one(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,

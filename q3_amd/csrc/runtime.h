@@ -100,6 +100,7 @@ struct q3x_hamiltonian_s {
     int npw = 0, nst = 0, ncol = 0, n_outside = 0;
     int sticks_per_block = 16, rows_per_pass64 = 8, rows_per_pass32 = 8;
     q3x::PlaneTlConfig tl64, tl32;  // two-level plane kernel (ok = false: generic kernel)
+    q3x::StickTlConfig tz64, tz32;  // two-level z-stick kernels (ok = false: generic kernels)
     q3x::DevBuf stick_off, stick_i2, stick_col, col_i1, ent_pw, ent_pos, ekin, outside;
     // projectors of the prepared k-point
     int nproj = 0;

@@ -8,14 +8,19 @@
 // 2x2 block [[Br, Bi], [-Bi, Br]], A' B' = [Re C, Im C] interleaved -- 8mnk real flops, the
 // same count as a complex GEMM.  Conjugation of op(A) = A^H is folded into the LDS load.
 //
-// Workgroup tile: 64 rows x 32 complex columns (64 real), 4 wave64s in a 2x2 layout, each
-// wave a 32x32 real block as 2x2 MFMA 16x16 tiles; K staged through LDS in chunks of 16
-// complex.  Tall-skinny products (k >> m, n: projector overlaps, subspace Gram blocks) split
+// Workgroup tile: up to 128 rows x 80 complex columns, 4 wave64s in a 2x2 layout, each wave
+// up to 4 x 5 MFMA 16x16 tiles; K staged through double-buffered LDS in chunks of 8 complex.
+// Tall-skinny products (k >> m, n: projector overlaps, subspace Gram blocks) split
 // K over the grid's z dimension into a workspace and are summed in a fixed order by a
 // second kernel, so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include "zgemm.h"
+
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
 
 namespace q3x {
 
@@ -42,17 +47,29 @@ struct Mma<float> {
     }
 };
 
-constexpr int BM = 64;   // rows per workgroup
-constexpr int BNC = 32;  // complex columns per workgroup
-constexpr int KC = 16;   // complex k per LDS stage
+constexpr int KC_DEFAULT = 8;  // complex k per LDS stage (16 real k, four MFMA k-steps)
 
-template <typename TA, typename TB, typename TM, typename TC, int OPA>
-__global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, int kchunk, const TA* __restrict__ A,
+// Workgroup: 2 x 2 wave64s; each wave owns TMW x TNW MFMA tiles of 16 x 16 real, so the
+// workgroup tile is BM = 32 TMW rows by BN = 32 TNW real columns (16 TNW complex columns).
+// K runs in stages of KC complex: the next stage's global loads are issued into registers
+// before the current stage's MFMAs, and stored into the other LDS buffer after them, so one
+// barrier per stage orders everything.  LDS rows are padded so that the k-contiguous
+// operand stores (threads walking k) and the fragment reads (lanes walking rows / columns)
+// both spread over the banks.
+template <typename TA, typename TB, typename TM, typename TC, int OPA, int TMW, int TNW, int KC, int MINW>
+__global__ void __launch_bounds__(256, MINW) zgemm_kernel(int m, int n, int k, int kchunk, const TA* __restrict__ A,
                                                     long lda, const TB* __restrict__ B, long ldb, TC* __restrict__ C,
                                                     long ldc, double alpha, double beta, TM* __restrict__ work) {
     using acc_t = typename Mma<TM>::acc_t;
-    __shared__ TM As[2 * KC][BM];
-    __shared__ TM Bs[KC][BNC][2];
+    constexpr int BM = 32 * TMW;                 // rows of C
+    constexpr int BNC = 16 * TNW;                // complex columns of C
+    constexpr int APAD = sizeof(TM) == 8 ? 2 : 4;
+    constexpr int LA = BM + APAD;                // As row stride (TM elements)
+    constexpr int LB = 2 * BNC + 1;              // Bs row stride (real elements of B')
+    constexpr int NLA = (BM * KC + 255) / 256;   // A loads per thread per stage
+    constexpr int NLB = (BNC * KC + 255) / 256;  // B loads per thread per stage
+    __shared__ TM As[2][2 * KC * LA];
+    __shared__ TM Bs[2][2 * KC * LB];            // B' = [[Br, Bi], [-Bi, Br]] blocks, row-major
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -61,89 +78,113 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, int kch
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(k, kbeg + kchunk);
 
-    acc_t acc[2][2];
+    acc_t acc[TMW][TNW];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TMW; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+        for (int b = 0; b < TNW; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
-    for (int k0 = kbeg; k0 < kend; k0 += KC) {
-        // ---- stage op(A) tile: As[2p + part][i]
-        if (OPA == 0) {
-            for (int e = tid; e < BM * KC; e += 256) {
-                const int i = e % BM, p = e / BM;
-                const int gi = m0 + i, gp = k0 + p;
-                TM re = 0, im = 0;
-                if (gi < m && gp < kend) {
-                    const long o = 2 * (gi + gp * lda);
-                    re = static_cast<TM>(A[o]);
-                    im = static_cast<TM>(A[o + 1]);
-                }
-                As[2 * p][i] = re;
-                As[2 * p + 1][i] = im;
+    TM ra[NLA][2], rb[NLB][2];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < NLA; ++q) {
+            const int e = tid + 256 * q;
+            int i, p;
+            if (OPA == 0) {
+                i = e % BM;
+                p = e / BM;
+            } else {
+                p = e % KC;
+                i = e / KC;
             }
-        } else {
-            for (int e = tid; e < BM * KC; e += 256) {
-                const int p = e % KC, i = e / KC;
-                const int gi = m0 + i, gp = k0 + p;
-                TM re = 0, im = 0;
-                if (gi < m && gp < kend) {
-                    const long o = 2 * (gp + gi * lda);
-                    re = static_cast<TM>(A[o]);
-                    im = -static_cast<TM>(A[o + 1]);
-                }
-                As[2 * p][i] = re;
-                As[2 * p + 1][i] = im;
+            const int gi = m0 + i, gp = k0 + p;
+            ra[q][0] = TM(0);
+            ra[q][1] = TM(0);
+            if (e < BM * KC && gi < m && gp < kend) {
+                const long o = OPA == 0 ? 2 * (gi + gp * lda) : 2 * (gp + gi * lda);
+                ra[q][0] = static_cast<TM>(A[o]);
+                ra[q][1] = OPA == 0 ? static_cast<TM>(A[o + 1]) : -static_cast<TM>(A[o + 1]);
             }
         }
-        // ---- stage B tile: Bs[p][j] = (Re, Im)
-        for (int e = tid; e < BNC * KC; e += 256) {
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+            const int e = tid + 256 * q;
             const int p = e % KC, j = e / KC;
             const int gj = n0 + j, gp = k0 + p;
-            TM re = 0, im = 0;
-            if (gj < n && gp < kend) {
+            rb[q][0] = TM(0);
+            rb[q][1] = TM(0);
+            if (e < BNC * KC && gj < n && gp < kend) {
                 const long o = 2 * (gp + gj * ldb);
-                re = static_cast<TM>(B[o]);
-                im = static_cast<TM>(B[o + 1]);
+                rb[q][0] = static_cast<TM>(B[o]);
+                rb[q][1] = static_cast<TM>(B[o + 1]);
             }
-            Bs[p][j][0] = re;
-            Bs[p][j][1] = im;
         }
-        __syncthreads();
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < NLA; ++q) {
+            const int e = tid + 256 * q;
+            if (e < BM * KC) {
+                const int i = OPA == 0 ? e % BM : e / KC;
+                const int p = OPA == 0 ? e / BM : e % KC;
+                As[buf][(2 * p) * LA + i] = ra[q][0];
+                As[buf][(2 * p + 1) * LA + i] = ra[q][1];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NLB; ++q) {
+            const int e = tid + 256 * q;
+            if (e < BNC * KC) {
+                const int p = e % KC, j = e / KC;
+                TM* b0 = &Bs[buf][(2 * p) * LB + 2 * j];
+                b0[0] = rb[q][0];
+                b0[1] = rb[q][1];
+                b0[LB] = -rb[q][1];
+                b0[LB + 1] = rb[q][0];
+            }
+        }
+    };
+
+    int cur = 0;
+    if (kbeg < kend) {
+        load(kbeg);
+        store(0);
+    }
+    __syncthreads();
+    for (int k0 = kbeg; k0 < kend; k0 += KC) {
+        const bool more = k0 + KC < kend;
+        if (more) load(k0 + KC);
+        const TM* as = As[cur];
+        const TM* bs = Bs[cur];
 #pragma unroll
         for (int kk = 0; kk < (2 * KC) / 4; ++kk) {
             const int kr = 4 * kk + lk;
-            const int p = kr >> 1, kpar = kr & 1;
-            TM af[2], bf[2];
+            TM af[TMW], bf[TNW];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) af[t] = As[kr][wm * 32 + t * 16 + li];
+            for (int t = 0; t < TMW; ++t) af[t] = as[kr * LA + wm * (16 * TMW) + t * 16 + li];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int jr = wn * 32 + t * 16 + li;
-                const int j = jr >> 1, cpar = jr & 1;
-                const TM br = Bs[p][j][0], bi = Bs[p][j][1];
-                // B' = [[Br, Bi], [-Bi, Br]] : row parity kpar, column parity cpar
-                bf[t] = kpar == 0 ? (cpar == 0 ? br : bi) : (cpar == 0 ? -bi : br);
-            }
+            for (int t = 0; t < TNW; ++t) bf[t] = bs[kr * LB + wn * (16 * TNW) + t * 16 + li];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < TMW; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = Mma<TM>::run(af[a], bf[b], acc[a][b]);
+                for (int b = 0; b < TNW; ++b) acc[a][b] = Mma<TM>::run(af[a], bf[b], acc[a][b]);
         }
+        if (more) store(cur ^ 1);
         __syncthreads();
+        cur ^= 1;
     }
 
     // ---- epilogue: lane holds D[row(lk, r)][li] of each 16x16 tile
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < TMW; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int jr = wn * 32 + b * 16 + li;
+        for (int b = 0; b < TNW; ++b) {
+            const int jr = wn * (16 * TNW) + b * 16 + li;
             const int gj = n0 + (jr >> 1), part = jr & 1;
             if (gj >= n) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gi = m0 + wm * 32 + a * 16 + Mma<TM>::row(lk, r);
+                const int gi = m0 + wm * (16 * TMW) + a * 16 + Mma<TM>::row(lk, r);
                 if (gi >= m) continue;
                 const TM v = acc[a][b][r];
                 if (work) {
@@ -174,17 +215,35 @@ __global__ void splitk_reduce(int m, int n, int ksplit, const TM* __restrict__ w
     C[o] = static_cast<TC>(out);
 }
 
-template <typename TA, typename TB, typename TM, typename TC>
-hipError_t zgemm_t(int opa, int m, int n, int k, double alpha, const void* A, long lda, const void* B, long ldb,
-                   double beta, void* C, long ldc, void* work, size_t work_bytes, hipStream_t st) {
-    if (m <= 0 || n <= 0) return hipSuccess;
+template <typename TA, typename TB, typename TM, typename TC, int TMW, int TNW, int KC = KC_DEFAULT, int MINW = 1>
+hipError_t zgemm_shape(int opa, int m, int n, int k, double alpha, const void* A, long lda, const void* B, long ldb,
+                       double beta, void* C, long ldc, void* work, size_t work_bytes, hipStream_t st,
+                       int force_split = 0) {
+    constexpr int BM = 32 * TMW, BNC = 16 * TNW;
     const int tm = (m + BM - 1) / BM, tn = (n + BNC - 1) / BNC;
     const long tiles = static_cast<long>(tm) * tn;
     int ksplit = 1;
-    if (k > 0 && tiles < 512 && work) {
+    const long by_ws = work ? static_cast<long>(work_bytes / (2ull * m * n * sizeof(TM))) : 1;
+    if (force_split < 0 && work && k > 0) {
+        // fill one round of the GPU: workgroups resident at once = CUs x blocks per CU of this kernel
+        static std::atomic<int> slots_cached{0};  // per kernel instantiation; same value from any thread
+        int slots = slots_cached.load();
+        if (slots == 0) {
+            int dev = 0, cus = 0, per_cu = 0;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(&zgemm_kernel<TA, TB, TM, TC, 1, TMW, TNW, KC, MINW>), 256, 0);
+            slots = std::max(1, cus * std::max(1, per_cu));
+            slots_cached.store(slots);
+        }
+        const long by_k = std::max(1, k / 128);  // at least 128 complex k per split
+        ksplit = static_cast<int>(std::max(1L, std::min(std::min(slots / tiles, by_k), by_ws)));
+    } else if (force_split > 0 && work) {
+        ksplit = static_cast<int>(std::max(1L, std::min<long>(force_split, by_ws)));
+    } else if (k > 0 && tiles < 512 && work) {
         long want = (1024 + tiles - 1) / tiles;
         long by_k = (k + 511) / 512;  // at least 512 complex k per split
-        long by_ws = static_cast<long>(work_bytes / (2ull * m * n * sizeof(TM)));
         ksplit = static_cast<int>(std::max(1L, std::min(want, std::min(by_k, by_ws))));
     }
     int kchunk = (k + ksplit - 1) / ksplit;
@@ -194,11 +253,11 @@ hipError_t zgemm_t(int opa, int m, int n, int k, double alpha, const void* A, lo
     dim3 grid(tm, tn, ksplit);
     TM* w = ksplit > 1 ? static_cast<TM*>(work) : nullptr;
     if (opa == 0)
-        hipLaunchKernelGGL((zgemm_kernel<TA, TB, TM, TC, 0>), grid, dim3(256), 0, st, m, n, k, kchunk,
+        hipLaunchKernelGGL((zgemm_kernel<TA, TB, TM, TC, 0, TMW, TNW, KC, MINW>), grid, dim3(256), 0, st, m, n, k, kchunk,
                            static_cast<const TA*>(A), lda, static_cast<const TB*>(B), ldb, static_cast<TC*>(C), ldc,
                            alpha, beta, w);
     else
-        hipLaunchKernelGGL((zgemm_kernel<TA, TB, TM, TC, 1>), grid, dim3(256), 0, st, m, n, k, kchunk,
+        hipLaunchKernelGGL((zgemm_kernel<TA, TB, TM, TC, 1, TMW, TNW, KC, MINW>), grid, dim3(256), 0, st, m, n, k, kchunk,
                            static_cast<const TA*>(A), lda, static_cast<const TB*>(B), ldb, static_cast<TC*>(C), ldc,
                            alpha, beta, w);
     hipError_t e = hipGetLastError();
@@ -207,6 +266,64 @@ hipError_t zgemm_t(int opa, int m, int n, int k, double alpha, const void* A, lo
     hipLaunchKernelGGL((splitk_reduce<TM, TC>), dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, m,
                        n, ksplit, static_cast<const TM*>(work), static_cast<TC*>(C), ldc, alpha, beta);
     return hipGetLastError();
+}
+
+// Tile configurations (wave tiles TMW x TNW, K stage KC, minimum waves per SIMD) and how a
+// product picks one.  The table comes from timing every candidate on the H|psi> and Davidson
+// shapes of the 64-atom cell (q3_amd/tune_gemm.py).  Deep-k A^H B products split K so that
+// the grid is one round of resident workgroups (CUs x occupancy of the kernel): a partial
+// second round costs as much as a full one.  FP64 A^H B: 2x5 tiles when n fills 80-column
+// tiles, else 2x2; FP64 A B (tall m): 4x2 tiles at two waves per SIMD; FP32: 2x2 tiles.
+// Q3X_GEMM_TILE="TMWxTNW/ksplit" overrides the choice (ksplit -1: one round, 0: default).
+enum class Tile { T2x2, T4x2, T2x5, T4x4, T4x5 };
+
+template <typename TA, typename TB, typename TM, typename TC>
+hipError_t zgemm_tile(Tile t, int split, int opa, int m, int n, int k, double alpha, const void* A, long lda,
+                      const void* B, long ldb, double beta, void* C, long ldc, void* work, size_t work_bytes,
+                      hipStream_t st) {
+    switch (t) {
+        case Tile::T2x2:
+            return zgemm_shape<TA, TB, TM, TC, 2, 2, 8, 1>(opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                                           work_bytes, st, split);
+        case Tile::T4x2:
+            return zgemm_shape<TA, TB, TM, TC, 4, 2, 8, 2>(opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                                           work_bytes, st, split);
+        case Tile::T2x5:
+            return zgemm_shape<TA, TB, TM, TC, 2, 5, 8, 2>(opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                                           work_bytes, st, split);
+        case Tile::T4x4:
+            return zgemm_shape<TA, TB, TM, TC, 4, 4, 8, 2>(opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                                           work_bytes, st, split);
+        case Tile::T4x5:
+            return zgemm_shape<TA, TB, TM, TC, 4, 5, 8, 2>(opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                                           work_bytes, st, split);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename TA, typename TB, typename TM, typename TC>
+hipError_t zgemm_t(int opa, int m, int n, int k, double alpha, const void* A, long lda, const void* B, long ldb,
+                   double beta, void* C, long ldc, void* work, size_t work_bytes, hipStream_t st) {
+    if (m <= 0 || n <= 0) return hipSuccess;
+    if (const char* cfg = std::getenv("Q3X_GEMM_TILE")) {
+        int tmw = 0, tnw = 0, sp = 0;
+        if (std::sscanf(cfg, "%dx%d/%d", &tmw, &tnw, &sp) == 3) {  // sp: -1 one round, 0 default, > 0 fixed
+            const Tile t = tmw == 4 ? (tnw == 2 ? Tile::T4x2 : tnw == 4 ? Tile::T4x4 : Tile::T4x5)
+                                    : (tnw == 5 ? Tile::T2x5 : Tile::T2x2);
+            return zgemm_tile<TA, TB, TM, TC>(t, sp, opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                              work_bytes, st);
+        }
+    }
+    const bool f64 = std::is_same<TM, double>::value;
+    Tile t = Tile::T2x2;
+    if (m > 64 && n > 32) {
+        if (opa == 1)
+            t = f64 && n % 80 == 0 ? Tile::T2x5 : Tile::T2x2;
+        else
+            t = f64 ? Tile::T4x2 : Tile::T2x2;
+    }
+    return zgemm_tile<TA, TB, TM, TC>(t, opa == 1 ? -1 : 0, opa, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, work,
+                                      work_bytes, st);
 }
 
 }  // namespace

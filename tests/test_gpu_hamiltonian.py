@@ -136,7 +136,7 @@ def test_cfg2_full_block_hermitian_and_linear(gpu_context):
     assert HX.shape == (npw, 160)
 
 
-# square plane sizes of the two-level plane kernel (plane_tl.hip), with the first-stage radix
+# box sizes of the two-level plane and z-stick kernels (plane_tl.hip)
 TL_SIZES = [24, 30, 32, 36, 40, 45, 48, 54, 60, 64, 72, 75, 80, 81, 90, 96, 100, 108, 120, 128, 135, 144]
 
 
@@ -156,7 +156,7 @@ def _cubic_case(n):
 
 @pytest.mark.parametrize("n", TL_SIZES)
 def test_two_level_plane_sizes(gpu_context, n):
-    """Every square plane size with a two-level kernel, FP64 (1e-10) and mixed (2e-6),
+    """Every box size with two-level plane and z-stick kernels, FP64 (1e-10) and mixed (2e-6),
     local part against the oracle."""
     case = _cubic_case(n)
     psi = W.random_psi(case.kp.kinetic_energy_squared, 2, seed=n)
@@ -166,14 +166,14 @@ def test_two_level_plane_sizes(gpu_context, n):
     assert rel_err(h.apply_batch(psi, MIXED), ref) <= 2e-6
 
 
-def test_plane_kernels_agree_cfg2(gpu_context, monkeypatch):
-    """The generic Stockham plane kernel (Q3X_PLANE_KERNEL=generic) and the two-level one give
-    the same H|psi> at cfg2 to rounding, in FP64 and in mixed precision."""
+def test_fft_kernels_agree_cfg2(gpu_context, monkeypatch):
+    """The generic Stockham kernels (Q3X_FFT_KERNELS=generic) and the two-level plane and
+    z-stick kernels give the same H|psi> at cfg2 to rounding, in FP64 and in mixed precision."""
     case = kcase(W.cfg2(), 0)
     psi = W.random_psi(case.kp.kinetic_energy_squared, 8, seed=5)
     h_tl = _applicator(gpu_context, case)
-    monkeypatch.setenv("Q3X_PLANE_KERNEL", "generic")
+    monkeypatch.setenv("Q3X_FFT_KERNELS", "generic")
     h_gen = _applicator(gpu_context, case)
-    monkeypatch.delenv("Q3X_PLANE_KERNEL")
+    monkeypatch.delenv("Q3X_FFT_KERNELS")
     for prec, tol in ((FP64, 1e-13), (MIXED, 2e-6)):
         assert rel_err(h_tl.apply_batch(psi, prec), h_gen.apply_batch(psi, prec)) <= tol
