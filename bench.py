@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-workers", type=int, default=8)
+    p.add_argument("--gamma-solves", type=int, default=8)
+    p.add_argument("--gamma-workers", type=int, default=4)
     p.add_argument("--no-serve", action="store_true")
     p.add_argument("--serve-requests", type=int, default=24)
     p.add_argument("--serve-max-atoms", type=int, default=16)
@@ -51,16 +53,17 @@ def parse():
     return p.parse_args()
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of the FP64 kernel at this workload, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (each counter scaled by its copy-kernel calibration), or None."""
+def measured_traffic(stage):
+    """HBM bytes per launch of the FP64 kernel of a stage (zsticks_inverse, plane_sandwich, ...)
+    at this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter
+    scaled by its copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get("bytes_per_launch", {}).get(f"q3x::{kernel}<double>")
+    return t.get("bytes_per_launch_by_stage", {}).get("fp64", {}).get(stage)
 
 
 def copy_reference(steps):
@@ -113,6 +116,8 @@ def batched_kpoints(args):
         kps = [basis.compute_gk(k) for k in w.kpoints]
         betas = [kp.kb_projectors(w.crystal, w.species) for kp in kps]
         npws = [kp.npw for kp in kps]
+        # damped random starting blocks, one seed per k-point
+        psi0s = [W.random_psi(kp.kinetic_energy_squared, w.nbands, seed=100 + i) for i, kp in enumerate(kps)]
         log(f"batched setup {time.time() - t0:.1f}s: {len(kps)} k-points, npw {min(npws)}..{max(npws)}")
         out = {"workload": "cfg4: Si8 cubic a=10.26 bohr, 20 Ha, 48^3, 48 bands (16 occupied), 64 random k (seed 3)",
                "kpoints": len(kps), "npw_min": min(npws), "npw_max": max(npws), "tolerance_ry": 2e-6}
@@ -122,9 +127,9 @@ def batched_kpoints(args):
             batch = KPointBatch(workers=workers)
             for kp, beta in zip(kps, betas):
                 batch.add(basis, kp, V, beta, bs, db)
-            batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec)  # warm the workspaces
+            batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=psi0s)  # warm-up
             t = time.perf_counter()
-            e, _ = batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec)
+            e, _ = batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=psi0s)
             dt = time.perf_counter() - t
             eigs[label] = e
             its = [r.iterations for r in batch.reports]
@@ -139,6 +144,51 @@ def batched_kpoints(args):
         return out
     except Exception as e:  # reported, not hidden
         log(f"batched failed: {e}")
+        return {"error": str(e)}
+
+
+def batched_gamma(args):
+    """Throughput of independent solves of the 64-atom Gamma problem (cfg2, 1e-6 Ha): N copies
+    over W worker streams in FP64 and in mixed precision, against the same N solved one after
+    another in FP64 on one worker (the plain FP64 path).  A single solve leaves most CUs idle
+    in its FP64 subspace eigensolve, which concurrent solves fill."""
+    from q3_amd import FP64, MIXED, KPointBatch
+    from q3_amd import workloads as W
+    from q3_amd.planewave import PlanewaveBasis
+    try:
+        w = W.cfg2()
+        basis = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, True, w.fft_factors)
+        kp = basis.compute_gk(w.kpoints[0])
+        V = W.local_potential(w.crystal, basis, w.species)
+        beta = kp.kb_projectors(w.crystal, w.species)
+        bs, db = W.d_blocks_for(w.crystal, w.species)
+        n = args.gamma_solves
+        psi0 = W.random_psi(kp.kinetic_energy_squared, w.nbands, seed=2)  # the cfg2 starting block
+        out = {"workload": f"{n} independent cfg2 solves (Si64 Gamma, 25 Ha, 96^3, 160 bands, random damped "
+                           f"psi0 seed 2) at 2e-6 Ry", "solves": n}
+        eigs = {}
+        for label, prec, workers in (("fp64_one_worker", FP64, 1), ("fp64", FP64, args.gamma_workers),
+                                     ("mixed", MIXED, args.gamma_workers)):
+            batch = KPointBatch(workers=workers)
+            for _ in range(n):
+                batch.add(basis, kp, V, beta, bs, db)
+            batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=[psi0] * n)  # warm-up
+            t = time.perf_counter()
+            e, _ = batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=[psi0] * n)
+            dt = time.perf_counter() - t
+            eigs[label] = e
+            out[label] = {"workers": workers, "s_all": dt, "solves_per_s": n / dt,
+                          "iterations_mean": float(np.mean([r.iterations for r in batch.reports]))}
+            log(f"batched cfg2 {label}: {dt:.3f}s for {n} solves ({workers} workers)")
+            batch.close()
+        out["mixed"]["max_abs_err_ha_occupied_vs_fp64"] = float(
+            np.abs(eigs["mixed"] - eigs["fp64"])[:, :w.noccupied].max() / 2)
+        base = out["fp64_one_worker"]["solves_per_s"]
+        out["throughput_fp64_batched_vs_plain"] = out["fp64"]["solves_per_s"] / base
+        out["throughput_mixed_batched_vs_plain_fp64"] = out["mixed"]["solves_per_s"] / base
+        return out
+    except Exception as e:  # reported, not hidden
+        log(f"batched cfg2 failed: {e}")
         return {"error": str(e)}
 
 
@@ -214,7 +264,7 @@ def main():
     if args.mode == "copy":
         return copy_reference(args.steps)
     if args.mode == "batch":
-        print(json.dumps({"mode": "batch", **batched_kpoints(args)}))
+        print(json.dumps({"mode": "batch", "kpoints": batched_kpoints(args), "gamma": batched_gamma(args)}))
         return 0
     if args.mode == "serve":
         print(json.dumps({"mode": "serve", **serving(args)}))
@@ -368,7 +418,7 @@ def main():
             davidson = {"error": str(e)}
             log(f"davidson failed: {e}")
 
-    batched = None if args.no_batch else batched_kpoints(args)
+    batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args)}
     serve = None if args.no_serve else serving(args)
 
     cpu = None

@@ -54,6 +54,7 @@ SIGNATURES = {
     "q3x_basis_destroy": (_i, [_p]),
     "q3x_kpoint_create": (_i, [_p, _p, _pp, _p]),
     "q3x_kpoint_maps": (_i, [_p, _p, _p, _p]),
+    "q3x_kpoint_random_psi": (_i, [_p, _i, ctypes.c_ulonglong, _p]),
     "q3x_kpoint_destroy": (_i, [_p]),
     "q3x_kb_projectors": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "q3x_hamiltonian_create": (_i, [_p, _p, _pp]),

@@ -149,8 +149,17 @@ struct q3x_server_s {
             std::lock_guard<std::mutex> g(mu);
             r.psi[k] = {ctx, psi};
         }
+        // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, so a
+        // request's result does not depend on what else runs (plane-wave unit vectors, the
+        // solver's own seed, can miss symmetry sectors of these undisplaced crystals)
+        {
+            std::vector<double> p0(static_cast<size_t>(npw) * r.nb * 2);
+            if ((s = q3x_kpoint_random_psi(kp, r.nb, static_cast<unsigned long long>(k), p0.data())) != Q3X_OK ||
+                (s = q3x_copy_to_device(ctx, psi, p0.data(), p0.size() * sizeof(double))) != Q3X_OK)
+                return fail(s);
+        }
         q3x_davidson_options o = r.opt;
-        o.seed_from_psi = 0;
+        o.seed_from_psi = 1;
         q3x_davidson_report rep{};
         s = q3x_davidson_solve_device(h, r.nb, &o, psi, &r.eig[static_cast<size_t>(k) * r.nb], &rep);
         if (s != Q3X_OK) return fail(s);

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <map>
 #include <numeric>
+#include <random>
 #include <stdexcept>
 
 #include "q3x_internal.h"
@@ -328,6 +329,25 @@ q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* 
         if (kinetic_ry) kinetic_ry[i] = d.ekin[i];
         if (fft_map) fft_map[i] = d.fftmap[i];
     }
+    return Q3X_OK;
+}
+
+q3x_status q3x_kpoint_random_psi(q3x_kpoint k, int nb, unsigned long long seed, double* psi_out) {
+    if (!k || nb < 0 || (!psi_out && nb > 0)) {
+        set_error("q3x_kpoint_random_psi: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const std::vector<double>& ek = k->d.ekin;
+    const size_t npw = ek.size();
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int b = 0; b < nb; ++b)
+        for (size_t g = 0; g < npw; ++g) {
+            const double damp = 1.0 / (1.0 + ek[g]);
+            const size_t o = 2 * (static_cast<size_t>(b) * npw + g);
+            psi_out[o] = nd(gen) * damp;
+            psi_out[o + 1] = nd(gen) * damp;
+        }
     return Q3X_OK;
 }
 

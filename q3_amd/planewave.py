@@ -178,6 +178,13 @@ class KPoint:
         check(lib().q3x_kpoint_maps(h, _ptr(self.g_vector_indices), _ptr(self.kinetic_energy_squared),
                                     _ptr(self.fft_map)))
 
+    def random_psi(self, nbands: int, seed: int) -> np.ndarray:
+        """Starting block (npw x nbands, Fortran order): complex normal entries damped by
+        1 / (1 + |k+G|^2), the generator the band-structure server seeds k-point i with (seed i)."""
+        out = np.zeros((self.npw, nbands), np.complex128, order="F")
+        check(lib().q3x_kpoint_random_psi(self.h, nbands, seed, _ptr(out)))
+        return out
+
     def kb_projectors(self, crystal: Crystal, species):
         """Projector set <k+G|beta> (npw x nproj, Fortran order).  species: list of dicts with
         r, rab, l (list), cutoff (list), beta_r (nbeta x nmesh)."""

@@ -122,3 +122,16 @@ def test_host_subspace_eigensolver_degenerate_spectrum(n, nvec):
 def test_host_subspace_eigensolver_rejects_indefinite_overlap():
     st, _, _ = _hegv(np.eye(3, dtype=complex), np.diag([1.0, -1.0, 1.0]).astype(complex), 1)
     assert st == 4
+
+
+def test_random_starting_block_is_seeded_and_damped():
+    """q3x_kpoint_random_psi: the same seed gives the same block, another seed another one, and
+    the entries times (1 + |k+G|^2) are standard complex normals (mean 0, E|z|^2 = 2)."""
+    w = W.cfg4()
+    b = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, True, w.fft_factors)
+    kp = b.compute_gk(w.kpoints[3])
+    a1, a2, c = kp.random_psi(6, 11), kp.random_psi(6, 11), kp.random_psi(6, 12)
+    assert a1.shape == (kp.npw, 6) and a1.flags.f_contiguous
+    assert np.array_equal(a1, a2) and not np.allclose(a1, c)
+    z = a1 * (1.0 + kp.kinetic_energy_squared)[:, None]
+    assert abs(z.mean()) < 0.05 and abs((np.abs(z) ** 2).mean() - 2.0) < 0.1
