@@ -244,7 +244,9 @@ def serving(args):
                "workers": args.batch_workers, "requests": len(reqs), "p50_latency_s": float(np.percentile(lat, 50)),
                "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
                "solves_per_s": len(reqs) / span, "kpoint_solves_per_s": nks / span, "span_s": span,
-               "max_abs_err_ha_occupied_sampled": ref_err, "tolerance_ry": 2e-6}
+               "max_abs_err_ha_occupied_sampled": ref_err, "tolerance_ry": 2e-6,
+               "worker_setup_s_per_kpoint": sum(st["setup_s"] for _, st in results) / nks,
+               "worker_solve_s_per_kpoint": sum(st["solve_s"] for _, st in results) / nks}
         log(f"serving: p50 {out['p50_latency_s']:.2f}s p99 {out['p99_latency_s']:.2f}s, "
             f"{out['solves_per_s']:.2f} req/s, {out['kpoint_solves_per_s']:.1f} k-solves/s")
         return out

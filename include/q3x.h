@@ -259,6 +259,8 @@ typedef struct {
     double submit_s, admit_s, done_s;  /* seconds since the server started */
     double footprint_bytes;            /* the admission estimate */
     int iterations;                    /* Davidson iterations summed over the k-points */
+    double setup_s;                    /* worker time in per-k-point setup (G list, projectors, upload) */
+    double solve_s;                    /* worker time in the Davidson solves */
 } q3x_request_stats;
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation */
 q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out);

@@ -38,7 +38,8 @@ class BandRequest(ctypes.Structure):
 
 
 class RequestStats(ctypes.Structure):
-    _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i)]
+    _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i),
+                ("setup_s", _d), ("solve_s", _d)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/q3x.h

@@ -280,6 +280,31 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
 
 using namespace q3x;
 
+namespace q3x {
+q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b) {
+    if (!h || !b) {
+        set_error("hamiltonian rebind: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!hip_ok(hipStreamSynchronize(h->ctx->stream), "sync")) return Q3X_ERR_DEVICE;
+    // the FFT tables depend on the grid sizes only: kept when they do not change
+    const bool same_grid = h->fft[0].plan.n == b->d.wave[0] && h->fft[1].plan.n == b->d.wave[1] &&
+                           h->fft[2].plan.n == b->d.wave[2] && h->fft[0].plain64.size() > 0;
+    h->basis = b;
+    for (int d = 0; d < 3; ++d) h->dims[d] = b->d.wave[d];
+    h->npoints = static_cast<long>(h->dims[0]) * h->dims[1] * h->dims[2];
+    h->have_potential = false;
+    h->kp = nullptr;
+    h->npw = h->nst = h->ncol = h->n_outside = 0;
+    h->nproj = 0;
+    h->have_projectors = false;
+    if (!same_grid)
+        for (int d = 0; d < 3; ++d)
+            if (!build_tables(h->fft[d], h->dims[d], h->ctx->stream)) return Q3X_ERR_DEVICE;
+    return Q3X_OK;
+}
+}  // namespace q3x
+
 extern "C" {
 
 const char* q3x_last_error(void) { return g_last_error.c_str(); }
@@ -446,6 +471,7 @@ q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigne
                : Q3X_ERR_DEVICE;
 }
 
+
 q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian* out) {
     if (!ctx || !b || !out) {
         set_error("q3x_hamiltonian_create: invalid argument");
@@ -453,14 +479,11 @@ q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian*
     }
     auto* h = new q3x_hamiltonian_s;
     h->ctx = ctx;
-    h->basis = b;
-    for (int d = 0; d < 3; ++d) h->dims[d] = b->d.wave[d];
-    h->npoints = static_cast<long>(h->dims[0]) * h->dims[1] * h->dims[2];
-    for (int d = 0; d < 3; ++d)
-        if (!build_tables(h->fft[d], h->dims[d], ctx->stream)) {
-            delete h;
-            return Q3X_ERR_DEVICE;
-        }
+    const q3x_status s = hamiltonian_rebind(h, b);
+    if (s != Q3X_OK) {
+        delete h;
+        return s;
+    }
     *out = h;
     return Q3X_OK;
 }

@@ -9,6 +9,9 @@
 // m = 0, +1, -1, +2, -2, ... (harmonics.hpp:40-69).
 #include <algorithm>
 #include <cmath>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "q3x_internal.h"
@@ -135,6 +138,43 @@ void real_ylm(int lmax, const double v[3], double v2, double* out) {
 
 using namespace q3x;
 
+namespace q3x {
+namespace {
+
+using Tables = std::vector<std::vector<std::vector<double>>>;  // [species][beta][q]
+
+// Last few table sets with the inputs they came from (compared in full on lookup); shared by
+// all threads (server and batch workers build projectors concurrently).
+struct TableCache {
+    std::mutex mu;
+    std::list<std::pair<std::vector<double>, std::shared_ptr<const Tables>>> entries;
+};
+TableCache& table_cache() {
+    static TableCache c;
+    return c;
+}
+
+std::shared_ptr<const Tables> cached_tables(const std::vector<double>& key) {
+    TableCache& c = table_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    for (auto it = c.entries.begin(); it != c.entries.end(); ++it)
+        if (it->first == key) {
+            c.entries.splice(c.entries.begin(), c.entries, it);  // most recently used first
+            return c.entries.front().second;
+        }
+    return nullptr;
+}
+
+void store_tables(std::vector<double> key, std::shared_ptr<const Tables> t) {
+    TableCache& c = table_cache();
+    std::lock_guard<std::mutex> g(c.mu);
+    c.entries.emplace_front(std::move(key), std::move(t));
+    while (c.entries.size() > 8) c.entries.pop_back();
+}
+
+}  // namespace
+}  // namespace q3x
+
 extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, const double* atom_pos,
                                         const int* atom_species, int nspecies, const int* nmesh,
                                         const double* r_all, const double* rab_all, const int* nbeta,
@@ -183,11 +223,33 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
     if (!beta_out) return Q3X_OK;
 
     const int nq = static_cast<int>(std::sqrt(B.ecutrho) * 1.1 / kDq) + 4;
-    std::vector<std::vector<std::vector<double>>> table(nspecies);
-    for (int s = 0; s < nspecies; ++s)
-        for (int i = 0; i < sp[s].nbeta; ++i)
-            table[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
-                                            sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
+    // The beta(q) tables depend on the species data and nq only, not on k: every k-point of a
+    // cell (a band path, a batch) reuses them from a small cache keyed by the exact inputs.
+    std::vector<double> key;
+    key.push_back(nq);
+    key.push_back(nspecies);
+    for (int s = 0; s < nspecies; ++s) {
+        key.push_back(sp[s].nmesh);
+        key.push_back(sp[s].nbeta);
+        for (int i = 0; i < sp[s].nbeta; ++i) {
+            key.push_back(sp[s].l[i]);
+            key.push_back(sp[s].cut[i]);
+        }
+        key.insert(key.end(), sp[s].r, sp[s].r + sp[s].nmesh);
+        key.insert(key.end(), sp[s].rab, sp[s].rab + sp[s].nmesh);
+        key.insert(key.end(), sp[s].beta_r, sp[s].beta_r + static_cast<size_t>(sp[s].nbeta) * sp[s].nmesh);
+    }
+    std::shared_ptr<const Tables> tab = cached_tables(key);
+    if (!tab) {
+        auto t = std::make_shared<Tables>(nspecies);
+        for (int s = 0; s < nspecies; ++s)
+            for (int i = 0; i < sp[s].nbeta; ++i)
+                (*t)[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
+                                               sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
+        store_tables(std::move(key), t);
+        tab = t;
+    }
+    const Tables& table = *tab;
     double kc[3];
     for (int d = 0; d < 3; ++d) kc[d] = K.k[0] * L.b[0][d] + K.k[1] * L.b[1][d] + K.k[2] * L.b[2][d];
     const int nlm = (lmax + 1) * (lmax + 1);

@@ -33,15 +33,29 @@
 
 namespace q3x {
 
+__device__ __forceinline__ void wave_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // PG = false: the plane P lives in LDS.  PG = true (planes too large for LDS): P lives in this
 // workgroup's slot of pglob, [gridDim.y][gridDim.x][ncol][lc], which stays in L2 while the
 // workgroup runs; the rows in flight stay in LDS.
+//
+// Work split: a wave owns groups of G = 64 / max(A, B) transforms (columns, then rows, then
+// columns again) and carries each group through both stages with wave-level ordering only; the
+// lanes of a group are contiguous in the wave (lane = g * R + index, R lanes per transform in a
+// stage).  Waves run their groups independently, so one wave's LDS and ALU latency overlaps
+// another's; block barriers separate only the scatter, the column pass, the row pass (a row
+// needs every column), the forward column pass and the gather.  Each wave has its own buffer of
+// G rows in flight.
 template <typename T, int A, int B, bool PG>
-__global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
-                                                cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg, int lc,
-                                                int lr, int rb, cx<T>* __restrict__ pglob) {
+__global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
+                                                 cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg, int lc,
+                                                 int lr, int rb, cx<T>* __restrict__ pglob) {
     constexpr int N = A * B;
     constexpr int S = B + 1;  // stored length of a block of B slots
+    constexpr int G = 64 / (A > B ? A : B);
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // exp(-2 pi i j / N), j < N
     int* colmap = reinterpret_cast<int*>(tw + N);    // i1 -> occupied column or -1
@@ -49,8 +63,10 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
     const int ncol = a.ncol;
     cx<T>* P = PG ? pglob + (static_cast<long>(blockIdx.y) * gridDim.x + blockIdx.x) * ncol * lc
                   : lds_next;                // [ncol][lc]
-    cx<T>* R = PG ? lds_next : P + ncol * lc;  // [rb][lr] rows in flight
     const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * G * lr;  // this wave's [G][lr]
+    (void)rb;
     const int z = blockIdx.x, band = blockIdx.y;
     const cx<T> zero = {T(0), T(0)};
 
@@ -68,39 +84,42 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
     }
     __syncthreads();
 
-    // columns, backward stage 1: task t -> column t % ncol, lane n2 = t / ncol
-    for (int t = tid; t < ncol * B; t += nt) {
-        const int n2 = t / ncol, c = t - n2 * ncol;
-        cx<T>* col = P + c * lc + n2;
-        cx<T> v[A];
+    // lanes of stage-1 / stage-B (B lanes per transform) and stage-2 / stage-A (A lanes)
+    const int gB = lane / B, iB = lane - gB * B;
+    const int gA = lane / A, iA = lane - gA * A;
+
+    // columns, backward: stage 1 (lane n2 = iB), stage 2 (lane k1 = iA)
+    for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
+        if (gB < G && c0 + gB < ncol) {
+            const int n2 = iB;
+            cx<T>* col = P + (c0 + gB) * lc + n2;
+            cx<T> v[A];
 #pragma unroll
-        for (int j = 0; j < A; ++j) v[j] = col[j * S];
-        Dft<T, A>::run(v, T(1));
+            for (int j = 0; j < A; ++j) v[j] = col[j * S];
+            Dft<T, A>::run(v, T(1));
 #pragma unroll
-        for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
 #pragma unroll
-        for (int k = 0; k < A; ++k) col[k * S] = v[k];
-    }
-    __syncthreads();
-    // columns, backward stage 2: lane k1 owns the block at k1 * S
-    for (int t = tid; t < ncol * A; t += nt) {
-        const int k1 = t / ncol, c = t - k1 * ncol;
-        cx<T>* blk = P + c * lc + k1 * S;
-        cx<T> u[B];
+            for (int k = 0; k < A; ++k) col[k * S] = v[k];
+        }
+        wave_order();
+        if (gA < G && c0 + gA < ncol) {
+            cx<T>* blk = P + (c0 + gA) * lc + iA * S;
+            cx<T> u[B];
 #pragma unroll
-        for (int j = 0; j < B; ++j) u[j] = blk[j];
-        Dft<T, B>::run(u, T(1));
+            for (int j = 0; j < B; ++j) u[j] = blk[j];
+            Dft<T, B>::run(u, T(1));
 #pragma unroll
-        for (int j = 0; j < B; ++j) blk[j] = u[j];
+            for (int j = 0; j < B; ++j) blk[j] = u[j];
+        }
+        wave_order();
     }
     __syncthreads();
 
-    // rows in chunks of rb; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
-    for (int r0 = 0; r0 < N; r0 += rb) {
-        const int nr = min(rb, N - r0);
-        for (int t = tid; t < nr * B; t += nt) {
-            const int n2 = t / nr, r = t - n2 * nr;
-            const int i2 = r0 + r;
+    // rows; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
+    for (int r0 = wave * G; r0 < N; r0 += nw * G) {
+        if (gB < G && r0 + gB < N) {  // backward stage 1, lane n2
+            const int n2 = iB, i2 = r0 + gB;
             const int pp = (i2 % A) * S + i2 / A;
             cx<T> v[A];
 #pragma unroll
@@ -112,16 +131,14 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
             Dft<T, A>::run(v, T(1));
 #pragma unroll
             for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
-            cx<T>* row = R + r * lr + n2;
+            cx<T>* row = Rw + gB * lr + n2;
 #pragma unroll
             for (int k = 0; k < A; ++k) row[k * S] = v[k];
         }
-        __syncthreads();
-        // backward stage 2, x V(r)/N, forward stage A: lane k1, values at i1 = k1 + A k2
-        for (int t = tid; t < nr * A; t += nt) {
-            const int r = t / A, k1 = t - r * A;
-            const int i2 = r0 + r;
-            cx<T>* blk = R + r * lr + k1 * S;
+        wave_order();
+        if (gA < G && r0 + gA < N) {  // backward stage 2, x V(r)/N, forward stage A; lane k1
+            const int k1 = iA, i2 = r0 + gA;
+            cx<T>* blk = Rw + gA * lr + k1 * S;
             const T* vrow = vplane + (static_cast<long>(z) * N + i2) * N + k1;
             T vv[B];
 #pragma unroll
@@ -138,13 +155,11 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
 #pragma unroll
             for (int j = 0; j < B; ++j) blk[j] = u[j];
         }
-        __syncthreads();
-        // forward stage B: lane m2, outputs i1 = m2 + B m1, occupied columns only
-        for (int t = tid; t < nr * B; t += nt) {
-            const int m2 = t / nr, r = t - m2 * nr;
-            const int i2 = r0 + r;
+        wave_order();
+        if (gB < G && r0 + gB < N) {  // forward stage B, lane m2; outputs i1 = m2 + B m1, occupied only
+            const int m2 = iB, i2 = r0 + gB;
             const int pp = (i2 % A) * S + i2 / A;
-            const cx<T>* row = R + r * lr + m2;
+            const cx<T>* row = Rw + gB * lr + m2;
             cx<T> v[A];
 #pragma unroll
             for (int j = 0; j < A; ++j) v[j] = row[j * S];
@@ -155,33 +170,35 @@ __global__ void __launch_bounds__(768) plane_tl(HLocalArgs a, const T* __restric
                 if (c >= 0) P[c * lc + pp] = v[m];
             }
         }
-        __syncthreads();
-    }
-
-    // columns, forward stage A: lane n1 owns the block at n1 * S (elements n1 + A n2)
-    for (int t = tid; t < ncol * A; t += nt) {
-        const int n1 = t / ncol, c = t - n1 * ncol;
-        cx<T>* blk = P + c * lc + n1 * S;
-        cx<T> u[B];
-#pragma unroll
-        for (int j = 0; j < B; ++j) u[j] = blk[j];
-        Dft<T, B>::run(u, T(-1));
-#pragma unroll
-        for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
-#pragma unroll
-        for (int j = 0; j < B; ++j) blk[j] = u[j];
+        wave_order();
     }
     __syncthreads();
-    // columns, forward stage B: natural order out
-    for (int t = tid; t < ncol * B; t += nt) {
-        const int m2 = t / ncol, c = t - m2 * ncol;
-        cx<T>* col = P + c * lc + m2;
-        cx<T> v[A];
+
+    // columns, forward: stage A (lane n1 = iA, block n1 * S holds elements n1 + A n2), stage B
+    for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
+        if (gA < G && c0 + gA < ncol) {
+            const int n1 = iA;
+            cx<T>* blk = P + (c0 + gA) * lc + n1 * S;
+            cx<T> u[B];
 #pragma unroll
-        for (int j = 0; j < A; ++j) v[j] = col[j * S];
-        Dft<T, A>::run(v, T(-1));
+            for (int j = 0; j < B; ++j) u[j] = blk[j];
+            Dft<T, B>::run(u, T(-1));
 #pragma unroll
-        for (int m = 0; m < A; ++m) col[m * S] = v[m];
+            for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+#pragma unroll
+            for (int j = 0; j < B; ++j) blk[j] = u[j];
+        }
+        wave_order();
+        if (gB < G && c0 + gB < ncol) {
+            cx<T>* col = P + (c0 + gB) * lc + iB;
+            cx<T> v[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) v[j] = col[j * S];
+            Dft<T, A>::run(v, T(-1));
+#pragma unroll
+            for (int m = 0; m < A; ++m) col[m * S] = v[m];
+        }
+        wave_order();
     }
     __syncthreads();
     for (int s = tid; s < a.nst; s += nt) {
@@ -333,26 +350,30 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
     for (const TlShape& s : kTlShapes)
         if (s.n == n) A = s.a, B = s.b;
     if (A == 0) return cfg;
+    const int G = 64 / std::max(A, B);  // transforms per wave group (plane_tl)
     const size_t cs = fp32 ? 8 : 16;
     cfg.lc = odd_at_least(n + A);
     cfg.lr = cfg.lc;
     const size_t base = cs * n + 4 * static_cast<size_t>((n + 3) & ~3);
     const size_t plane = cs * static_cast<size_t>(ncol) * cfg.lc;
-    // the plane goes to LDS when at least min(n, 8) rows fit beside it
-    const long min_rows = std::min(n, 8);
-    cfg.plane_global = base + plane + cs * min_rows * cfg.lr > max_lds;
+    const size_t per_wave = cs * static_cast<size_t>(G) * cfg.lr;  // a wave's rows in flight
+    // the plane goes to LDS when at least four waves' rows fit beside it
+    cfg.plane_global = base + plane + 4 * per_wave > max_lds;
     const size_t fixed = base + (cfg.plane_global ? 0 : plane);
-    if (fixed >= max_lds) return cfg;
-    long rb = static_cast<long>((max_lds - fixed) / (cs * cfg.lr));
-    if (rb < 1) return cfg;
-    rb = std::min<long>(rb, n);
-    const long chunks = (n + rb - 1) / rb;
-    rb = (n + chunks - 1) / chunks;
-    cfg.rb = static_cast<int>(rb);
-    cfg.lds = fixed + cs * static_cast<size_t>(rb) * cfg.lr;
+    if (fixed + per_wave > max_lds) return cfg;
+    // waves: enough to cover the larger pass in one round of groups, up to 8 (512 threads keep
+    // the FP64 radix-12/15/16 stages free of spills); two workgroups
+    // per CU when the plane and four or more waves fit in half the LDS (the complex64 plane at
+    // cfg2), which overlaps their passes; a plane kept in global memory instead runs 2x slower
+    const int want = std::min(8, std::max(4, (std::max(n, ncol) + G - 1) / G));
+    int nw = static_cast<int>(std::min<size_t>(want, (max_lds - fixed) / per_wave));
+    const int nw_half = fixed < max_lds / 2 ? static_cast<int>((max_lds / 2 - fixed) / per_wave) : 0;
+    if (!cfg.plane_global && nw_half >= 4) nw = std::min(nw, nw_half);
+    if (nw < 1) return cfg;
+    cfg.rb = G;
+    cfg.threads = 64 * nw;
+    cfg.lds = fixed + per_wave * nw;
     cfg.plane_bytes = plane;
-    const int tasks = std::max(std::max(ncol * B, ncol * A), std::max(cfg.rb * B, cfg.rb * A));
-    cfg.threads = std::min(768, std::max(64, (tasks + 63) / 64 * 64));
     cfg.ok = true;
     return cfg;
 }

@@ -112,6 +112,10 @@ struct q3x_hamiltonian_s {
 };
 
 namespace q3x {
+// Points an existing Hamiltonian at another basis (tables rebuilt, k-point, potential and
+// projectors cleared) keeping its device buffers, which only grow: a long-lived worker reuses one
+// Hamiltonian across requests without allocating or freeing device memory once warm.
+q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b);
 // Applies H to nvec device vectors (ld = npw) on the context stream.
 q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long ldpsi, double* d_hpsi, long ldh,
                                     int nvec, int precision);

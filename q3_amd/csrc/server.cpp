@@ -47,8 +47,10 @@ struct Request {
     std::string error;
     std::vector<double> eig;
     std::vector<std::pair<q3x_context, void*>> psi;  // device wavefunctions, one per k-point
+    std::vector<size_t> psi_bytes;
     int iterations = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
+    double setup_s = 0, solve_s = 0;
 };
 
 double now_s(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -68,6 +70,25 @@ struct q3x_server_s {
     std::map<long, std::shared_ptr<Request>> all;
     long next_ticket = 1;
     bool stop = false;
+    // one Hamiltonian per worker, rebound to each k-point's cell; device buffers only grow
+    std::vector<q3x_hamiltonian> hams;
+    // device wavefunction blocks returned by finished requests, by size, for reuse: after warm-up
+    // a k-point takes a block from here instead of allocating (hipFree synchronises the device)
+    std::multimap<size_t, void*> psi_pool;
+
+    void* take_psi(q3x_context ctx, size_t bytes) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            auto it = psi_pool.lower_bound(bytes);
+            if (it != psi_pool.end()) {
+                void* p = it->second;
+                psi_pool.erase(it);
+                return p;
+            }
+        }
+        void* p = nullptr;
+        return q3x_device_alloc(ctx, static_cast<unsigned long>(bytes), &p) == Q3X_OK ? p : nullptr;
+    }
 
     // admission in arrival order: the head waits until it fits (or nothing else is in flight)
     void admit_locked() {
@@ -98,7 +119,7 @@ struct q3x_server_s {
                 if (r->next_task >= r->nk) running.pop_front();
             }
             std::string err;
-            const q3x_status s = solve_kpoint(ctx, *r, k, err);
+            const q3x_status s = solve_kpoint(w, ctx, *r, k, err);
             std::lock_guard<std::mutex> g(mu);
             if (s != Q3X_OK && r->status == Q3X_OK) {
                 r->status = s;
@@ -115,14 +136,14 @@ struct q3x_server_s {
         }
     }
 
-    q3x_status solve_kpoint(q3x_context ctx, Request& r, int k, std::string& err) {
+    q3x_status solve_kpoint(int w, q3x_context ctx, Request& r, int k, std::string& err) {
+        const clk::time_point ts = clk::now();
         q3x_kpoint kp = nullptr;
-        q3x_hamiltonian h = nullptr;
+        q3x_hamiltonian& h = hams[w];
         int npw = 0, nproj = 0;
         q3x_status s = q3x_kpoint_create(r.basis, &r.kpts[3 * k], &kp, &npw);
         auto fail = [&](q3x_status st) {
             err = q3x_last_error();
-            if (h) q3x_hamiltonian_destroy(h);
             if (kp) q3x_kpoint_destroy(kp);
             return st;
         };
@@ -137,17 +158,19 @@ struct q3x_server_s {
         if ((s = kb(beta.data())) != Q3X_OK) return fail(s);
         // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
         // column-major layout the Hamiltonian takes
-        if ((s = q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
+        if ((s = h ? q3x::hamiltonian_rebind(h, r.basis) : q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
             (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK ||
             (s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
             (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, beta.data(), nproj, r.natoms, r.block_sizes.data(),
                                                                r.d_blocks.data())) != Q3X_OK))
             return fail(s);
-        void* psi = nullptr;
-        if ((s = q3x_device_alloc(ctx, static_cast<unsigned long>(npw) * r.nb * 16, &psi)) != Q3X_OK) return fail(s);
+        const size_t psi_bytes = static_cast<size_t>(npw) * r.nb * 16;
+        void* psi = take_psi(ctx, psi_bytes);
+        if (!psi) return fail(Q3X_ERR_DEVICE);
         {
             std::lock_guard<std::mutex> g(mu);
             r.psi[k] = {ctx, psi};
+            r.psi_bytes[k] = psi_bytes;
         }
         // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, so a
         // request's result does not depend on what else runs (plane-wave unit vectors, the
@@ -161,13 +184,16 @@ struct q3x_server_s {
         q3x_davidson_options o = r.opt;
         o.seed_from_psi = 1;
         q3x_davidson_report rep{};
+        const clk::time_point tsolve = clk::now();
         s = q3x_davidson_solve_device(h, r.nb, &o, psi, &r.eig[static_cast<size_t>(k) * r.nb], &rep);
         if (s != Q3X_OK) return fail(s);
+        const clk::time_point tend = clk::now();
         {
             std::lock_guard<std::mutex> g(mu);
             r.iterations += rep.iterations;
+            r.setup_s += std::chrono::duration<double>(tsolve - ts).count();
+            r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
         }
-        q3x_hamiltonian_destroy(h);
         q3x_kpoint_destroy(kp);
         return Q3X_OK;
     }
@@ -221,6 +247,7 @@ q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
     s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
+    s->hams.assign(workers, nullptr);
     for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
     *out = s;
     return Q3X_OK;
@@ -242,6 +269,9 @@ q3x_status q3x_server_destroy(q3x_server s) {
     for (auto& kv : s->all)
         for (auto& cp : kv.second->psi)
             if (cp.second) q3x_device_free(cp.first, cp.second);
+    for (auto& kv : s->psi_pool) q3x_device_free(s->workers[0], kv.second);
+    for (q3x_hamiltonian h : s->hams)
+        if (h) q3x_hamiltonian_destroy(h);
     for (q3x_context c : s->workers) q3x_context_destroy(c);
     delete s;
     return Q3X_OK;
@@ -307,6 +337,7 @@ q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* tick
     r->remaining = q->nk;
     r->eig.assign(static_cast<size_t>(q->nk) * q->nbands, 0.0);
     r->psi.assign(q->nk, {nullptr, nullptr});
+    r->psi_bytes.assign(q->nk, 0);
     std::lock_guard<std::mutex> g(s->mu);
     r->ticket = s->next_ticket++;
     r->t_submit = now_s(s->t0);
@@ -341,9 +372,14 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
         stats->done_s = r->t_done;
         stats->footprint_bytes = r->bytes;
         stats->iterations = r->iterations;
+        stats->setup_s = r->setup_s;
+        stats->solve_s = r->solve_s;
     }
-    for (auto& cp : r->psi)
-        if (cp.second) q3x_device_free(cp.first, cp.second);
+    {
+        std::lock_guard<std::mutex> g(s->mu);  // the request's wavefunction blocks go back to the pool
+        for (size_t k = 0; k < r->psi.size(); ++k)
+            if (r->psi[k].second) s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
+    }
     r->psi.clear();
     if (r->status != Q3X_OK) set_error(r->error);
     return r->status;

@@ -469,7 +469,8 @@ class BandStructureServer:
         finally:
             del self._live[ticket]
         return eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
-                     "footprint_bytes": st.footprint_bytes, "iterations": st.iterations}
+                     "footprint_bytes": st.footprint_bytes, "iterations": st.iterations,
+                     "setup_s": st.setup_s, "solve_s": st.solve_s}
 
     def in_flight_bytes(self) -> float:
         return lib().q3x_server_in_flight_bytes(self.h)
