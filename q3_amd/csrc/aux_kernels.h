@@ -21,5 +21,7 @@ hipError_t launch_gather_columns(int rows, int ncol, const int* src, const doubl
 // plain copy of n16 16-byte elements (HBM byte-counter reference)
 hipError_t launch_copy16(const void* src, void* dst, long n16, hipStream_t st);
 hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st);
+// X (n x n, leading dimension ldx, complex128) = identity
+hipError_t launch_identity(int n, double* X, long ldx, hipStream_t st);
 
 }  // namespace q3x

@@ -182,6 +182,22 @@ __global__ void set_unit_columns(const int* __restrict__ idx, int ncol, double* 
     V[2 * (idx[c] + static_cast<long>(c) * ldv)] = 1.0;
 }
 
+// n x n complex identity in X (leading dimension ldx).
+__global__ void set_identity(int n, double* __restrict__ X, long ldx) {
+    const long e = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= static_cast<long>(n) * n) return;
+    const int i = static_cast<int>(e % n), j = static_cast<int>(e / n);
+    X[2 * (i + j * ldx)] = i == j ? 1.0 : 0.0;
+    X[2 * (i + j * ldx) + 1] = 0.0;
+}
+
+hipError_t launch_identity(int n, double* X, long ldx, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const long total = static_cast<long>(n) * n;
+    hipLaunchKernelGGL(set_identity, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, st, n, X, ldx);
+    return hipGetLastError();
+}
+
 hipError_t launch_unit_columns(const int* idx, int ncol, double* V, long ldv, hipStream_t st) {
     if (ncol == 0) return hipSuccess;
     hipLaunchKernelGGL(set_unit_columns, dim3((ncol + 63) / 64), dim3(64), 0, st, idx, ncol, V, ldv);

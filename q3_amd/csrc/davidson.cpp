@@ -198,7 +198,9 @@ struct Solver {
         return stage_sync();
     }
 
-    int collapse(int n, bool have_matrices) {
+    // Collapse to the nb lowest Ritz vectors (serial.hpp:825-945).  w_out (optional) receives the
+    // Ritz values: the rebuilt nb x nb subspace is diag(w) to rounding, with eigenvectors I.
+    int collapse(int n, bool have_matrices, std::vector<double>* w_out = nullptr) {
         const double t0 = now_ms();
         std::vector<double> w;
         int info = -2;
@@ -216,8 +218,18 @@ struct Solver {
         }
         if (!rotate(n, nb)) return -1;
         if (!update(0, nb, nb)) return -1;
+        if (w_out) *w_out = w;
         stage[5] += now_ms() - t0;
         return nb;
+    }
+
+    // The eigenpairs of the subspace right after a collapse: the collapse's Ritz values and the
+    // identity (the basis is made of the Ritz vectors), in place of a solve of the rebuilt
+    // subspace, whose result differs from these at rounding level.
+    bool diag_after_collapse(const std::vector<double>& wc, std::vector<double>& w) {
+        w.assign(std::max<size_t>(wc.size(), static_cast<size_t>(mmax)), 0.0);
+        std::copy(wc.begin(), wc.end(), w.begin());
+        return ok(launch_identity(nb, X_d.as<double>(), nb, st), "identity eigenvectors");
     }
 };
 
@@ -321,11 +333,19 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     const double empty_tol = std::max(kEmptyFactor * tol, kEmptyFloor);
     int it = 0, unconv = nb, iterations_run = 0, refinements = 0;
     bool done = false;
+    bool collapsed = false;  // the subspace was just rebuilt from Ritz vectors with values wc
+    std::vector<double> wc;
     int max_iter = opt->max_iterations;
     for (;;) {
         for (it = 0; it < max_iter; ++it) {
             ++iterations_run;
-            int info = s.diag(basis, w, nb);
+            int info = 0;
+            if (collapsed && basis == nb) {
+                if (!s.diag_after_collapse(wc, w)) return s.status;
+            } else {
+                info = s.diag(basis, w, nb);
+            }
+            collapsed = false;
             if (info < 0) return s.status;
             if (info != 0 && basis > nb) {  // davidson.hpp:520-590
                 basis = s.collapse(basis, false);
@@ -388,8 +408,9 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
             basis += accepted;
             if (!s.update(old, accepted, basis)) return s.status;
             if (basis >= mmax) {
-                basis = s.collapse(basis, true);
+                basis = s.collapse(basis, true, &wc);
                 if (basis < 0) return s.status;
+                collapsed = static_cast<int>(wc.size()) >= nb;
             }
             for (int b = 0; b < nb; ++b) prev[b] = w[b];
         }
