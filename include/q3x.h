@@ -78,11 +78,11 @@ q3x_status q3x_basis_destroy(q3x_basis b);
 q3x_status q3x_kpoint_create(q3x_basis b, const double k_crystal[3], q3x_kpoint* out, int* npw);
 /* g_index: npw ints into the basis G list; kinetic: |k+G|^2 in Ry; fft_map: wave-grid offsets */
 q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* fft_map);
-/* Starting block for a solve: npw x nb complex128, column-major, entries complex normal (from a
- * 64-bit Mersenne twister seeded with seed) damped by 1 / (1 + |k+G|^2 [Ry]).  The band-
- * structure server seeds k-point i of a request with seed i, so a request's solve is the same
- * alone and among others.  Host-only; no counterpart in the reference's interface (its random
- * starting wavefunctions come from its own generator). */
+/* Starting block for a solve: npw x nb complex128, column-major, real and imaginary parts
+ * uniform in [-1, 1) (a splitmix64 hash of seed and the entry's position) damped by
+ * 1 / (1 + |k+G|^2 [Ry]).  The band-structure server seeds k-point i of a request with seed i,
+ * so a request's solve is the same alone and among others.  Host-only; no counterpart in the
+ * reference's interface (its random starting wavefunctions come from its own generator). */
 q3x_status q3x_kpoint_random_psi(q3x_kpoint k, int nb, unsigned long long seed, double* psi_out);
 q3x_status q3x_kpoint_destroy(q3x_kpoint k);
 

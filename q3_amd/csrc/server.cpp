@@ -72,6 +72,7 @@ struct q3x_server_s {
     bool stop = false;
     // one Hamiltonian per worker, rebound to each k-point's cell; device buffers only grow
     std::vector<q3x_hamiltonian> hams;
+    std::vector<long> ham_ticket;  // request whose cell and potential hams[w] holds (0: none)
     // device wavefunction blocks returned by finished requests, by size, for reuse: after warm-up
     // a k-point takes a block from here instead of allocating (hipFree synchronises the device)
     std::multimap<size_t, void*> psi_pool;
@@ -158,9 +159,16 @@ struct q3x_server_s {
         if ((s = kb(beta.data())) != Q3X_OK) return fail(s);
         // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
         // column-major layout the Hamiltonian takes
-        if ((s = h ? q3x::hamiltonian_rebind(h, r.basis) : q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
-            (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK ||
-            (s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
+        // a worker's next k-point usually belongs to the same request: cell, tables and V stay
+        const bool same_cell = h && ham_ticket[w] == r.ticket;
+        if (!same_cell) {
+            ham_ticket[w] = 0;
+            if ((s = h ? q3x::hamiltonian_rebind(h, r.basis) : q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
+                (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK)
+                return fail(s);
+            ham_ticket[w] = r.ticket;
+        }
+        if ((s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
             (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, beta.data(), nproj, r.natoms, r.block_sizes.data(),
                                                                r.d_blocks.data())) != Q3X_OK))
             return fail(s);
@@ -248,6 +256,7 @@ q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q
     (void)hipMemGetInfo(&free_b, &total_b);
     s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
     s->hams.assign(workers, nullptr);
+    s->ham_ticket.assign(workers, 0);
     for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
     *out = s;
     return Q3X_OK;

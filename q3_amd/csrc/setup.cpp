@@ -11,7 +11,6 @@
 #include <cmath>
 #include <map>
 #include <numeric>
-#include <random>
 #include <stdexcept>
 
 #include "q3x_internal.h"
@@ -339,14 +338,23 @@ q3x_status q3x_kpoint_random_psi(q3x_kpoint k, int nb, unsigned long long seed, 
     }
     const std::vector<double>& ek = k->d.ekin;
     const size_t npw = ek.size();
-    std::mt19937_64 gen(seed);
-    std::normal_distribution<double> nd(0.0, 1.0);
+    // entry (g, b), part r: splitmix64 of (seed, 2 (b npw + g) + r) -> uniform in [-1, 1)
+    auto mix = [](unsigned long long x) {
+        x += 0x9E3779B97F4A7C15ull;
+        x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+        x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+        return x ^ (x >> 31);
+    };
+    const unsigned long long s0 = mix(seed);
+    const double scale = 1.0 / 4503599627370496.0;  // 2^-52
     for (int b = 0; b < nb; ++b)
         for (size_t g = 0; g < npw; ++g) {
             const double damp = 1.0 / (1.0 + ek[g]);
             const size_t o = 2 * (static_cast<size_t>(b) * npw + g);
-            psi_out[o] = nd(gen) * damp;
-            psi_out[o + 1] = nd(gen) * damp;
+            const double re = static_cast<double>(mix(s0 + o) >> 11) * scale - 1.0;
+            const double im = static_cast<double>(mix(s0 + o + 1) >> 11) * scale - 1.0;
+            psi_out[o] = re * damp;
+            psi_out[o + 1] = im * damp;
         }
     return Q3X_OK;
 }

@@ -179,8 +179,8 @@ class KPoint:
                                     _ptr(self.fft_map)))
 
     def random_psi(self, nbands: int, seed: int) -> np.ndarray:
-        """Starting block (npw x nbands, Fortran order): complex normal entries damped by
-        1 / (1 + |k+G|^2), the generator the band-structure server seeds k-point i with (seed i)."""
+        """Starting block (npw x nbands, Fortran order): entries uniform in the unit square damped
+        by 1 / (1 + |k+G|^2), the generator the band-structure server seeds k-point i with (seed i)."""
         out = np.zeros((self.npw, nbands), np.complex128, order="F")
         check(lib().q3x_kpoint_random_psi(self.h, nbands, seed, _ptr(out)))
         return out

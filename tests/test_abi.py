@@ -126,7 +126,8 @@ def test_host_subspace_eigensolver_rejects_indefinite_overlap():
 
 def test_random_starting_block_is_seeded_and_damped():
     """q3x_kpoint_random_psi: the same seed gives the same block, another seed another one, and
-    the entries times (1 + |k+G|^2) are standard complex normals (mean 0, E|z|^2 = 2)."""
+    the entries times (1 + |k+G|^2) have real and imaginary parts uniform in [-1, 1) (mean 0,
+    E|z|^2 = 2/3, all within the unit square)."""
     w = W.cfg4()
     b = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, True, w.fft_factors)
     kp = b.compute_gk(w.kpoints[3])
@@ -134,4 +135,5 @@ def test_random_starting_block_is_seeded_and_damped():
     assert a1.shape == (kp.npw, 6) and a1.flags.f_contiguous
     assert np.array_equal(a1, a2) and not np.allclose(a1, c)
     z = a1 * (1.0 + kp.kinetic_energy_squared)[:, None]
-    assert abs(z.mean()) < 0.05 and abs((np.abs(z) ** 2).mean() - 2.0) < 0.1
+    assert abs(z.mean()) < 0.02 and abs((np.abs(z) ** 2).mean() - 2.0 / 3.0) < 0.02
+    assert np.abs(z.real).max() <= 1.0 and np.abs(z.imag).max() <= 1.0
