@@ -68,6 +68,28 @@ __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_mov<0xB1>(v);
     return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
+// Sums of V values per lane over the wavefront at once (V a power of 2, <= 64): each exchange
+// stage swaps half of the live values with the partner lane and adds, halving V, so V sums
+// cost about log2(64) + V - 1 exchanges instead of 6 V.  On return lane l holds the full sum of
+// value l / (64 / V).
+template <int V>
+__device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
+    int o = 32;
+#pragma unroll
+    for (int h = V / 2; h >= 1; h /= 2, o /= 2) {
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            const double send = up ? x[i] : x[i + h];
+            const double keep = up ? x[i + h] : x[i];
+            x[i] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    double s = x[0];
+#pragma unroll
+    for (; o >= 1; o /= 2) s += __shfl_xor(s, o, 64);
+    return s;
+}
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -298,51 +320,78 @@ __device__ double column_step(z2* M, int n, int j, double tn, const z2* vn, cons
 }
 
 // Pass of step k over columns / rows [k+2, n): rank-2 update with (v, q) when tk != 0, then
-// p[c] = tn * (column)^H vn when tn != 0.  Wavefront w takes CB adjacent columns at a time and
-// issues all their loads before using them.  v, q index rows from k+1; vn and p from k+2.
+// p[c] = tn * (column)^H vn when tn != 0.  Wavefront w takes CB adjacent columns at a time.  The
+// row vectors v, q, vn are the same for every column of the pass, so each lane loads its rows of
+// them into registers once (they share LDS with the matrix, so re-reading them after a column
+// store would be a dependent LDS round trip per column); the CB columns' loads are issued
+// together, and their 2 CB wave reductions run side by side with no branch between them.
+// v, q index rows from k+1; vn and p from k+2.
 template <int RPL, int CB, bool LM>
 __device__ void fused_pass(z2* M, int n, int k, double tk, const z2* v, const z2* q, double tn, const z2* vn,
                            z2* p) {
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = blockDim.x >> 6;
     const int len = n - k - 2, base = k + 2;
+    if (wv * CB >= len) return;
+    z2 vr[RPL], qr[RPL], wr[RPL];
+    double msk[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        const int rc = min(r, len - 1);  // LDS vectors: stay in range; rows past the end are 0
+        vr[t] = tk != 0.0 ? v[rc + 1] : z2{0.0, 0.0};
+        qr[t] = tk != 0.0 ? q[rc + 1] : z2{0.0, 0.0};
+        wr[t] = vn[rc];
+        msk[t] = r < len ? 1.0 : 0.0;
+    }
     for (int c0 = wv * CB; c0 < len; c0 += nw * CB) {
-        z2 a[CB][RPL];
+        z2 a[CB][RPL], qj[CB], vj[CB];
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
             const int c = min(c0 + b, len - 1);  // a missing column re-reads the last one (unused)
             const ColumnView<LM> col(M + static_cast<long>(base + c) * n + base, len);
 #pragma unroll
             for (int t = 0; t < RPL; ++t) a[b][t] = col.load(lane + 64 * t);
+            qj[b] = tk != 0.0 ? q[c + 1] : z2{0.0, 0.0};
+            vj[b] = tk != 0.0 ? v[c + 1] : z2{0.0, 0.0};
         }
+        double sr[CB], si[CB];
 #pragma unroll
         for (int b = 0; b < CB; ++b) {
-            const int c = c0 + b;
-            if (c >= len) break;
-            const ColumnView<LM> col(M + static_cast<long>(base + c) * n + base, len);
-            const z2 qj = tk != 0.0 ? q[c + 1] : z2{0.0, 0.0};
-            const z2 vj = tk != 0.0 ? v[c + 1] : z2{0.0, 0.0};
-            double sr = 0.0, si = 0.0;
+            sr[b] = 0.0;
+            si[b] = 0.0;
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
-                const int r = lane + 64 * t;
-                const int rc = min(r, len - 1);  // LDS vectors: stay in range; rows past the end are 0
                 if (tk != 0.0) {
-                    const z2 t1 = zmulc(v[rc + 1], qj), t2 = zmulc(q[rc + 1], vj);
+                    const z2 t1 = zmulc(vr[t], qj[b]), t2 = zmulc(qr[t], vj[b]);
                     a[b][t].x -= t1.x + t2.x;
                     a[b][t].y -= t1.y + t2.y;
-                    col.store(r, a[b][t]);
                 }
-                const z2 w = vn[rc];
-                const double msk = r < len ? 1.0 : 0.0;
-                sr += msk * (a[b][t].x * w.x + a[b][t].y * w.y);
-                si += msk * (a[b][t].x * w.y - a[b][t].y * w.x);
+                sr[b] += msk[t] * (a[b][t].x * wr[t].x + a[b][t].y * wr[t].y);
+                si[b] += msk[t] * (a[b][t].x * wr[t].y - a[b][t].y * wr[t].x);
             }
-            if (tn != 0.0) {
-                sr = wave_sum(sr);
-                si = wave_sum(si);
-                if (lane == 0) p[c] = {tn * sr, tn * si};
+        }
+        if (tk != 0.0) {
+#pragma unroll
+            for (int b = 0; b < CB; ++b) {
+                if (c0 + b < len) {
+                    const ColumnView<LM> col(M + static_cast<long>(base + c0 + b) * n + base, len);
+#pragma unroll
+                    for (int t = 0; t < RPL; ++t) col.store(lane + 64 * t, a[b][t]);
+                }
             }
+        }
+        if (tn != 0.0) {
+            double x[2 * CB];
+#pragma unroll
+            for (int b = 0; b < CB; ++b) {
+                x[2 * b] = sr[b];
+                x[2 * b + 1] = si[b];
+            }
+            const double tot = wave_multi_sum<2 * CB>(x, lane);
+            constexpr int span = 64 / (2 * CB);
+            const int idx = lane / span, b = idx >> 1;
+            if (lane % span == 0 && c0 + b < len) reinterpret_cast<double*>(p + c0 + b)[idx & 1] = tn * tot;
         }
     }
 }
@@ -359,12 +408,14 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
     z2* vb1 = sh + n;
     z2* q = sh + 2 * n;
     z2* p = sh + 3 * n;
-    double* red = reinterpret_cast<double*>(sh + 4 * n);  // 32 doubles
-    double& tnext = *reinterpret_cast<double*>(sh + 4 * n + 16);
+    double& tnext = *reinterpret_cast<double*>(sh + 4 * n + 16);  // sh + 4n .. 4n + 15: spare
     z2* M = LM ? sh + 4 * n + 17 : A;
+    // d, beta and tau are kept in LDS and written to global memory once at the end
+    z2* sbeta = sh + 4 * n + 17 + (LM ? static_cast<long>(n) * n : 0);
+    double* sd = reinterpret_cast<double*>(sbeta + n);
+    double* stau = sd + n;
     const int tid = threadIdx.x, nt = blockDim.x, wv = tid >> 6;
     const long nn = static_cast<long>(n) * n;
-    int slot = 0;
     if (LM) {
         for (long e = tid; e < nn; e += nt) M[e] = A[e];
         __syncthreads();
@@ -381,23 +432,24 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
             M[e].y = 0.0;
         }
     }
-    if (tid == 0) {
-        beta[0] = {0.0, 0.0};
-        for (int k = 0; k < n; ++k) tau[k] = 0.0;
+    for (int i = tid; i < n; i += nt) {
+        sbeta[i] = {0.0, 0.0};
+        stau[i] = 0.0;
+        sd[i] = 0.0;
     }
     __syncthreads();
     if (n <= 2) {
         if (tid == 0) {
-            d[0] = M[0].x;
+            sd[0] = M[0].x;
             if (n == 2) {
-                d[1] = M[3].x;
-                beta[1] = M[1];
+                sd[1] = M[3].x;
+                sbeta[1] = M[1];
             }
         }
     } else {
         // reflector 0 (column 0 needs no update)
         if (wv == 0) {
-            const double t0 = column_step<RPL, LM>(M, n, 0, 0.0, nullptr, nullptr, 0.0, vb0, d, beta, tau);
+            const double t0 = column_step<RPL, LM>(M, n, 0, 0.0, nullptr, nullptr, 0.0, vb0, sd, sbeta, stau);
             if ((tid & 63) == 0) tnext = t0;
         }
         __syncthreads();
@@ -407,17 +459,18 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
         for (int k = -1; k <= n - 3; ++k) {
             if (tk != 0.0 || tn != 0.0) fused_pass<RPL, CB, LM>(M, n, k, tk, v, q, tn, vn, p);
             __syncthreads();
-            double f = 0.0;
             const int len = n - k - 2;
-            if (tn != 0.0) {
-                double part = 0.0;
-                for (int i = tid; i < len; i += nt) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
-                f = 0.5 * tn * block_sum(part, red, slot);
-                for (int i = tid; i < len; i += nt) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
-            }
-            if (wv == 0) {
-                const double tj = column_step<RPL, LM>(M, n, k + 2, tn, vn, p, f, v, d, beta, tau);
-                if ((tid & 63) == 0) tnext = tj;
+            if (wv == 0) {  // f = tn vn^H p / 2, q = p - f vn, next column and reflector: one wavefront
+                const int lane = tid & 63;
+                double f = 0.0;
+                if (tn != 0.0) {
+                    double part = 0.0;
+                    for (int i = lane; i < len; i += 64) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
+                    f = 0.5 * tn * wave_sum(part);
+                    for (int i = lane; i < len; i += 64) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
+                }
+                const double tj = column_step<RPL, LM>(M, n, k + 2, tn, vn, p, f, v, sd, sbeta, stau);
+                if (lane == 0) tnext = tj;
             }
             __syncthreads();
             tk = tn;
@@ -428,12 +481,17 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
         }
     }
     __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+        d[i] = sd[i];
+        beta[i] = sbeta[i];
+        tau[i] = stau[i];
+    }
     if (tid == 0) {
         z2 ph = {1.0, 0.0};
         phase[0] = ph;
         eabs[0] = 0.0;
         for (int i = 1; i < n; ++i) {
-            const z2 b = beta[i];
+            const z2 b = sbeta[i];
             const double r = hypot(b.x, b.y);
             eabs[i] = r;
             if (r != 0.0) ph = zmul(ph, z2{b.x / r, b.y / r});
@@ -965,7 +1023,7 @@ Layout layout(int n, int nvec) {
     return L;
 }
 
-size_t tridiag_lds(int n, bool lm) { return (4 * static_cast<size_t>(n) + 17 + (lm ? static_cast<size_t>(n) * n : 0)) * 16; }
+size_t tridiag_lds(int n, bool lm) { return (6 * static_cast<size_t>(n) + 17 + (lm ? static_cast<size_t>(n) * n : 0)) * 16; }
 
 }  // namespace
 
