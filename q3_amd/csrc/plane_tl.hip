@@ -207,6 +207,20 @@ __global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restric
     }
 }
 
+// Workgroup -> (stick block, band) for the z-stick kernels.  Workgroups are dispatched to the
+// 8 XCDs round-robin by linear id, so id mod 8 names the XCD; giving every stick block of band b
+// to XCD b mod 8 makes each band's psi / Hpsi column live in one L2, where the scattered 16-byte
+// gathers and scatters of the sphere entries then hit (instead of every XCD fetching most of
+// every band).  1D grid of 8 * nblk * ceil(nvec / 8) workgroups; the surplus ones return.
+__device__ __forceinline__ bool stick_block(int nblk, int nvec, int& sb, int& band) {
+    const int i = blockIdx.x;
+    const int x = i & 7, r = i >> 3;
+    const int q = r / nblk;
+    band = x + 8 * q;
+    sb = r - q * nblk;
+    return band < nvec;
+}
+
 // z-stick transforms with the same two-level split, N = n3 = A * B.  The sticks of a block
 // sit in LDS at sl * lz + slot(q), slot(q) = q + q / B, lanes running across sticks fastest.
 //
@@ -215,26 +229,28 @@ __global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restric
 template <typename T, int A, int B>
 __global__ void __launch_bounds__(256) zsticks_inverse_tl(HLocalArgs a, const double* __restrict__ psi, long ldpsi,
                                                           cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg,
-                                                          int spb, int lz) {
+                                                          int spb, int lz, int nblk, int nvec) {
     constexpr int N = A * B;
     constexpr int S = B + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* buf = tw + N;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int s0 = blockIdx.x * spb;
+    int sb, band;
+    if (!stick_block(nblk, nvec, sb, band)) return;
+    const int s0 = sb * spb;
     const int nsb = min(spb, a.nst - s0);
-    const int band = blockIdx.y;
     for (int i = tid; i < N; i += nt) tw[i] = twg[i];
     for (int i = tid; i < nsb * lz; i += nt) buf[i] = cx<T>{T(0), T(0)};
     __syncthreads();
-    const double* src = psi + band * ldpsi * 2;
+    const double2* src = reinterpret_cast<const double2*>(psi + band * ldpsi * 2);
     const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
     const int base = s0 * N;
     for (int j = j0 + tid; j < j1; j += nt) {
         const int p = a.ent_pw[j];
         const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
-        buf[sl * lz + z + z / B] = cx<T>{T(src[2 * p]), T(src[2 * p + 1])};
+        const double2 x = src[p];
+        buf[sl * lz + z + z / B] = cx<T>{T(x.x), T(x.y)};
     }
     __syncthreads();
     for (int t = tid; t < nsb * B; t += nt) {
@@ -270,16 +286,18 @@ template <typename T, int A, int B>
 __global__ void __launch_bounds__(256) zsticks_forward_tl(HLocalArgs a, const cx<T>* __restrict__ zbuf,
                                                           const double* __restrict__ psi, long ldpsi,
                                                           double* __restrict__ hpsi, long ldh,
-                                                          const cx<T>* __restrict__ twg, int spb, int lz) {
+                                                          const cx<T>* __restrict__ twg, int spb, int lz, int nblk,
+                                                          int nvec) {
     constexpr int N = A * B;
     constexpr int S = B + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
     cx<T>* buf = tw + N;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int s0 = blockIdx.x * spb;
+    int sb, band;
+    if (!stick_block(nblk, nvec, sb, band)) return;
+    const int s0 = sb * spb;
     const int nsb = min(spb, a.nst - s0);
-    const int band = blockIdx.y;
     for (int i = tid; i < N; i += nt) tw[i] = twg[i];
     __syncthreads();
     const cx<T>* in = zbuf + static_cast<long>(band) * N * a.nst + s0;
@@ -307,8 +325,8 @@ __global__ void __launch_bounds__(256) zsticks_forward_tl(HLocalArgs a, const cx
         for (int m = 0; m < A; ++m) x[m * S] = v[m];
     }
     __syncthreads();
-    const double* src = psi + band * ldpsi * 2;
-    double* dst = hpsi + band * ldh * 2;
+    const double2* src = reinterpret_cast<const double2*>(psi + band * ldpsi * 2);
+    double2* dst = reinterpret_cast<double2*>(hpsi + band * ldh * 2);
     const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
     const int base = s0 * N;
     for (int j = j0 + tid; j < j1; j += nt) {
@@ -316,8 +334,8 @@ __global__ void __launch_bounds__(256) zsticks_forward_tl(HLocalArgs a, const cx
         const double t = a.ekin[p];
         const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
         const cx<T> v = buf[sl * lz + z + z / B];
-        dst[2 * p] = t * src[2 * p] + static_cast<double>(v.x);
-        dst[2 * p + 1] = t * src[2 * p + 1] + static_cast<double>(v.y);
+        const double2 x = src[p];
+        dst[p] = double2{t * x.x + static_cast<double>(v.x), t * x.y + static_cast<double>(v.y)};
     }
 }
 
@@ -412,14 +430,15 @@ static hipError_t launch_sticks_shape(const HLocalArgs& a, int dir, const double
         if (e != hipSuccess) return e;
         raised = true;
     }
-    const dim3 grid((a.nst + c.spb - 1) / c.spb, nvec);
+    const int nblk = (a.nst + c.spb - 1) / c.spb;
+    const dim3 grid(static_cast<unsigned>(8L * nblk * ((nvec + 7) / 8)));
     const cx<T>* tw = static_cast<const cx<T>*>(a.twp3);
     if (dir > 0)
         hipLaunchKernelGGL((zsticks_inverse_tl<T, A, B>), grid, dim3(c.threads), c.lds, st, a, psi, ldpsi, zbuf, tw,
-                           c.spb, c.lz);
+                           c.spb, c.lz, nblk, nvec);
     else
         hipLaunchKernelGGL((zsticks_forward_tl<T, A, B>), grid, dim3(c.threads), c.lds, st, a, zbuf, psi, ldpsi,
-                           hpsi, ldh, tw, c.spb, c.lz);
+                           hpsi, ldh, tw, c.spb, c.lz, nblk, nvec);
     return hipGetLastError();
 }
 
