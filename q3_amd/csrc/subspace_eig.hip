@@ -72,6 +72,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 // stage swaps half of the live values with the partner lane and adds, halving V, so V sums
 // cost about log2(64) + V - 1 exchanges instead of 6 V.  On return lane l holds the full sum of
 // value l / (64 / V).
+// Orders a wavefront's LDS accesses across lanes (they execute in issue order; this keeps the
+// compiler from moving them across the point).
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int V>
 __device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
     int o = 32;
@@ -568,6 +575,39 @@ __device__ double block_house(const z2* col, int L, int n, int j, z2* vout, doub
     return t;
 }
 
+// Wavefront-level block_house: the calling wavefront alone forms the reflector of
+// x = col[1..L) into vout (length L-1) and returns tau; workgroup 0's wavefront stores beta[j+1],
+// tau[j] and v[1..] into R.  No block barrier.
+__device__ double wave_house(const z2* col, int L, int n, int j, z2* vout, bool writer, z2* beta, double* tau,
+                             z2* R) {
+    const int lane = threadIdx.x & 63;
+    double part = 0.0;
+    for (int i = lane + 2; i < L; i += 64) part += col[i].x * col[i].x + col[i].y * col[i].y;
+    const double tail = wave_sum(part);
+    const z2 x0 = col[1];
+    double t = 0.0;
+    if (tail == 0.0) {
+        if (writer && lane == 0) beta[j + 1] = x0;
+    } else {
+        const double a0 = hypot(x0.x, x0.y);
+        const double xnorm = sqrt(tail + a0 * a0);
+        const z2 ph = a0 == 0.0 ? z2{1.0, 0.0} : z2{x0.x / a0, x0.y / a0};
+        const z2 b = {-ph.x * xnorm, -ph.y * xnorm};
+        const z2 u0 = {x0.x - b.x, x0.y - b.y};
+        const double iu2 = 1.0 / (u0.x * u0.x + u0.y * u0.y);
+        t = 2.0 / (1.0 + tail * iu2);
+        for (int i = lane + 1; i < L; i += 64) {
+            const z2 a = col[i];
+            const z2 w = i == 1 ? z2{1.0, 0.0} : z2{(a.x * u0.x + a.y * u0.y) * iu2, (a.y * u0.x - a.x * u0.y) * iu2};
+            vout[i - 1] = w;
+            if (writer && i >= 2) R[(j + i) + static_cast<long>(j) * n] = w;
+        }
+        if (writer && lane == 0) beta[j + 1] = b;
+    }
+    if (writer && lane == 0) tau[j] = t;
+    return t;
+}
+
 __global__ __launch_bounds__(kThreads) void tridiag_coop(const z2* __restrict__ A, int n, z2* R, double* d,
                                                          z2* beta, double* tau, z2* phase, double* eabs,
                                                          z2* xP, z2* xC, unsigned* bar, int* fail) {
@@ -642,33 +682,39 @@ __global__ __launch_bounds__(kThreads) void tridiag_coop(const z2* __restrict__ 
         }
         nbar += static_cast<unsigned>(W);
         grid_barrier(bar, nbar, fail);
-        for (int i = tid; i < len; i += nt) {
-            p[i] = P[i];
-            colb[i] = Cc[i];
+        // one wavefront: p and the pivot column, f = tn vn^H p / 2, q = p - f vn, the pivot column
+        // with reflector k+1, the next reflector; the other wavefronts wait at the barrier below
+        if (wv == 0) {
+            for (int i = lane; i < len; i += 64) {
+                p[i] = P[i];
+                colb[i] = Cc[i];
+            }
+            double f = 0.0;
+            if (tn != 0.0) {
+                double part = 0.0;
+                for (int i = lane; i < len; i += 64) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
+                f = 0.5 * tn * wave_sum(part);
+                for (int i = lane; i < len; i += 64) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
+                wave_fence();
+                const z2 q0 = q[0], v0 = vn[0];
+                for (int i = lane; i < len; i += 64) {  // pivot column with reflector k+1
+                    const z2 t1 = zmulc(vn[i], q0), t2 = zmulc(q[i], v0);
+                    colb[i].x -= t1.x + t2.x;
+                    colb[i].y -= t1.y + t2.y;
+                }
+            }
+            wave_fence();
+            if (writer && lane == 0) d[base] = colb[0].x;
+            double tj = 0.0;
+            if (base <= n - 3) {
+                tj = wave_house(colb, len, n, base, v, writer, beta, tau, R);
+            } else if (base == n - 2) {
+                if (writer && lane == 0) beta[n - 1] = colb[1];
+            }
+            if (lane == 0) red[0] = tj;
         }
         __syncthreads();
-        double f = 0.0;
-        if (tn != 0.0) {
-            double part = 0.0;
-            for (int i = tid; i < len; i += nt) part += vn[i].x * p[i].x + vn[i].y * p[i].y;
-            f = 0.5 * tn * block_sum(part, red, slot);
-            for (int i = tid; i < len; i += nt) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
-            __syncthreads();
-            const z2 q0 = q[0], v0 = vn[0];
-            for (int i = tid; i < len; i += nt) {  // pivot column with reflector k+1
-                const z2 t1 = zmulc(vn[i], q0), t2 = zmulc(q[i], v0);
-                colb[i].x -= t1.x + t2.x;
-                colb[i].y -= t1.y + t2.y;
-            }
-            __syncthreads();
-        }
-        if (writer && tid == 0) d[base] = colb[0].x;
-        double tj = 0.0;
-        if (base <= n - 3) {
-            tj = block_house(colb, len, n, base, v, red, slot, writer, beta, tau, R);
-        } else if (base == n - 2) {
-            if (writer && tid == 0) beta[n - 1] = colb[1];
-        }
+        const double tj = red[0];
         tk = tn;
         tn = tj;
         z2* t = v;
