@@ -51,6 +51,11 @@ def lib():
         L.orc_sph_bessel.restype = _d
         L.orc_sph_bessel.argtypes = [_i, _d]
         L.orc_ylm.argtypes = [_i, _i, _p, _p]
+        L.orc_lagrange4.restype = _d
+        L.orc_lagrange4.argtypes = [_p, _i, _d, _d]
+        L.orc_simpson.restype = _d
+        L.orc_simpson.argtypes = [_p, _p, _i]
+        L.orc_sph_bessel_batch.argtypes = [_i, _d, _p, _i, _p]
         L.orc_kb_projectors.restype = _i
         L.orc_kb_projectors.argtypes = [_p, _p, _i, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
         L.orc_zhegv.restype = _i
@@ -168,6 +173,26 @@ def beta_table(r, rab, l, cutoff, beta_r, ecutrho):
 
 def sph_bessel(l, x):
     return lib().orc_sph_bessel(l, x)
+
+
+def sph_bessel_batch(l, q, r):
+    r = _c(r, np.float64)
+    out = np.zeros(len(r))
+    lib().orc_sph_bessel_batch(l, q, _ptr(r), len(r), _ptr(out))
+    return out
+
+
+def lagrange4(table, q, dq):
+    """4-point Lagrange interpolation of a uniform table (lagrange.hpp:16-30)."""
+    t = _c(table, np.float64)
+    return lib().orc_lagrange4(_ptr(t), len(t), q, dq)
+
+
+def simpson(f, rab):
+    """Simpson's rule on a radial mesh with derivative rab (simpson.hpp:22-48)."""
+    f = _c(f, np.float64)
+    rab = _c(rab, np.float64)
+    return lib().orc_simpson(_ptr(f), _ptr(rab), len(f))
 
 
 def ylm(lmax, g):

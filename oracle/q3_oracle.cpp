@@ -824,6 +824,16 @@ extern "C" {
 
 int orc_good_fft_order(int n, int factor) { return good_fft_order(n, factor); }
 
+double orc_lagrange4(const double* table, int n, double q, double dq) {
+    return lagrange4(std::vector<double>(table, table + n), q, dq, n);
+}
+
+double orc_simpson(const double* f, const double* rab, int n) {
+    return simpson(std::vector<double>(f, f + n), rab, n);
+}
+
+void orc_sph_bessel_batch(int l, double q, const double* r, int n, double* out) { sph_bessel_batch(l, q, r, n, out); }
+
 void orc_crystal(const double* lat9, double alat, double* recip9, double* volume, double* tpa2) {
     Crystal c;
     c.init(lat9, alat);

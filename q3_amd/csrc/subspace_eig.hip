@@ -518,8 +518,9 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
 // operations in the same order in each workgroup, so all hold identical values and nothing else
 // has to be broadcast.  p and the pivot column are double-buffered by step parity: a workgroup
 // can be at most one step ahead of another.  Workgroup 0 writes the outputs (d, beta, tau and the
-// reflectors into R, n x n, column k rows k+2..).  Launched cooperatively (all workgroups
-// resident); the barrier's wait is bounded and sets *fail instead of waiting forever.
+// reflectors into R, n x n, column k rows k+2..).  At most one workgroup per CU, so all are
+// resident on an otherwise idle device; the barrier's wait is bounded and sets *fail instead of
+// waiting forever.
 
 // Once any workgroup has given up (about a second of polling), every later barrier returns at
 // once, so the kernel always drains quickly; the host turns *fail into an error.
@@ -1150,6 +1151,10 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     };
     // C = Zl Hf Zl, then steps 3-6; one synchronisation at the end, which also brings back
     // max |S - I| (errd, written by herm_prep) into *sdev_out.  0 or -1.
+    // The multi-workgroup tridiagonalisation is an ordinary launch of at most one workgroup per
+    // CU; if another stream's work keeps one of its workgroups from being placed until a grid
+    // barrier gives up, the solve is redone with the one-workgroup kernel (return code -2).
+    bool multi_wg_ok = true;
     auto solve_from_zl = [&](double* sdev_out) -> int {
         // C = Zl Hf Zl
         if (!gemm(0, n, n, n, Hf, n, Zl, n, T1, n) || !gemm(0, n, n, n, Zl, n, T1, n, C, n)) return -1;
@@ -1177,7 +1182,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
         // about 16 columns (one per wavefront) each, fewer when LDS is short
         z2* refl = C;  // where the reflectors end up
         bool coop_done = false;
-        if (!lm && n >= 100) {
+        if (!lm && n >= 100 && multi_wg_ok) {
             const long room = static_cast<long>(max_lds / 16) - 5L * n - 16;
             const int ncl = static_cast<int>(std::min<long>(16, room / n));
             if (ncl >= 1) {
@@ -1200,16 +1205,11 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
                     z2* xp = P(B_XP);
                     z2* xc = P(B_XC);
                     int* fail = reinterpret_cast<int*>(bar + 1);
-                    int nn_ = n;
-                    void* args[] = {&Cc, &nn_, &R, &d, &beta, &tau, &ph, &e, &xp, &xc, &bar, &fail};
-                    const hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&tridiag_coop),
-                                                                     dim3(W), dim3(kThreads), args, lds, st);
-                    if (le == hipSuccess) {
-                        coop_done = true;
-                        refl = T1;
-                    } else {
-                        (void)hipGetLastError();  // not co-resident: the one-workgroup kernel below
-                    }
+                    hipLaunchKernelGGL(tridiag_coop, dim3(W), dim3(kThreads), lds, st, Cc, n, R, d, beta, tau, ph, e,
+                                       xp, xc, bar, fail);
+                    if (!ok(hipGetLastError())) return -1;
+                    coop_done = true;
+                    refl = T1;
                 }
             }
         }
@@ -1266,21 +1266,27 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
             (coop_done && !ok(hipMemcpyAsync(&coop_fail, base + L.off[B_BAR] + 4, 4, hipMemcpyDeviceToHost, st))) ||
             !ok(hipStreamSynchronize(st)))
             return -1;
-        if (coop_fail) {  // a grid barrier gave up waiting
-            *err = hipErrorLaunchTimeOut;
-            return -1;
-        }
+        if (coop_fail) return -2;  // a grid barrier gave up waiting
         return 0;
     };
     double sdev = 0.0, sdev2 = 0.0;
     if (stats) stats->newton_schulz_iterations = 0;
-    int r = solve_from_zl(&sdev);
+    auto solve_once = [&](double* sdev_out) -> int {
+        int rc = solve_from_zl(sdev_out);
+        if (rc == -2) {
+            multi_wg_ok = false;
+            rc = solve_from_zl(sdev_out);
+            multi_wg_ok = true;
+        }
+        return rc;
+    };
+    int r = solve_once(&sdev);
     if (r != 0) return r;
     if (stats) stats->s_deviation = sdev;
     // first-order Loewdin factor error ~ (3/8) |E|_2^2 <= (3/8) (n max|E|)^2
     if (!(sdev * n <= 1e-7)) {
         if ((r = newton_schulz(sdev)) != 0) return r;
-        if ((r = solve_from_zl(&sdev2)) != 0) return r;
+        if ((r = solve_once(&sdev2)) != 0) return r;
     }
     for (int i = 0; i < nvec; ++i)
         if (!(w_host[i] == w_host[i])) return n + 1;
