@@ -16,6 +16,10 @@ hipError_t launch_axpby(int npw, int ncol, const double* a, const double* X, lon
                         long ldy, hipStream_t st);
 hipError_t launch_zero(double* p, long ndouble, hipStream_t st);
 // Y[:, c] = s[c] X[:, src[c]] (s null -> 1), complex columns of length rows
+// R[:, c] = Teter(HV[:, src[c]] - eps[c] V[:, src[c]]) (complex columns, leading dims in complex)
+hipError_t launch_residual_columns_teter(int npw, int ncol, const int* src, const double* eps, const double* ekin,
+                                         const double* V, long ldv, const double* HV, long ldh, double* R, long ldr,
+                                         hipStream_t st);
 hipError_t launch_gather_columns(int rows, int ncol, const int* src, const double* s, const double* X, long ldx,
                                  double* Y, long ldy, hipStream_t st);
 // plain copy of n16 16-byte elements (HBM byte-counter reference)

@@ -66,6 +66,36 @@ hipError_t launch_teter(int npw, int ncol, const double* ekin, const double* eps
     return hipGetLastError();
 }
 
+// Corrections when the Ritz vectors are columns of the basis (right after a collapse the
+// subspace eigenvectors are the identity): R_c = P_c (HV[:, src_c] - e_c V[:, src_c]) with the
+// Teter preconditioner P_c of teter_precondition, in one pass; the two GEMMs against X = I of
+// the general case (serial.hpp:995-1132) reduce to this gather.
+__global__ void residual_columns_teter(int npw, const int* __restrict__ src, const double* __restrict__ eps,
+                                       const double* __restrict__ ekin, const double2* __restrict__ V, long ldv,
+                                       const double2* __restrict__ HV, long ldh, double2* __restrict__ R,
+                                       long ldr) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (g >= npw) return;
+    const long b = src[c];
+    const double e = eps[c];
+    const double2 v = V[g + b * ldv], h = HV[g + b * ldh];
+    const double x = ekin[g] - e;
+    double den = 0.5 * (1.0 + x + sqrt(1.0 + (x - 1.0) * (x - 1.0)));
+    if (den < 1e-4) den = 1e-4;
+    R[g + c * ldr] = double2{(h.x - e * v.x) / den, (h.y - e * v.y) / den};
+}
+
+hipError_t launch_residual_columns_teter(int npw, int ncol, const int* src, const double* eps, const double* ekin,
+                                         const double* V, long ldv, const double* HV, long ldh, double* R, long ldr,
+                                         hipStream_t st) {
+    if (npw == 0 || ncol == 0) return hipSuccess;
+    hipLaunchKernelGGL(residual_columns_teter, dim3((npw + 255) / 256, ncol), dim3(256), 0, st, npw, src, eps, ekin,
+                       reinterpret_cast<const double2*>(V), ldv, reinterpret_cast<const double2*>(HV), ldh,
+                       reinterpret_cast<double2*>(R), ldr);
+    return hipGetLastError();
+}
+
 // Residual columns and their squared norms in one pass:
 //   R_c = HV x_c - e_c V x_c is formed by the caller as two GEMMs; this reduces |R_c|^2
 //   deterministically (one workgroup per column, fixed-order tree).

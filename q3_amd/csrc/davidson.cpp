@@ -340,7 +340,8 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
         for (it = 0; it < max_iter; ++it) {
             ++iterations_run;
             int info = 0;
-            if (collapsed && basis == nb) {
+            const bool x_identity = collapsed && basis == nb;  // Ritz vectors = basis columns
+            if (x_identity) {
                 if (!s.diag_after_collapse(wc, w)) return s.status;
             } else {
                 info = s.diag(basis, w, nb);
@@ -383,21 +384,29 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                 double* corr = s.Vc(basis);
                 double* xc = s.Xc_d.as<double>();
                 if (!s.ok(hipMemcpyAsync(s.idx_d.get(), src.data(), nc * 4, hipMemcpyHostToDevice, s.st), "H2D idx") ||
-                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps") ||
-                    !s.ok(hipMemcpyAsync(s.negeps_d.get(), nec.data(), nc * 8, hipMemcpyHostToDevice, s.st),
-                          "H2D -eps") ||
-                    !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), nullptr, s.X_d.as<double>(), basis, xc,
-                                                basis, s.st),
-                          "gather X") ||
-                    !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), s.negeps_d.as<double>(),
-                                                s.X_d.as<double>(), basis, xc + 2L * basis * nc, basis, s.st),
-                          "gather -e X"))
+                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps"))
                     return s.status;
-                if (!s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xc + 2L * basis * nc, basis, 0.0, corr, npw) ||
-                    !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xc, basis, 1.0, corr, npw) ||
-                    !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), s.eps_d.as<double>(), corr, npw, s.st),
-                          "Teter"))
+                if (x_identity) {
+                    if (!s.ok(launch_residual_columns_teter(npw, nc, s.idx_d.as<int>(), s.eps_d.as<double>(),
+                                                            h->ekin.as<double>(), s.Vc(0), npw, s.HVc(0), npw, corr,
+                                                            npw, s.st),
+                              "corrections"))
+                        return s.status;
+                } else if (!s.ok(hipMemcpyAsync(s.negeps_d.get(), nec.data(), nc * 8, hipMemcpyHostToDevice, s.st),
+                                 "H2D -eps") ||
+                           !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), nullptr, s.X_d.as<double>(),
+                                                       basis, xc, basis, s.st),
+                                 "gather X") ||
+                           !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), s.negeps_d.as<double>(),
+                                                       s.X_d.as<double>(), basis, xc + 2L * basis * nc, basis, s.st),
+                                 "gather -e X") ||
+                           !s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xc + 2L * basis * nc, basis, 0.0, corr,
+                                   npw) ||
+                           !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xc, basis, 1.0, corr, npw) ||
+                           !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), s.eps_d.as<double>(), corr, npw, s.st),
+                                 "Teter")) {
                     return s.status;
+                }
             }
             if (!s.stage_sync()) return s.status;
             s.stage[6] += s.now_ms() - t0;
