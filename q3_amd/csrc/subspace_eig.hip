@@ -819,47 +819,69 @@ __global__ __launch_bounds__(64) void bisect_kernel(int n, const double* __restr
 // a cluster is computed in round r and orthogonalised against the earlier members (the
 // procedure of LAPACK's dstein with a tighter cluster criterion; the looser orthogonality of
 // vectors in different clusters, about eps |T| / gap, is removed by the final Loewdin step).
-__global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __restrict__ d,
-                                                         const double* __restrict__ e, double* w, int nvec,
+// Loads of one thread's own column, kPf at a time ahead of the recurrence that consumes them:
+// the recurrences are serial chains, so without this every step waits for an L2 round trip.
+constexpr int kPf = 16;
+
+constexpr int kInvitThreads = 256;  // one thread per eigenvector; registers for the prefetch buffers
+
+__global__ __launch_bounds__(kInvitThreads) void invit_kernel(int n, const double* __restrict__ dg,
+                                                         const double* __restrict__ eg, double* w, int nvec,
                                                          double* __restrict__ Zt, double* __restrict__ U1,
                                                          double* __restrict__ U2, double* __restrict__ U3,
                                                          double* __restrict__ LO, int* __restrict__ SW) {
+    // d, e of the tridiagonal and the eigenvalues w in LDS: every thread reads them at every
+    // step, and the ordering pass below is a serial chain
+    extern __shared__ double tde[];
+    double* d = tde;
+    double* e = tde + n;
+    double* ws = tde + 2 * n;
     __shared__ double red[32];
     __shared__ double sgl[4];
     int slot = 0;
     const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < n; i += nt) {
+        d[i] = dg[i];
+        e[i] = eg[i];
+    }
+    for (int c = tid; c < nvec; c += nt) ws[c] = w[c];
+    __syncthreads();
     if (tid < 64) {
         double gl, gu, tn, pivmin;
         tri_bounds(n, d, e, gl, gu, tn, pivmin);
         if (tid == 0) sgl[0] = tn;
     }
     if (tid == 0)  // eigenvalues from independent bisections: enforce ascending order
-        for (int c = 1; c < nvec; ++c) w[c] = fmax(w[c], w[c - 1]);
+        for (int c = 1; c < nvec; ++c) ws[c] = fmax(ws[c], ws[c - 1]);
     __syncthreads();
+    for (int c = tid; c < nvec; c += nt) w[c] = ws[c];
     const double tn = sgl[0];
-    const double ctol = 1e-6 * tn;
+    // Levels closer than ctol form a cluster (LU inverse iteration with Gram-Schmidt inside the
+    // cluster).  Above it the twisted vector is off by about eps tn / gap <= 2e-8, and the
+    // back-transformation's Y (1.5 I - 0.5 Y^H Y) step leaves its square, ~5e-16.
+    const double ctol = 1e-8 * tn;
     const double pivmin = fmax(kEps * tn, 1e-300);
     int maxrank = 0;
     for (int c = tid; c < nvec; c += nt) {
         int c0 = c;
-        while (c0 > 0 && w[c0] - w[c0 - 1] <= ctol) --c0;
+        while (c0 > 0 && ws[c0] - ws[c0 - 1] <= ctol) --c0;
         maxrank = max(maxrank, c - c0);
     }
     const int rounds = static_cast<int>(block_max(static_cast<double>(maxrank), red, slot)) + 1;
     for (int r = 0; r < rounds; ++r) {
         for (int c = tid; c < nvec; c += nt) {
             int c0 = c;
-            while (c0 > 0 && w[c0] - w[c0 - 1] <= ctol) --c0;
+            while (c0 > 0 && ws[c0] - ws[c0 - 1] <= ctol) --c0;
             if (c - c0 != r) continue;
             // shifts of a cluster kept apart by at least 10 eps |mu|
-            double mu = w[c0];
+            double mu = ws[c0];
             for (int cc = c0 + 1; cc <= c; ++cc) {
-                double m = w[cc];
+                double m = ws[cc];
                 if (m - mu < 10.0 * kEps * fabs(m)) m = mu + 10.0 * kEps * fmax(fabs(m), tn * kEps);
                 mu = m;
             }
             auto at = [nvec, c](int i) { return static_cast<long>(i) * nvec + c; };
-            const bool isolated = c == c0 && (c + 1 >= nvec || w[c + 1] - w[c] > ctol);
+            const bool isolated = c == c0 && (c + 1 >= nvec || ws[c + 1] - ws[c] > ctol);
             if (isolated && n >= 2) {
                 // Twisted factorisation of Tr - mu I (the core step of the MRRR algorithm,
                 // Dhillon & Parlett, LAPACK's dlar1v): L+ D+ L+^T from the top, U- D- U-^T from
@@ -868,7 +890,6 @@ __global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __
                 // each way instead of repeated LU solves; for an eigenvalue apart from the others
                 // the vector is accurate to about eps |T| / gap.  U1 holds L+, U2 D+, U3 U-.
                 double dp = d[0] - mu;
-#pragma unroll 4
                 for (int i = 0; i + 1 < n; ++i) {
                     if (fabs(dp) < pivmin) dp = dp >= 0.0 ? pivmin : -pivmin;
                     const double l = e[i + 1] / dp;
@@ -881,35 +902,67 @@ __global__ __launch_bounds__(kThreads) void invit_kernel(int n, const double* __
                 double dm = d[n - 1] - mu;
                 double best = fabs(dp + dm - (d[n - 1] - mu));
                 int rt = n - 1;
-#pragma unroll 4
-                for (int i = n - 1; i >= 1; --i) {
-                    if (fabs(dm) < pivmin) dm = dm >= 0.0 ? pivmin : -pivmin;
-                    const double u = e[i] / dm;
-                    U3[at(i)] = u;
-                    dm = (d[i - 1] - mu) - u * e[i];
-                    const double g = fabs(U2[at(i - 1)] + dm - (d[i - 1] - mu));
-                    if (g < best) {
-                        best = g;
-                        rt = i - 1;
+                for (int i0 = n - 1; i0 >= 1; i0 -= kPf) {  // steps i = i0, i0-1, ... >= 1
+                    double dpb[kPf];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k)
+                        if (i0 - k >= 1) dpb[k] = U2[at(i0 - k - 1)];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k) {
+                        const int i = i0 - k;
+                        if (i < 1) break;
+                        if (fabs(dm) < pivmin) dm = dm >= 0.0 ? pivmin : -pivmin;
+                        const double u = e[i] / dm;
+                        U3[at(i)] = u;
+                        dm = (d[i - 1] - mu) - u * e[i];
+                        const double g = fabs(dpb[k] + dm - (d[i - 1] - mu));
+                        if (g < best) {
+                            best = g;
+                            rt = i - 1;
+                        }
                     }
                 }
                 double z = 1.0, nrm = 1.0;
                 Zt[at(rt)] = 1.0;
-#pragma unroll 4
-                for (int i = rt - 1; i >= 0; --i) {
-                    z = -U1[at(i)] * z;
-                    Zt[at(i)] = z;
-                    nrm += z * z;
+                for (int i0 = rt - 1; i0 >= 0; i0 -= kPf) {
+                    double lb[kPf];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k)
+                        if (i0 - k >= 0) lb[k] = U1[at(i0 - k)];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k) {
+                        const int i = i0 - k;
+                        if (i < 0) break;
+                        z = -lb[k] * z;
+                        Zt[at(i)] = z;
+                        nrm += z * z;
+                    }
                 }
                 z = 1.0;
-#pragma unroll 4
-                for (int i = rt + 1; i < n; ++i) {
-                    z = -U3[at(i)] * z;
-                    Zt[at(i)] = z;
-                    nrm += z * z;
+                for (int i0 = rt + 1; i0 < n; i0 += kPf) {
+                    double ub[kPf];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k)
+                        if (i0 + k < n) ub[k] = U3[at(i0 + k)];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k) {
+                        const int i = i0 + k;
+                        if (i >= n) break;
+                        z = -ub[k] * z;
+                        Zt[at(i)] = z;
+                        nrm += z * z;
+                    }
                 }
                 const double inv = 1.0 / sqrt(nrm);
-                for (int i = 0; i < n; ++i) Zt[at(i)] *= inv;
+                for (int i0 = 0; i0 < n; i0 += kPf) {
+                    double zb[kPf];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k)
+                        if (i0 + k < n) zb[k] = Zt[at(i0 + k)];
+#pragma unroll
+                    for (int k = 0; k < kPf; ++k)
+                        if (i0 + k < n) Zt[at(i0 + k)] = zb[k] * inv;
+                }
                 continue;
             }
             // LU with partial pivoting of Tr - mu I: row i keeps (u1 diag, u2, u3 super-diagonals).
@@ -1007,25 +1060,40 @@ __global__ __launch_bounds__(64) void backtransform_kernel(int n, const z2* __re
             y[r] = {0.0, 0.0};
         }
     }
-    for (int k = n - 3; k >= 0; --k) {
-        const double t = tau[k];
-        if (t == 0.0) continue;
-        const z2* col = A + static_cast<long>(k) * n;
-        z2 v[RPL];
-        double sr = 0.0, si = 0.0;
+    // reflectors k = n-3 .. 0, PF at a time: their entries are loaded together, then applied in
+    // order, so one L2 round trip serves PF serial steps
+    constexpr int PF = RPL <= 16 ? 32 / RPL : 2;
+    for (int k0 = n - 3; k0 >= 0; k0 -= PF) {
+        z2 vb[PF][RPL];
+        double tb[PF];
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-            const int i = lane + 64 * r;
-            v[r] = i == k + 1 ? z2{1.0, 0.0} : (i > k + 1 && i < n ? col[i] : z2{0.0, 0.0});
-            sr += v[r].x * y[r].x + v[r].y * y[r].y;
-            si += v[r].x * y[r].y - v[r].y * y[r].x;
+        for (int q = 0; q < PF; ++q) {
+            const int k = k0 - q;
+            tb[q] = k >= 0 ? tau[k] : 0.0;
+            const z2* col = A + static_cast<long>(k >= 0 ? k : 0) * n;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                const int i = lane + 64 * r;
+                vb[q][r] = (k >= 0 && i > k + 1 && i < n) ? col[i] : z2{i == k + 1 ? 1.0 : 0.0, 0.0};
+            }
         }
-        sr = t * wave_sum(sr);
-        si = t * wave_sum(si);
 #pragma unroll
-        for (int r = 0; r < RPL; ++r) {
-            y[r].x -= v[r].x * sr - v[r].y * si;
-            y[r].y -= v[r].x * si + v[r].y * sr;
+        for (int q = 0; q < PF; ++q) {
+            const double t = tb[q];
+            if (t == 0.0) continue;  // also every k < 0 of the last chunk
+            double sr = 0.0, si = 0.0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                sr += vb[q][r].x * y[r].x + vb[q][r].y * y[r].y;
+                si += vb[q][r].x * y[r].y - vb[q][r].y * y[r].x;
+            }
+            sr = t * wave_sum(sr);
+            si = t * wave_sum(si);
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                y[r].x -= vb[q][r].x * sr - vb[q][r].y * si;
+                y[r].y -= vb[q][r].x * si + vb[q][r].y * sr;
+            }
         }
     }
 #pragma unroll
@@ -1232,7 +1300,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
         hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(64), static_cast<size_t>(n) * 16, st, n, d, e, w);
         if (!ok(hipGetLastError())) return -1;
         stage_done(K_EIG_BISECT);
-        hipLaunchKernelGGL(invit_kernel, dim3(1), dim3(kThreads), 0, st, n, d, e, w, nvec, D(B_ZT), D(B_U1), D(B_U2),
+        hipLaunchKernelGGL(invit_kernel, dim3(1), dim3(kInvitThreads), static_cast<size_t>(2 * n + nvec) * 8, st, n, d, e, w, nvec, D(B_ZT), D(B_U1), D(B_U2),
                            D(B_U3), D(B_LO), reinterpret_cast<int*>(base + L.off[B_SW]));
         if (!ok(hipGetLastError())) return -1;
         stage_done(K_EIG_INVIT);
