@@ -264,7 +264,8 @@ def main():
                                    "DESIGN.md and not launched", "n_gpus": args.gpus}))
         return 2
     # one hardware queue per batch worker stream (HIP's default is 4), set before HIP starts
-    os.environ["GPU_MAX_HW_QUEUES"] = str(max(4, min(16, args.batch_workers)))
+    # unless the caller chose a value
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(16, args.batch_workers))))
     if args.mode == "copy":
         return copy_reference(args.steps)
     if args.mode == "batch":
