@@ -53,6 +53,51 @@ bool DevBuf::upload(const void* src, size_t bytes, hipStream_t st) {
            hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
+HostStage::~HostStage() {
+    if (p_) (void)hipHostFree(p_);
+}
+
+char* HostStage::take(size_t bytes, hipStream_t st) {
+    const size_t n = (std::max<size_t>(bytes, 1) + 255) & ~static_cast<size_t>(255);
+    if (off_ + n > cap_) {
+        if (hipStreamSynchronize(st) != hipSuccess) return nullptr;
+        off_ = 0;
+        if (n > cap_) {
+            const size_t want = std::max<size_t>(n, std::max<size_t>(2 * cap_, size_t(4) << 20));
+            if (p_) (void)hipHostFree(p_);
+            p_ = nullptr;
+            cap_ = 0;
+            void* q = nullptr;
+            if (hipHostMalloc(&q, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+            p_ = static_cast<char*>(q);
+            cap_ = want;
+        }
+    }
+    char* r = p_ + off_;
+    off_ += n;
+    return r;
+}
+
+hipError_t HostStage::h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return hipSuccess;
+    char* s = take(bytes, st);
+    if (!s) return hipErrorOutOfMemory;
+    std::memcpy(s, src, bytes);
+    return hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, st);
+}
+
+hipError_t HostStage::d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return hipSuccess;
+    char* s = take(bytes, st);
+    if (!s) return hipErrorOutOfMemory;
+    hipError_t e = hipMemcpyAsync(s, src, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    std::memcpy(dst, s, bytes);
+    off_ = 0;
+    return hipSuccess;
+}
+
 int Profiler::mark(hipStream_t st) {
     if (!on) return -1;
     if (used == pool.size()) {
@@ -424,7 +469,8 @@ q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, cons
     hipError_t e = hipSuccess;
     const int info = subspace_eig(n, static_cast<const double*>(H), ldh, static_cast<const double*>(S), lds, nvec, w,
                                   static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
-                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr);
+                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr,
+                                  nullptr, &ctx->pin);
     if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
     if (info > 0) {
         set_error("q3x_hegv_device: S is not positive definite");

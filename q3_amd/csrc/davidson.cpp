@@ -75,6 +75,7 @@ struct Solver {
         return false;
     }
     bool sync() { return ok(hipStreamSynchronize(st), "hipStreamSynchronize"); }
+    HostStage& pin() { return h->ctx->pin; }
     double now_ms() { return std::chrono::duration<double, std::milli>(clk::now().time_since_epoch()).count(); }
 
     // C (m x n) = alpha * op(A) * B + beta * C, fp64 everywhere
@@ -122,7 +123,7 @@ struct Solver {
         hipError_t e = hipSuccess;
         const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
                                       X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
-                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr, &h->ctx->prof);
+                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr, &h->ctx->prof, &pin());
         if (info < 0) {
             ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
             return -1;
@@ -143,13 +144,11 @@ struct Solver {
         }
         if (!gemm(1, nc, nc, npw, 1.0, cand, npw, cand, npw, 0.0, G.get(), nc)) return -1;
         std::vector<zc> Gh(static_cast<size_t>(nc) * nc);
-        if (!ok(hipMemcpyAsync(Gh.data(), G.get(), Gh.size() * 16, hipMemcpyDeviceToHost, st), "D2H Gram") || !sync())
-            return -1;
+        if (!ok(pin().d2h(Gh.data(), G.get(), Gh.size() * 16, st), "D2H Gram")) return -1;
         int accepted = nc;
         if (chol_upper(nc, Gh.data(), nc) == 0) {
             tri_inv_upper(nc, Gh.data(), nc);  // cand <- cand U^-1 (the ztrsm R,U,N,N of serial.hpp:337-348)
-            if (!ok(hipMemcpyAsync(G.get(), Gh.data(), Gh.size() * 16, hipMemcpyHostToDevice, st), "H2D U^-1"))
-                return -1;
+            if (!ok(pin().h2d(G.get(), Gh.data(), Gh.size() * 16, st), "H2D U^-1")) return -1;
             if (!gemm(0, npw, nc, nc, 1.0, cand, npw, G.get(), nc, 0.0, R.get(), npw)) return -1;
             if (!ok(hipMemcpyAsync(cand, R.get(), static_cast<size_t>(npw) * nc * 16, hipMemcpyDeviceToDevice, st),
                     "D2D orthonormal block"))
@@ -170,11 +169,11 @@ struct Solver {
                     if (!gemm(0, npw, 1, nex, -1.0, cand, npw, C.get(), nex, 1.0, Vc(dst), npw)) return -1;
                 }
                 if (!ok(launch_column_norm2(npw, 1, Vc(dst), npw, norms_d.as<double>(), st), "norm") ||
-                    !ok(hipMemcpyAsync(n2.data(), norms_d.get(), 8, hipMemcpyDeviceToHost, st), "D2H norm") || !sync())
+                    !ok(pin().d2h(n2.data(), norms_d.get(), 8, st), "D2H norm"))
                     return -1;
                 if (n2[0] > kNormThreshold) {
                     const double inv = 1.0 / std::sqrt(n2[0]);
-                    if (!ok(hipMemcpyAsync(eps_d.get(), &inv, 8, hipMemcpyHostToDevice, st), "H2D scale") ||
+                    if (!ok(pin().h2d(eps_d.get(), &inv, 8, st), "H2D scale") ||
                         !ok(launch_axpby(npw, 1, eps_d.as<double>(), Vc(dst), npw, 0.0, Vc(dst), npw, st), "scale") ||
                         !sync())
                         return -1;
@@ -308,7 +307,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
         std::vector<int> idx(nc);
         for (int c = 0; c < nc; ++c) idx[c] = ke[start + c].second;
         if (!s.ok(launch_zero(s.Vc(start), 2L * npw * nc, s.st), "zero seed") ||
-            !s.ok(hipMemcpyAsync(s.idx_d.get(), idx.data(), nc * 4, hipMemcpyHostToDevice, s.st), "H2D seed idx") ||
+            !s.ok(s.pin().h2d(s.idx_d.get(), idx.data(), nc * 4, s.st), "H2D seed idx") ||
             !s.ok(launch_unit_columns(s.idx_d.as<int>(), nc, s.Vc(start), npw, s.st), "unit columns"))
             return s.status;
         const int acc = s.orthogonalize(start, nc);
@@ -320,11 +319,18 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     std::vector<double> prev(nb, 0.0), w;
     {  // starting values: the Rayleigh quotients on the diagonal of H_sub
         std::vector<zc> diag(basis);
-        if (!s.ok(hipMemcpy2DAsync(diag.data(), 16, s.Hs_d.get(), static_cast<size_t>(mmax + 1) * 16, 16, basis,
+        char* dst = s.pin().take(static_cast<size_t>(basis) * 16, s.st);
+        if (!dst) {
+            s.ok(hipErrorOutOfMemory, "pinned staging");
+            return s.status;
+        }
+        if (!s.ok(hipMemcpy2DAsync(dst, 16, s.Hs_d.get(), static_cast<size_t>(mmax + 1) * 16, 16, basis,
                                    hipMemcpyDeviceToHost, s.st),
                   "D2H diagonal") ||
             !s.sync())
             return s.status;
+        std::memcpy(diag.data(), dst, static_cast<size_t>(basis) * 16);
+        s.pin().rewind();
         for (int b = 0; b < nb && b < basis; ++b) prev[b] = diag[b].real();
     }
     s.stage[0] += s.now_ms() - t0;
@@ -383,8 +389,8 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                 }
                 double* corr = s.Vc(basis);
                 double* xc = s.Xc_d.as<double>();
-                if (!s.ok(hipMemcpyAsync(s.idx_d.get(), src.data(), nc * 4, hipMemcpyHostToDevice, s.st), "H2D idx") ||
-                    !s.ok(hipMemcpyAsync(s.eps_d.get(), ec.data(), nc * 8, hipMemcpyHostToDevice, s.st), "H2D eps"))
+                if (!s.ok(s.pin().h2d(s.idx_d.get(), src.data(), nc * 4, s.st), "H2D idx") ||
+                    !s.ok(s.pin().h2d(s.eps_d.get(), ec.data(), nc * 8, s.st), "H2D eps"))
                     return s.status;
                 if (x_identity) {
                     if (!s.ok(launch_residual_columns_teter(npw, nc, s.idx_d.as<int>(), s.eps_d.as<double>(),
@@ -392,8 +398,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                                                             npw, s.st),
                               "corrections"))
                         return s.status;
-                } else if (!s.ok(hipMemcpyAsync(s.negeps_d.get(), nec.data(), nc * 8, hipMemcpyHostToDevice, s.st),
-                                 "H2D -eps") ||
+                } else if (!s.ok(s.pin().h2d(s.negeps_d.get(), nec.data(), nc * 8, s.st), "H2D -eps") ||
                            !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), nullptr, s.X_d.as<double>(),
                                                        basis, xc, basis, s.st),
                                  "gather X") ||
@@ -454,12 +459,11 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
         std::vector<double> e(nb), n2(nb);
         for (int b = 0; b < nb; ++b) e[b] = -w[b];
         double* res = s.R.as<double>();
-        if (!s.ok(hipMemcpyAsync(s.eps_d.get(), e.data(), nb * 8, hipMemcpyHostToDevice, s.st), "H2D e") ||
+        if (!s.ok(s.pin().h2d(s.eps_d.get(), e.data(), nb * 8, s.st), "H2D e") ||
             !s.ok(launch_axpby(npw, nb, s.eps_d.as<double>(), s.Vc(0), npw, 0.0, res, npw, s.st), "r = -e psi") ||
             !s.ok(launch_axpby(npw, nb, nullptr, s.HVc(0), npw, 1.0, res, npw, s.st), "r += H psi") ||
             !s.ok(launch_column_norm2(npw, nb, res, npw, s.norms_d.as<double>(), s.st), "norms") ||
-            !s.ok(hipMemcpyAsync(n2.data(), s.norms_d.get(), nb * 8, hipMemcpyDeviceToHost, s.st), "D2H norms") ||
-            !s.sync())
+            !s.ok(s.pin().d2h(n2.data(), s.norms_d.get(), nb * 8, s.st), "D2H norms"))
             return s.status;
         for (int b = 0; b < nb; ++b) maxres = std::max(maxres, std::sqrt(n2[b]));
     }

@@ -36,6 +36,28 @@ class DevBuf {
     size_t bytes_ = 0;
 };
 
+// Pinned host memory for the small copies of a solve (eigenvalues, Gram blocks, index lists).
+// A copy between device memory and pageable host memory goes through the runtime's shared
+// staging buffer and holds it until its stream reaches the copy, which serialises the solves of
+// other host threads; from pinned memory it is a plain DMA on the stream.  Regions are handed
+// out as a ring: take() synchronises the stream and rewinds when the ring is full (so no region
+// is reused while a copy from it may be pending), and d2h() rewinds after its synchronisation.
+class HostStage {
+  public:
+    HostStage() = default;
+    HostStage(const HostStage&) = delete;
+    HostStage& operator=(const HostStage&) = delete;
+    ~HostStage();
+    char* take(size_t bytes, hipStream_t st);  // nullptr on failure
+    void rewind() { off_ = 0; }                 // only right after a synchronisation of the stream
+    hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t st);  // asynchronous
+    hipError_t d2h(void* dst, const void* src, size_t bytes, hipStream_t st);  // waits for the stream
+
+  private:
+    char* p_ = nullptr;
+    size_t cap_ = 0, off_ = 0;
+};
+
 // Per-kernel-family timing with HIP events recorded on the context stream around each launch.
 enum KernelId {
     K_ZINV = 0, K_PLANE, K_ZFWD, K_KB_OVERLAP, K_KB_DBLOCK, K_KB_UPDATE,
@@ -80,6 +102,7 @@ struct q3x_context_s {
     q3x::DevBuf gemm_work;  // split-K workspace of q3x_zgemm
     q3x::DevBuf eig_work;   // workspace of q3x_hegv_device
     q3x::DavidsonWorkspace dav;
+    q3x::HostStage pin;  // pinned staging of the small host copies
     int device = 0;
     hipStream_t stream = nullptr;
     size_t max_lds = 65536;

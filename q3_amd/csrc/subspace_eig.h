@@ -24,9 +24,11 @@ struct SubspaceEigStats {
 // S-orthonormal eigenvectors; returns 0.  Returns n + 1 when S is not numerically positive
 // definite (the zhegv info > n case) and -1 on a device error (*err set).
 // work: subspace_eig_workspace(n, nvec) bytes; gemm_work: split-K workspace of zgemm.
-// num_cus: compute units of the device (bounds the workgroups of the cooperative reduction).
+// num_cus: compute units of the device (bounds the workgroups of the multi-workgroup reduction).
+// pin (optional): pinned staging for the copies of eigenvalues and status words to the host.
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof = nullptr);
+                 int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof = nullptr,
+                 HostStage* pin = nullptr);
 
 }  // namespace q3x

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstring>
 
 #include <algorithm>
 #include <cfloat>
@@ -1146,7 +1147,7 @@ size_t subspace_eig_workspace(int n, int nvec) { return layout(std::max(n, 1), s
 
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof) {
+                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof, HostStage* pin) {
     // stage timing on the caller's profiler (events on this stream; nothing when it is off)
     int mark_prev = prof ? prof->mark(st) : -1;
     auto stage_done = [&](int id) {
@@ -1201,9 +1202,12 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
             if (!gemm(0, n, n, n, Z, n, Y, n, ZY, n)) return -1;
             hipLaunchKernelGGL(ns_residual, dim3(1), dim3(kThreads), 0, st, n, ZY, Tm, errd);
             double r = 0.0;
-            if (!ok(hipGetLastError()) || !ok(hipMemcpyAsync(&r, errd, 8, hipMemcpyDeviceToHost, st)) ||
-                !ok(hipStreamSynchronize(st)))
+            if (!ok(hipGetLastError())) return -1;
+            if (pin) {
+                if (!ok(pin->d2h(&r, errd, 8, st))) return -1;
+            } else if (!ok(hipMemcpyAsync(&r, errd, 8, hipMemcpyDeviceToHost, st)) || !ok(hipStreamSynchronize(st))) {
                 return -1;
+            }
             if (!(r < 1e3)) return n + 1;  // diverging: S is not positive definite
             conv = r < 1e-8;                // the update below squares the error
             if (!gemm(0, n, n, n, Y, n, Tm, n, Yn, n) || !gemm(0, n, n, n, Tm, n, Z, n, Z2, n)) return -1;
@@ -1329,11 +1333,26 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
         if (!gemm(0, n, nvec, nvec, Yb, n, G, nvec, Y2, n) || !gemm(0, n, nvec, n, Zl, n, Y2, n, X, ldx)) return -1;
         stage_done(K_EIG_BACK);
         int coop_fail = 0;
-        if (!ok(hipMemcpyAsync(w_host, w, static_cast<size_t>(nvec) * 8, hipMemcpyDeviceToHost, st)) ||
-            !ok(hipMemcpyAsync(sdev_out, errd, 8, hipMemcpyDeviceToHost, st)) ||
-            (coop_done && !ok(hipMemcpyAsync(&coop_fail, base + L.off[B_BAR] + 4, 4, hipMemcpyDeviceToHost, st))) ||
+        const size_t wb = static_cast<size_t>(nvec) * 8;
+        char* sw = pin ? pin->take(wb + 16, st) : nullptr;
+        if (pin && !sw) {
+            *err = hipErrorOutOfMemory;
+            return -1;
+        }
+        double* w_dst = sw ? reinterpret_cast<double*>(sw) : w_host;
+        double* sdev_dst = sw ? reinterpret_cast<double*>(sw + wb) : sdev_out;
+        int* fail_dst = sw ? reinterpret_cast<int*>(sw + wb + 8) : &coop_fail;
+        if (!ok(hipMemcpyAsync(w_dst, w, wb, hipMemcpyDeviceToHost, st)) ||
+            !ok(hipMemcpyAsync(sdev_dst, errd, 8, hipMemcpyDeviceToHost, st)) ||
+            (coop_done && !ok(hipMemcpyAsync(fail_dst, base + L.off[B_BAR] + 4, 4, hipMemcpyDeviceToHost, st))) ||
             !ok(hipStreamSynchronize(st)))
             return -1;
+        if (sw) {
+            std::memcpy(w_host, sw, wb);
+            std::memcpy(sdev_out, sw + wb, 8);
+            if (coop_done) std::memcpy(&coop_fail, sw + wb + 8, 4);
+            pin->rewind();
+        }
         if (coop_fail) return -2;  // a grid barrier gave up waiting
         return 0;
     };
