@@ -46,9 +46,10 @@ void DevBuf::release() {
     bytes_ = 0;
 }
 
-bool DevBuf::upload(const void* src, size_t bytes, hipStream_t st) {
+bool DevBuf::upload(const void* src, size_t bytes, hipStream_t st, HostStage* pin) {
     if (!ensure(bytes)) return false;
     if (bytes == 0) return true;
+    if (pin && bytes <= (size_t(16) << 20)) return hip_ok(pin->h2d(p_, src, bytes, st), "pinned H2D");
     return hip_ok(hipMemcpyAsync(p_, src, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D") &&
            hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
@@ -160,7 +161,7 @@ FftPlan make_plan(int n) {
     return p;
 }
 
-bool build_tables(FftTables& t, int n, hipStream_t st) {
+bool build_tables(FftTables& t, int n, hipStream_t st, HostStage* pin) {
     t.plan = make_plan(n);
     if (t.plan.nstage < 0) {
         set_error("FFT length is not 2,3,5-smooth");
@@ -194,10 +195,10 @@ bool build_tables(FftTables& t, int n, hipStream_t st) {
         pf[2 * j] = static_cast<float>(pd[2 * j]);
         pf[2 * j + 1] = static_cast<float>(pd[2 * j + 1]);
     }
-    return t.tw64.upload(d.data(), d.size() * sizeof(double), st) &&
-           t.tw32.upload(f.data(), f.size() * sizeof(float), st) &&
-           t.plain64.upload(pd.data(), pd.size() * sizeof(double), st) &&
-           t.plain32.upload(pf.data(), pf.size() * sizeof(float), st);
+    return t.tw64.upload(d.data(), d.size() * sizeof(double), st, pin) &&
+           t.tw32.upload(f.data(), f.size() * sizeof(float), st, pin) &&
+           t.plain64.upload(pd.data(), pd.size() * sizeof(double), st, pin) &&
+           t.plain32.upload(pf.data(), pf.size() * sizeof(float), st, pin);
 }
 
 HLocalArgs make_args(const q3x_hamiltonian_s* h, bool fp32) {
@@ -345,7 +346,7 @@ q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b) {
     h->have_projectors = false;
     if (!same_grid)
         for (int d = 0; d < 3; ++d)
-            if (!build_tables(h->fft[d], h->dims[d], h->ctx->stream)) return Q3X_ERR_DEVICE;
+            if (!build_tables(h->fft[d], h->dims[d], h->ctx->stream, &h->ctx->pin)) return Q3X_ERR_DEVICE;
     return Q3X_OK;
 }
 }  // namespace q3x
@@ -557,8 +558,8 @@ q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v, lon
                 d[o] = s;
                 f[o] = static_cast<float>(s);
             }
-    if (!h->v64.upload(d.data(), d.size() * sizeof(double), h->ctx->stream) ||
-        !h->v32.upload(f.data(), f.size() * sizeof(float), h->ctx->stream))
+    if (!h->v64.upload(d.data(), d.size() * sizeof(double), h->ctx->stream, &h->ctx->pin) ||
+        !h->v32.upload(f.data(), f.size() * sizeof(float), h->ctx->stream, &h->ctx->pin))
         return Q3X_ERR_DEVICE;
     h->have_potential = true;
     return Q3X_OK;
@@ -607,14 +608,14 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
         }
         (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(std::max<long>(rp, 1));
     }
-    bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st) &&
-              h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st) &&
-              h->stick_col.upload(S.stick_col.data(), S.stick_col.size() * sizeof(int), st) &&
-              h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st) &&
-              h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st) &&
-              h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st) &&
-              h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st) &&
-              h->outside.upload(out.data(), out.size() * sizeof(int), st);
+    bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->stick_col.upload(S.stick_col.data(), S.stick_col.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st, &h->ctx->pin) &&
+              h->outside.upload(out.data(), out.size() * sizeof(int), st, &h->ctx->pin);
     if (!ok) return Q3X_ERR_DEVICE;
     h->kp = k;
     h->have_projectors = false;
@@ -662,11 +663,11 @@ q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta,
         off += static_cast<long>(bs) * bs;
     }
     if (D.empty()) D.push_back(0.0);
-    bool ok = h->beta64.upload(beta, n * 16, st) && h->beta32.upload(b32.data(), n * 8, st) &&
-              h->dblocks.upload(D.data(), D.size() * sizeof(double), st) &&
-              h->pstart.upload(ps.data(), ps.size() * sizeof(int), st) &&
-              h->psize.upload(pz.data(), pz.size() * sizeof(int), st) &&
-              h->poff.upload(po.data(), po.size() * sizeof(long), st);
+    bool ok = h->beta64.upload(beta, n * 16, st, &h->ctx->pin) && h->beta32.upload(b32.data(), n * 8, st, &h->ctx->pin) &&
+              h->dblocks.upload(D.data(), D.size() * sizeof(double), st, &h->ctx->pin) &&
+              h->pstart.upload(ps.data(), ps.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->psize.upload(pz.data(), pz.size() * sizeof(int), st, &h->ctx->pin) &&
+              h->poff.upload(po.data(), po.size() * sizeof(long), st, &h->ctx->pin);
     if (!ok) return Q3X_ERR_DEVICE;
     h->nproj = nproj;
     h->have_projectors = true;

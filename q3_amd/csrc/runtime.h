@@ -29,7 +29,9 @@ class DevBuf {
     }
     void* get() const { return p_; }
     size_t size() const { return bytes_; }
-    bool upload(const void* src, size_t bytes, hipStream_t st);
+    // Copies bytes from host memory in, growing the buffer.  With pin (and up to 16 MiB) the
+    // copy goes through that pinned staging and returns without waiting; otherwise it waits.
+    bool upload(const void* src, size_t bytes, hipStream_t st, class HostStage* pin = nullptr);
 
   private:
     void* p_ = nullptr;

@@ -5,8 +5,11 @@
 // result.  Requests are admitted in arrival order while their estimated HBM footprint fits the
 // budget (the reference's PlanewaveMemoryEstimate::compute terms,
 // iterator/kohn_sham/planewave/internal/memory_estimate.hpp:75-215); an admitted request's
-// k-points join one shared task queue, and W worker threads -- each with its own context,
-// stream and Davidson workspace -- take k-point tasks from it in order.  A request that arrives
+// k-points join one shared task queue.  P preparation threads take k-point tasks from it in
+// order and do their host work (the k+G list, the KB projectors, the seeded starting block);
+// W worker threads -- each with its own context, stream and Davidson workspace -- take prepared
+// k-points, upload them and solve, so the host setup of the next k-points overlaps the device
+// work of the current ones.  A request that arrives
 // while others are running is admitted as soon as the budget allows and its k-points interleave
 // with theirs on the device; when its last k-point finishes, its footprint is released and the
 // next waiting requests are admitted.
@@ -55,6 +58,18 @@ struct Request {
 
 double now_s(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
+// The host part of a k-point task, done ahead of the device part by a preparation thread.
+struct Prepared {
+    std::shared_ptr<Request> r;
+    int k = -1;
+    q3x_kpoint kp = nullptr;
+    int npw = 0, nproj = 0;
+    std::vector<double> beta, p0;  // projectors npw x nproj (column-major), starting block npw x nb
+    q3x_status status = Q3X_OK;
+    std::string err;
+    double host_s = 0.0;
+};
+
 }  // namespace
 
 struct q3x_server_s {
@@ -62,9 +77,12 @@ struct q3x_server_s {
     double budget = 0.0, in_flight = 0.0;
     clk::time_point t0;
     std::vector<q3x_context> workers;
-    std::vector<std::thread> threads;
+    std::vector<std::thread> threads, prep_threads;
     std::mutex mu;
-    std::condition_variable cv_task, cv_done;
+    std::condition_variable cv_task, cv_done, cv_ready;
+    std::deque<std::unique_ptr<Prepared>> ready;  // prepared k-points waiting for a worker
+    int preparing = 0;                           // tasks taken by preparation threads, not yet ready
+    size_t max_ready = 16;                       // bound on prepared-but-unsolved k-points
     std::deque<std::shared_ptr<Request>> waiting;   // submitted, not admitted
     std::deque<std::shared_ptr<Request>> running;   // admitted, tasks left to hand out
     std::map<long, std::shared_ptr<Request>> all;
@@ -105,31 +123,90 @@ struct q3x_server_s {
         cv_task.notify_all();
     }
 
-    void worker(int w) {
-        q3x_context ctx = workers[w];
-        if (hipSetDevice(device) != hipSuccess) return;
+    // preparation thread: next task in order -> host work -> ready queue
+    void preparer() {
         for (;;) {
             std::shared_ptr<Request> r;
             int k = -1;
             {
                 std::unique_lock<std::mutex> g(mu);
-                cv_task.wait(g, [&] { return stop || !running.empty(); });
-                if (stop && running.empty()) return;
+                cv_task.wait(g, [&] {
+                    return stop || (!running.empty() && ready.size() + static_cast<size_t>(preparing) < max_ready);
+                });
+                if (stop) return;
                 r = running.front();
                 k = r->next_task++;
                 if (r->next_task >= r->nk) running.pop_front();
+                ++preparing;
             }
-            std::string err;
-            const q3x_status s = solve_kpoint(w, ctx, *r, k, err);
+            std::unique_ptr<Prepared> p = prepare(r, k);
             std::lock_guard<std::mutex> g(mu);
-            if (s != Q3X_OK && r->status == Q3X_OK) {
-                r->status = s;
-                r->error = "k-point " + std::to_string(k) + ": " + err;
+            --preparing;
+            ready.push_back(std::move(p));
+            cv_ready.notify_one();
+        }
+    }
+
+    std::unique_ptr<Prepared> prepare(const std::shared_ptr<Request>& r, int k) {
+        const clk::time_point ts = clk::now();
+        auto p = std::make_unique<Prepared>();
+        p->r = r;
+        p->k = k;
+        auto fail = [&](q3x_status st) {
+            p->status = st;
+            p->err = q3x_last_error();
+            return std::move(p);
+        };
+        q3x_status s = q3x_kpoint_create(r->basis, &r->kpts[3 * k], &p->kp, &p->npw);
+        if (s != Q3X_OK) return fail(s);
+        auto kb = [&](double* out) {
+            return q3x_kb_projectors(r->basis, p->kp, r->natoms, r->atom_pos.data(), r->atom_species.data(),
+                                     r->nspecies, r->nmesh.data(), r->r_all.data(), r->rab_all.data(),
+                                     r->nbeta.data(), r->l_all.data(), r->cutoff_all.data(), r->beta_r_all.data(),
+                                     out, &p->nproj);
+        };
+        if ((s = kb(nullptr)) != Q3X_OK) return fail(s);
+        // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
+        // column-major layout the Hamiltonian takes
+        p->beta.resize(static_cast<size_t>(p->npw) * std::max(p->nproj, 1) * 2);
+        if ((s = kb(p->beta.data())) != Q3X_OK) return fail(s);
+        // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, so a
+        // request's result does not depend on what else runs (plane-wave unit vectors, the
+        // solver's own seed, can miss symmetry sectors of these undisplaced crystals)
+        p->p0.resize(static_cast<size_t>(p->npw) * r->nb * 2);
+        if ((s = q3x_kpoint_random_psi(p->kp, r->nb, static_cast<unsigned long long>(k), p->p0.data())) != Q3X_OK)
+            return fail(s);
+        p->host_s = std::chrono::duration<double>(clk::now() - ts).count();
+        return p;
+    }
+
+    void worker(int w) {
+        q3x_context ctx = workers[w];
+        if (hipSetDevice(device) != hipSuccess) return;
+        for (;;) {
+            std::unique_ptr<Prepared> p;
+            {
+                std::unique_lock<std::mutex> g(mu);
+                cv_ready.wait(g, [&] { return stop || !ready.empty(); });
+                if (ready.empty()) return;  // stop, and nothing prepared is left
+                p = std::move(ready.front());
+                ready.pop_front();
+                cv_task.notify_all();  // room in the ready queue
             }
-            if (--r->remaining == 0) {
-                r->done = true;
-                r->t_done = now_s(t0);
-                in_flight -= r->bytes;
+            Request& r = *p->r;
+            std::string err;
+            const q3x_status s = p->status != Q3X_OK ? p->status : solve_kpoint(w, ctx, *p, err);
+            if (p->status != Q3X_OK) err = p->err;
+            if (p->kp) q3x_kpoint_destroy(p->kp);
+            std::lock_guard<std::mutex> g(mu);
+            if (s != Q3X_OK && r.status == Q3X_OK) {
+                r.status = s;
+                r.error = "k-point " + std::to_string(p->k) + ": " + err;
+            }
+            if (--r.remaining == 0) {
+                r.done = true;
+                r.t_done = now_s(t0);
+                in_flight -= r.bytes;
                 if (in_flight < 1.0) in_flight = 0.0;
                 admit_locked();
                 cv_done.notify_all();
@@ -137,28 +214,17 @@ struct q3x_server_s {
         }
     }
 
-    q3x_status solve_kpoint(int w, q3x_context ctx, Request& r, int k, std::string& err) {
+    // device part of a k-point task: bind the worker's Hamiltonian, upload, solve
+    q3x_status solve_kpoint(int w, q3x_context ctx, Prepared& p, std::string& err) {
         const clk::time_point ts = clk::now();
-        q3x_kpoint kp = nullptr;
+        Request& r = *p.r;
+        const int k = p.k, npw = p.npw, nproj = p.nproj;
         q3x_hamiltonian& h = hams[w];
-        int npw = 0, nproj = 0;
-        q3x_status s = q3x_kpoint_create(r.basis, &r.kpts[3 * k], &kp, &npw);
         auto fail = [&](q3x_status st) {
             err = q3x_last_error();
-            if (kp) q3x_kpoint_destroy(kp);
             return st;
         };
-        if (s != Q3X_OK) return fail(s);
-        auto kb = [&](double* out) {
-            return q3x_kb_projectors(r.basis, kp, r.natoms, r.atom_pos.data(), r.atom_species.data(), r.nspecies,
-                                     r.nmesh.data(), r.r_all.data(), r.rab_all.data(), r.nbeta.data(), r.l_all.data(),
-                                     r.cutoff_all.data(), r.beta_r_all.data(), out, &nproj);
-        };
-        if ((s = kb(nullptr)) != Q3X_OK) return fail(s);
-        std::vector<double> beta(static_cast<size_t>(npw) * std::max(nproj, 1) * 2);
-        if ((s = kb(beta.data())) != Q3X_OK) return fail(s);
-        // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
-        // column-major layout the Hamiltonian takes
+        q3x_status s = Q3X_OK;
         // a worker's next k-point usually belongs to the same request: cell, tables and V stay
         const bool same_cell = h && ham_ticket[w] == r.ticket;
         if (!same_cell) {
@@ -168,8 +234,8 @@ struct q3x_server_s {
                 return fail(s);
             ham_ticket[w] = r.ticket;
         }
-        if ((s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
-            (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, beta.data(), nproj, r.natoms, r.block_sizes.data(),
+        if ((s = q3x_hamiltonian_prepare_kpoint(h, p.kp)) != Q3X_OK ||
+            (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, p.beta.data(), nproj, r.natoms, r.block_sizes.data(),
                                                                r.d_blocks.data())) != Q3X_OK))
             return fail(s);
         const size_t psi_bytes = static_cast<size_t>(npw) * r.nb * 16;
@@ -180,15 +246,7 @@ struct q3x_server_s {
             r.psi[k] = {ctx, psi};
             r.psi_bytes[k] = psi_bytes;
         }
-        // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, so a
-        // request's result does not depend on what else runs (plane-wave unit vectors, the
-        // solver's own seed, can miss symmetry sectors of these undisplaced crystals)
-        {
-            std::vector<double> p0(static_cast<size_t>(npw) * r.nb * 2);
-            if ((s = q3x_kpoint_random_psi(kp, r.nb, static_cast<unsigned long long>(k), p0.data())) != Q3X_OK ||
-                (s = q3x_copy_to_device(ctx, psi, p0.data(), p0.size() * sizeof(double))) != Q3X_OK)
-                return fail(s);
-        }
+        if ((s = q3x_copy_to_device(ctx, psi, p.p0.data(), p.p0.size() * sizeof(double))) != Q3X_OK) return fail(s);
         q3x_davidson_options o = r.opt;
         o.seed_from_psi = 1;
         q3x_davidson_report rep{};
@@ -199,10 +257,9 @@ struct q3x_server_s {
         {
             std::lock_guard<std::mutex> g(mu);
             r.iterations += rep.iterations;
-            r.setup_s += std::chrono::duration<double>(tsolve - ts).count();
+            r.setup_s += p.host_s + std::chrono::duration<double>(tsolve - ts).count();
             r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
         }
-        q3x_kpoint_destroy(kp);
         return Q3X_OK;
     }
 };
@@ -257,7 +314,12 @@ q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q
     s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
     s->hams.assign(workers, nullptr);
     s->ham_ticket.assign(workers, 0);
+    // host preparation runs ahead on half as many threads as workers (its share of a k-point's
+    // time is about a third of the device part at the cfg5 cell sizes)
+    const int nprep = std::max(2, (workers + 1) / 2);
+    s->max_ready = static_cast<size_t>(2 * workers);
     for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
+    for (int i = 0; i < nprep; ++i) s->prep_threads.emplace_back(&q3x_server_s::preparer, s);
     *out = s;
     return Q3X_OK;
 }
@@ -273,7 +335,9 @@ q3x_status q3x_server_destroy(q3x_server s) {
         });
         s->stop = true;
         s->cv_task.notify_all();
+        s->cv_ready.notify_all();
     }
+    for (auto& t : s->prep_threads) t.join();
     for (auto& t : s->threads) t.join();
     for (auto& kv : s->all)
         for (auto& cp : kv.second->psi)
