@@ -260,8 +260,8 @@ def serving(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                                   "DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "one GPU only: the multi-GPU sharding of this path is a design and is "
+                                   "not launched", "n_gpus": args.gpus}))
         return 2
     # one hardware queue per batch worker stream (HIP's default is 4), set before HIP starts
     # unless the caller chose a value

@@ -36,7 +36,7 @@ def test_supercell_grid_is_twice_primitive(ecut):
 
 
 def test_box_sizes_of_the_workloads():
-    # the survey's brute-force restatement: cfg1 30^3, cfg2 96^3, cfg4 48^3
+    # box sizes counted by brute force over the 2,3,5-smooth orders: cfg1 30^3, cfg2 96^3, cfg4 48^3
     assert O.grid_dims(FCC, 10.26, 120.0, (2, 2, 2)) == (30, 30, 30)
     assert O.grid_dims(np.eye(3), 20.52, 200.0, (1, 1, 1)) == (96, 96, 96)
     assert O.grid_dims(np.eye(3), 10.26, 160.0, (4, 4, 4)) == (48, 48, 48)
@@ -92,7 +92,7 @@ def test_kpoint_lists_match_brute_force():
 
 
 def test_cfg_plane_wave_counts():
-    # survey restatement: cfg1 725 at Gamma, cfg2 51,627 (full sphere Gamma)
+    # brute-force sphere counts: cfg1 725 at Gamma, cfg2 51,627 (full sphere Gamma)
     b1 = O.Basis(FCC, 10.26, 30.0, 0.0, True, (2, 2, 2))
     assert len(b1.kpoint([0, 0, 0])[0]) == 725
     b2 = O.Basis(np.eye(3), 20.52, 50.0, 0.0, True, (1, 1, 1))
