@@ -55,17 +55,17 @@ def parse():
     return p.parse_args()
 
 
-def measured_traffic(stage):
-    """HBM bytes per launch of the FP64 kernel of a stage (zsticks_inverse, plane_sandwich, ...)
-    at this workload, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter
-    scaled by its copy-kernel calibration), or None."""
+def measured_traffic(stage, precision="fp64"):
+    """HBM bytes per launch of the kernel of a stage (zsticks_inverse, plane_sandwich, ...) in
+    one precision ("fp64" or "mixed") at this workload, from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (each counter scaled by its copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    return t.get("bytes_per_launch_by_stage", {}).get("fp64", {}).get(stage)
+    return t.get("bytes_per_launch_by_stage", {}).get(precision, {}).get(stage)
 
 
 def copy_reference(steps):
@@ -397,7 +397,12 @@ def main():
         for name, k in results[label]["kernels"].items():
             b = kernel_bytes(name, nb, npw, nst, n3, npts, cs, sv)
             if b is not None:
-                per_kernel[f"{label}:{name}"] = {"avg_ms": k["avg_ms"], "GB/s": b / (k["avg_ms"] * 1e-3) / 1e9}
+                rec = {"avg_ms": k["avg_ms"], "GB/s": b / (k["avg_ms"] * 1e-3) / 1e9}
+                tb = measured_traffic(name, label)
+                if tb:  # rocprof-measured HBM bytes over the live kernel time (the FFT bandwidth figure)
+                    rec["hbm_measured_GB/s"] = tb / (k["avg_ms"] * 1e-3) / 1e9
+                    rec["hbm_measured_frac"] = rec["hbm_measured_GB/s"] / HBM_PEAK_GBS
+                per_kernel[f"{label}:{name}"] = rec
             elif name.endswith("gemm"):
                 tf = 8.0 * npw * nproj * nb / (k["avg_ms"] * 1e-3) / 1e12
                 per_kernel[f"{label}:{name}"] = {"avg_ms": k["avg_ms"], "TFLOP/s": tf, "frac_peak": tf / peak}
