@@ -56,6 +56,16 @@ def synthetic_species(seed: int = 0):
             "r_loc": R_LOC}
 
 
+def write_synthetic_upf(path: str, seed: int = 0) -> None:
+    """The synthetic species of a seed written as a UPF version 2 file (q3_amd.upf); reading it
+    back with upf.read_upf gives the same arrays bit for bit."""
+    from . import upf
+    upf.write_upf_v2(path, synthetic_species(seed), element="Si",
+                     info=f"Synthetic norm-conserving Si-like species, seed {seed}: Z_val 4, Gaussian local "
+                          f"part r_loc {R_LOC}, s and p projectors r^l exp(-r^2 / 2 sigma^2) with sigma "
+                          f"{SIGMA_S} and {SIGMA_P} bohr, D ~ U[1, 5] Ha.")
+
+
 def d_blocks_for(crystal: Crystal, species_list):
     """Per-atom D blocks (column-major, packed) and their sizes: D_ij = dion[beta_i][beta_j]
     when (l,m) of the two channels agree (norm_conserving/init.hpp:84-100)."""
