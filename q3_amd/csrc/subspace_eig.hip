@@ -525,20 +525,23 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
 
 // Once any workgroup has given up (about a second of polling), every later barrier returns at
 // once, so the kernel always drains quickly; the host turns *fail into an error.
+// Release fence, relaxed add, relaxed polls, one acquire fence: the fences synchronise through
+// the release sequence of the adds, and no poll pays for an acquire (an agent-scope acquire
+// invalidates cache lines; a seq_cst fence on each side would write back and invalidate twice).
 __device__ void grid_barrier(unsigned* bar, unsigned target, int* fail) {
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();  // release this workgroup's global writes
-        atomicAdd(bar, 1u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this workgroup's global writes
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         unsigned spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1u << 20)) {
                 atomicExch(fail, 1);
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __threadfence();  // acquire the other workgroups' writes
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other workgroups' writes
     }
     __syncthreads();
 }
