@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -357,74 +358,99 @@ const char* q3x_last_error(void) { return g_last_error.c_str(); }
 const char* q3x_version(void) { return "q3x 0.1 (gfx950)"; }
 
 q3x_status q3x_context_create(int device, q3x_context* out) {
-    if (!out) {
-        set_error("q3x_context_create: null out");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!out) {
+            set_error("q3x_context_create: null out");
+            return Q3X_ERR_ARGUMENT;
+        }
+        int n = 0;
+        if (!hip_ok(hipGetDeviceCount(&n), "hipGetDeviceCount")) return Q3X_ERR_DEVICE;
+        if (device < 0 || device >= n) {
+            set_error("q3x_context_create: no such device");
+            return Q3X_ERR_DEVICE;
+        }
+        if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return Q3X_ERR_DEVICE;
+        auto* c = new q3x_context_s;
+        c->device = device;
+        if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
+            delete c;
+            return Q3X_ERR_DEVICE;
+        }
+        int lds = 0, cus = 0;
+        (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        c->max_lds = lds > 0 ? static_cast<size_t>(lds) : 65536;
+        c->num_cus = cus;
+        *out = c;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_context_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    int n = 0;
-    if (!hip_ok(hipGetDeviceCount(&n), "hipGetDeviceCount")) return Q3X_ERR_DEVICE;
-    if (device < 0 || device >= n) {
-        set_error("q3x_context_create: no such device");
-        return Q3X_ERR_DEVICE;
-    }
-    if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return Q3X_ERR_DEVICE;
-    auto* c = new q3x_context_s;
-    c->device = device;
-    if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {
-        delete c;
-        return Q3X_ERR_DEVICE;
-    }
-    int lds = 0, cus = 0;
-    (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-    c->max_lds = lds > 0 ? static_cast<size_t>(lds) : 65536;
-    c->num_cus = cus;
-    *out = c;
-    return Q3X_OK;
 }
 
 q3x_status q3x_context_destroy(q3x_context ctx) {
-    if (!ctx) return Q3X_OK;
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-    return Q3X_OK;
+    try {
+        if (!ctx) return Q3X_OK;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_context_destroy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_context_synchronize(q3x_context ctx) {
-    if (!ctx) {
-        set_error("null context");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx) {
+            set_error("null context");
+            return Q3X_ERR_ARGUMENT;
+        }
+        return hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize") ? Q3X_OK : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_context_synchronize: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    return hip_ok(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize") ? Q3X_OK : Q3X_ERR_DEVICE;
 }
 
 q3x_status q3x_profile_enable(q3x_context ctx, int enable) {
-    if (!ctx) {
-        set_error("null context");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx) {
+            set_error("null context");
+            return Q3X_ERR_ARGUMENT;
+        }
+        if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
+        ctx->prof.on = enable != 0;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_profile_enable: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
-    ctx->prof.on = enable != 0;
-    return Q3X_OK;
 }
 
 q3x_status q3x_profile_read(q3x_context ctx, int n, double* total_ms, long* count, int reset) {
-    if (!ctx || n < 0) {
-        set_error("q3x_profile_read: invalid argument");
-        return Q3X_ERR_ARGUMENT;
-    }
-    if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
-    for (int i = 0; i < n && i < K_COUNT; ++i) {
-        if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
-        if (count) count[i] = ctx->prof.count[i];
-    }
-    if (reset)
-        for (int i = 0; i < K_COUNT; ++i) {
-            ctx->prof.total_ms[i] = 0;
-            ctx->prof.count[i] = 0;
+    try {
+        if (!ctx || n < 0) {
+            set_error("q3x_profile_read: invalid argument");
+            return Q3X_ERR_ARGUMENT;
         }
-    return Q3X_OK;
+        if (!ctx->prof.flush(ctx->stream)) return Q3X_ERR_DEVICE;
+        for (int i = 0; i < n && i < K_COUNT; ++i) {
+            if (total_ms) total_ms[i] = ctx->prof.total_ms[i];
+            if (count) count[i] = ctx->prof.count[i];
+        }
+        if (reset)
+            for (int i = 0; i < K_COUNT; ++i) {
+                ctx->prof.total_ms[i] = 0;
+                ctx->prof.count[i] = 0;
+            }
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_profile_read: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 const char* q3x_profile_kernel_name(int id) {
@@ -437,275 +463,350 @@ const char* q3x_profile_kernel_name(int id) {
 
 q3x_status q3x_zgemm(q3x_context ctx, char transa, int m, int n, int k, double alpha, const void* A, long lda,
                      const void* B, long ldb, double beta, void* C, long ldc, int precision) {
-    if (!ctx || m < 0 || n < 0 || k < 0 || (transa != 'N' && transa != 'C') ||
-        (precision != Q3X_FP64 && precision != Q3X_MIXED)) {
-        set_error("q3x_zgemm: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx || m < 0 || n < 0 || k < 0 || (transa != 'N' && transa != 'C') ||
+            (precision != Q3X_FP64 && precision != Q3X_MIXED)) {
+            set_error("q3x_zgemm: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        const long rows_a = transa == 'N' ? m : k;
+        if (lda < std::max<long>(1, rows_a) || ldb < std::max(1, k) || ldc < std::max(1, m)) {
+            set_error("q3x_zgemm: leading dimension too small");
+            return Q3X_ERR_ARGUMENT;
+        }
+        const size_t ws = size_t(64) << 20;
+        if (!ctx->gemm_work.ensure(ws)) return Q3X_ERR_DEVICE;
+        const GemmTypes t = precision == Q3X_FP64 ? GemmTypes::D_D_D_D : GemmTypes::D_D_F_D;
+        return hip_ok(zgemm(t, transa == 'C' ? 1 : 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                            ctx->gemm_work.get(), ws, ctx->stream),
+                      "q3x_zgemm")
+                   ? Q3X_OK
+                   : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_zgemm: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    const long rows_a = transa == 'N' ? m : k;
-    if (lda < std::max<long>(1, rows_a) || ldb < std::max(1, k) || ldc < std::max(1, m)) {
-        set_error("q3x_zgemm: leading dimension too small");
-        return Q3X_ERR_ARGUMENT;
-    }
-    const size_t ws = size_t(64) << 20;
-    if (!ctx->gemm_work.ensure(ws)) return Q3X_ERR_DEVICE;
-    const GemmTypes t = precision == Q3X_FP64 ? GemmTypes::D_D_D_D : GemmTypes::D_D_F_D;
-    return hip_ok(zgemm(t, transa == 'C' ? 1 : 0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                        ctx->gemm_work.get(), ws, ctx->stream),
-                  "q3x_zgemm")
-               ? Q3X_OK
-               : Q3X_ERR_DEVICE;
 }
 
 q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, const void* S, long lds, int nvec,
                            double* w, void* X, long ldx) {
-    if (!ctx || n < 0 || nvec < 0 || nvec > n ||
-        (n > 0 && (!H || !S || (nvec > 0 && (!w || !X)) || ldh < n || lds < n || ldx < n))) {
-        set_error("q3x_hegv_device: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx || n < 0 || nvec < 0 || nvec > n ||
+            (n > 0 && (!H || !S || (nvec > 0 && (!w || !X)) || ldh < n || lds < n || ldx < n))) {
+            set_error("q3x_hegv_device: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        if (n == 0 || nvec == 0) return Q3X_OK;
+        const size_t ws = size_t(64) << 20;
+        if (!ctx->gemm_work.ensure(ws) || !ctx->eig_work.ensure(subspace_eig_workspace(n, nvec))) return Q3X_ERR_DEVICE;
+        hipError_t e = hipSuccess;
+        const int info = subspace_eig(n, static_cast<const double*>(H), ldh, static_cast<const double*>(S), lds,
+                                      nvec, w,
+                                      static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
+                                      ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr,
+                                      nullptr, &ctx->pin);
+        if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
+        if (info > 0) {
+            set_error("q3x_hegv_device: S is not positive definite");
+            return Q3X_ERR_NUMERIC;
+        }
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hegv_device: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    if (n == 0 || nvec == 0) return Q3X_OK;
-    const size_t ws = size_t(64) << 20;
-    if (!ctx->gemm_work.ensure(ws) || !ctx->eig_work.ensure(subspace_eig_workspace(n, nvec))) return Q3X_ERR_DEVICE;
-    hipError_t e = hipSuccess;
-    const int info = subspace_eig(n, static_cast<const double*>(H), ldh, static_cast<const double*>(S), lds, nvec, w,
-                                  static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
-                                  ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr,
-                                  nullptr, &ctx->pin);
-    if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
-    if (info > 0) {
-        set_error("q3x_hegv_device: S is not positive definite");
-        return Q3X_ERR_NUMERIC;
-    }
-    return Q3X_OK;
 }
 
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {
-    if (!ctx || !out) {
-        set_error("q3x_device_alloc: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx || !out) {
+            set_error("q3x_device_alloc: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        return hip_ok(hipMalloc(out, bytes ? bytes : 16), "hipMalloc") ? Q3X_OK : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_device_alloc: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    return hip_ok(hipMalloc(out, bytes ? bytes : 16), "hipMalloc") ? Q3X_OK : Q3X_ERR_DEVICE;
 }
 
 q3x_status q3x_device_copy(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
-    if (!ctx || (bytes % 16) != 0) {
-        set_error("q3x_device_copy: bytes must be a multiple of 16");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx || (bytes % 16) != 0) {
+            set_error("q3x_device_copy: bytes must be a multiple of 16");
+            return Q3X_ERR_ARGUMENT;
+        }
+        return hip_ok(launch_copy16(src, dst, static_cast<long>(bytes / 16), ctx->stream), "copy kernel")
+                   ? Q3X_OK
+                   : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_device_copy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    return hip_ok(launch_copy16(src, dst, static_cast<long>(bytes / 16), ctx->stream), "copy kernel") ? Q3X_OK
-                                                                                                      : Q3X_ERR_DEVICE;
 }
 
 q3x_status q3x_device_free(q3x_context ctx, void* p) {
-    (void)ctx;
-    return hip_ok(hipFree(p), "hipFree") ? Q3X_OK : Q3X_ERR_DEVICE;
+    try {
+        (void)ctx;
+        return hip_ok(hipFree(p), "hipFree") ? Q3X_OK : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_device_free: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_copy_to_device(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
-    if (!ctx) return Q3X_ERR_ARGUMENT;
-    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D") &&
-                   hip_ok(hipStreamSynchronize(ctx->stream), "sync")
-               ? Q3X_OK
-               : Q3X_ERR_DEVICE;
+    try {
+        if (!ctx) return Q3X_ERR_ARGUMENT;
+        return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "H2D") &&
+                       hip_ok(hipStreamSynchronize(ctx->stream), "sync")
+                   ? Q3X_OK
+                   : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_copy_to_device: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_copy_to_host(q3x_context ctx, void* dst, const void* src, unsigned long bytes) {
-    if (!ctx) return Q3X_ERR_ARGUMENT;
-    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H") &&
-                   hip_ok(hipStreamSynchronize(ctx->stream), "sync")
-               ? Q3X_OK
-               : Q3X_ERR_DEVICE;
+    try {
+        if (!ctx) return Q3X_ERR_ARGUMENT;
+        return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "D2H") &&
+                       hip_ok(hipStreamSynchronize(ctx->stream), "sync")
+                   ? Q3X_OK
+                   : Q3X_ERR_DEVICE;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_copy_to_host: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 
 q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian* out) {
-    if (!ctx || !b || !out) {
-        set_error("q3x_hamiltonian_create: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!ctx || !b || !out) {
+            set_error("q3x_hamiltonian_create: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        auto* h = new q3x_hamiltonian_s;
+        h->ctx = ctx;
+        const q3x_status s = hamiltonian_rebind(h, b);
+        if (s != Q3X_OK) {
+            delete h;
+            return s;
+        }
+        *out = h;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    auto* h = new q3x_hamiltonian_s;
-    h->ctx = ctx;
-    const q3x_status s = hamiltonian_rebind(h, b);
-    if (s != Q3X_OK) {
-        delete h;
-        return s;
-    }
-    *out = h;
-    return Q3X_OK;
 }
 
 q3x_status q3x_hamiltonian_destroy(q3x_hamiltonian h) {
-    if (h && h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
-    delete h;
-    return Q3X_OK;
+    try {
+        if (h && h->ctx) (void)hipStreamSynchronize(h->ctx->stream);
+        delete h;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_destroy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v, long n) {
-    if (!h || !v || n != h->npoints) {
-        set_error("set_potential: V must hold n1*n2*n3 wave-grid points");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!h || !v || n != h->npoints) {
+            set_error("set_potential: V must hold n1*n2*n3 wave-grid points");
+            return Q3X_ERR_ARGUMENT;
+        }
+        const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
+        const double inv = 1.0 / static_cast<double>(h->npoints);  // serial.hpp:69-73
+        std::vector<double> d(h->npoints);
+        std::vector<float> f(h->npoints);
+        for (int i1 = 0; i1 < n1; ++i1)
+            for (int i2 = 0; i2 < n2; ++i2)
+                for (int i3 = 0; i3 < n3; ++i3) {
+                    const double s = v[(static_cast<long>(i1) * n2 + i2) * n3 + i3] * inv;
+                    const long o = (static_cast<long>(i3) * n2 + i2) * n1 + i1;
+                    d[o] = s;
+                    f[o] = static_cast<float>(s);
+                }
+        if (!h->v64.upload(d.data(), d.size() * sizeof(double), h->ctx->stream, &h->ctx->pin) ||
+            !h->v32.upload(f.data(), f.size() * sizeof(float), h->ctx->stream, &h->ctx->pin))
+            return Q3X_ERR_DEVICE;
+        h->have_potential = true;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_set_potential: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
-    const double inv = 1.0 / static_cast<double>(h->npoints);  // serial.hpp:69-73
-    std::vector<double> d(h->npoints);
-    std::vector<float> f(h->npoints);
-    for (int i1 = 0; i1 < n1; ++i1)
-        for (int i2 = 0; i2 < n2; ++i2)
-            for (int i3 = 0; i3 < n3; ++i3) {
-                const double s = v[(static_cast<long>(i1) * n2 + i2) * n3 + i3] * inv;
-                const long o = (static_cast<long>(i3) * n2 + i2) * n1 + i1;
-                d[o] = s;
-                f[o] = static_cast<float>(s);
-            }
-    if (!h->v64.upload(d.data(), d.size() * sizeof(double), h->ctx->stream, &h->ctx->pin) ||
-        !h->v32.upload(f.data(), f.size() * sizeof(float), h->ctx->stream, &h->ctx->pin))
-        return Q3X_ERR_DEVICE;
-    h->have_potential = true;
-    return Q3X_OK;
 }
 
 q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
-    if (!h || !k || k->basis != h->basis) {
-        set_error("prepare_kpoint: k-point must come from the Hamiltonian's basis");
-        return Q3X_ERR_ARGUMENT;
-    }
-    const KPointData& K = k->d;
-    const StickLayout& S = K.sticks;
-    hipStream_t st = h->ctx->stream;
-    h->npw = static_cast<int>(K.gidx.size());
-    h->nst = S.nst;
-    h->ncol = S.ncol;
-    std::vector<int> out;
-    for (int p = 0; p < h->npw; ++p)
-        if (K.fftmap[p] < 0) out.push_back(p);
-    h->n_outside = static_cast<int>(out.size());
-    const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
-    h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
-    // two-level plane kernel where the plane is square and of a size it is built for;
-    // Q3X_FFT_KERNELS=generic keeps the Stockham kernels (used to compare the two)
-    const char* pk = std::getenv("Q3X_FFT_KERNELS");
-    const bool use_tl = !(pk && std::strcmp(pk, "generic") == 0) && plane_tl_supported(n1, n2);
-    h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds) : PlaneTlConfig{};
-    h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds) : PlaneTlConfig{};
-    const bool use_tz = !(pk && std::strcmp(pk, "generic") == 0) && stick_tl_supported(n3);
-    h->tz64 = use_tz ? stick_tl_config(n3, false) : StickTlConfig{};
-    h->tz32 = use_tz ? stick_tl_config(n3, true) : StickTlConfig{};
-    // generic plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of
-    // 8 waves
-    for (int fp32 = 0; fp32 < 2; ++fp32) {
-        const size_t cs = fp32 ? 8 : 16;
-        const int ps = fp32 ? lds_pad_shift<cx<float>>() : lds_pad_shift<cx<double>>();
-        const size_t l1 = padded_len(n1, ps), l2 = padded_len(n2, ps);
-        const size_t fixed = cs * (n1 + n2 + l2 * S.ncol);
-        long rp = 0;
-        if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * l1 * 8));
-        rp = std::min<long>(rp, 4);
-        rp = std::min<long>(rp, n2);
-        if (rp < 1 && !(fp32 ? h->tl32.ok : h->tl64.ok)) {
-            set_error("prepare_kpoint: the occupied plane does not fit in LDS for this box");
+    try {
+        if (!h || !k || k->basis != h->basis) {
+            set_error("prepare_kpoint: k-point must come from the Hamiltonian's basis");
             return Q3X_ERR_ARGUMENT;
         }
-        (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(std::max<long>(rp, 1));
+        const KPointData& K = k->d;
+        const StickLayout& S = K.sticks;
+        hipStream_t st = h->ctx->stream;
+        h->npw = static_cast<int>(K.gidx.size());
+        h->nst = S.nst;
+        h->ncol = S.ncol;
+        std::vector<int> out;
+        for (int p = 0; p < h->npw; ++p)
+            if (K.fftmap[p] < 0) out.push_back(p);
+        h->n_outside = static_cast<int>(out.size());
+        const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
+        h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
+        // two-level plane kernel where the plane is square and of a size it is built for;
+        // Q3X_FFT_KERNELS=generic keeps the Stockham kernels (used to compare the two)
+        const char* pk = std::getenv("Q3X_FFT_KERNELS");
+        const bool use_tl = !(pk && std::strcmp(pk, "generic") == 0) && plane_tl_supported(n1, n2);
+        h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds) : PlaneTlConfig{};
+        h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds) : PlaneTlConfig{};
+        const bool use_tz = !(pk && std::strcmp(pk, "generic") == 0) && stick_tl_supported(n3);
+        h->tz64 = use_tz ? stick_tl_config(n3, false) : StickTlConfig{};
+        h->tz32 = use_tz ? stick_tl_config(n3, true) : StickTlConfig{};
+        // generic plane kernel LDS: twiddles + [ncol][n2] plane + a buffer of rows_per_pass rows of n1 for each of
+        // 8 waves
+        for (int fp32 = 0; fp32 < 2; ++fp32) {
+            const size_t cs = fp32 ? 8 : 16;
+            const int ps = fp32 ? lds_pad_shift<cx<float>>() : lds_pad_shift<cx<double>>();
+            const size_t l1 = padded_len(n1, ps), l2 = padded_len(n2, ps);
+            const size_t fixed = cs * (n1 + n2 + l2 * S.ncol);
+            long rp = 0;
+            if (fixed < h->ctx->max_lds) rp = static_cast<long>((h->ctx->max_lds - fixed) / (cs * l1 * 8));
+            rp = std::min<long>(rp, 4);
+            rp = std::min<long>(rp, n2);
+            if (rp < 1 && !(fp32 ? h->tl32.ok : h->tl64.ok)) {
+                set_error("prepare_kpoint: the occupied plane does not fit in LDS for this box");
+                return Q3X_ERR_ARGUMENT;
+            }
+            (fp32 ? h->rows_per_pass32 : h->rows_per_pass64) = static_cast<int>(std::max<long>(rp, 1));
+        }
+        bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->stick_col.upload(S.stick_col.data(), S.stick_col.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st, &h->ctx->pin) &&
+                  h->outside.upload(out.data(), out.size() * sizeof(int), st, &h->ctx->pin);
+        if (!ok) return Q3X_ERR_DEVICE;
+        h->kp = k;
+        h->have_projectors = false;
+        h->nproj = 0;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_prepare_kpoint: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    bool ok = h->stick_off.upload(S.stick_off.data(), S.stick_off.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->stick_i2.upload(S.stick_i2.data(), S.stick_i2.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->stick_col.upload(S.stick_col.data(), S.stick_col.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st, &h->ctx->pin) &&
-              h->outside.upload(out.data(), out.size() * sizeof(int), st, &h->ctx->pin);
-    if (!ok) return Q3X_ERR_DEVICE;
-    h->kp = k;
-    h->have_projectors = false;
-    h->nproj = 0;
-    return Q3X_OK;
 }
 
 q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta, int nproj, int natoms,
                                           const int* block_sizes, const double* d_blocks) {
-    if (!h || !h->kp) {
-        set_error("set_projectors: prepare_kpoint() first");
-        return Q3X_ERR_STATE;
-    }
-    if (nproj < 0 || natoms < 0 || (nproj > 0 && (!beta || !block_sizes || !d_blocks))) {
-        set_error("set_projectors: invalid argument");
-        return Q3X_ERR_ARGUMENT;
-    }
-    int total = 0;
-    for (int a = 0; a < natoms; ++a) total += block_sizes[a];
-    if (total != nproj) {
-        set_error("set_projectors: block sizes must sum to nproj");
-        return Q3X_ERR_ARGUMENT;
-    }
-    hipStream_t st = h->ctx->stream;
-    const size_t n = static_cast<size_t>(h->npw) * nproj;
-    std::vector<float> b32(2 * n);
-    for (size_t i = 0; i < 2 * n; ++i) b32[i] = static_cast<float>(beta[i]);
-    std::vector<int> ps(nproj), pz(nproj);
-    std::vector<long> po(nproj);
-    std::vector<double> D;
-    int start = 0;
-    long off = 0;
-    for (int a = 0; a < natoms; ++a) {
-        const int bs = block_sizes[a];
-        for (int q = 0; q < bs * bs; ++q) {
-            const double dv = d_blocks[off + q];
-            D.push_back(std::abs(dv) > 1e-15 ? dv : 0.0);  // d_matrix_element_threshold
+    try {
+        if (!h || !h->kp) {
+            set_error("set_projectors: prepare_kpoint() first");
+            return Q3X_ERR_STATE;
         }
-        for (int i = 0; i < bs; ++i) {
-            ps[start + i] = start;
-            pz[start + i] = bs;
-            po[start + i] = off;
+        if (nproj < 0 || natoms < 0 || (nproj > 0 && (!beta || !block_sizes || !d_blocks))) {
+            set_error("set_projectors: invalid argument");
+            return Q3X_ERR_ARGUMENT;
         }
-        start += bs;
-        off += static_cast<long>(bs) * bs;
+        int total = 0;
+        for (int a = 0; a < natoms; ++a) total += block_sizes[a];
+        if (total != nproj) {
+            set_error("set_projectors: block sizes must sum to nproj");
+            return Q3X_ERR_ARGUMENT;
+        }
+        hipStream_t st = h->ctx->stream;
+        const size_t n = static_cast<size_t>(h->npw) * nproj;
+        std::vector<float> b32(2 * n);
+        for (size_t i = 0; i < 2 * n; ++i) b32[i] = static_cast<float>(beta[i]);
+        std::vector<int> ps(nproj), pz(nproj);
+        std::vector<long> po(nproj);
+        std::vector<double> D;
+        int start = 0;
+        long off = 0;
+        for (int a = 0; a < natoms; ++a) {
+            const int bs = block_sizes[a];
+            for (int q = 0; q < bs * bs; ++q) {
+                const double dv = d_blocks[off + q];
+                D.push_back(std::abs(dv) > 1e-15 ? dv : 0.0);  // d_matrix_element_threshold
+            }
+            for (int i = 0; i < bs; ++i) {
+                ps[start + i] = start;
+                pz[start + i] = bs;
+                po[start + i] = off;
+            }
+            start += bs;
+            off += static_cast<long>(bs) * bs;
+        }
+        if (D.empty()) D.push_back(0.0);
+        bool ok = h->beta64.upload(beta, n * 16, st, &h->ctx->pin) &&
+                  h->beta32.upload(b32.data(), n * 8, st, &h->ctx->pin) &&
+                  h->dblocks.upload(D.data(), D.size() * sizeof(double), st, &h->ctx->pin) &&
+                  h->pstart.upload(ps.data(), ps.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->psize.upload(pz.data(), pz.size() * sizeof(int), st, &h->ctx->pin) &&
+                  h->poff.upload(po.data(), po.size() * sizeof(long), st, &h->ctx->pin);
+        if (!ok) return Q3X_ERR_DEVICE;
+        h->nproj = nproj;
+        h->have_projectors = true;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_set_projectors: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    if (D.empty()) D.push_back(0.0);
-    bool ok = h->beta64.upload(beta, n * 16, st, &h->ctx->pin) && h->beta32.upload(b32.data(), n * 8, st, &h->ctx->pin) &&
-              h->dblocks.upload(D.data(), D.size() * sizeof(double), st, &h->ctx->pin) &&
-              h->pstart.upload(ps.data(), ps.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->psize.upload(pz.data(), pz.size() * sizeof(int), st, &h->ctx->pin) &&
-              h->poff.upload(po.data(), po.size() * sizeof(long), st, &h->ctx->pin);
-    if (!ok) return Q3X_ERR_DEVICE;
-    h->nproj = nproj;
-    h->have_projectors = true;
-    return Q3X_OK;
 }
 
 q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi, int nvec,
                                         int precision) {
-    if (!h) {
-        set_error("apply_device: null Hamiltonian");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!h) {
+            set_error("apply_device: null Hamiltonian");
+            return Q3X_ERR_ARGUMENT;
+        }
+        return hamiltonian_apply_device(h, static_cast<const double*>(d_psi), h->npw, static_cast<double*>(d_hpsi),
+                                        h->npw,
+                                        nvec, precision);
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_apply_device: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    return hamiltonian_apply_device(h, static_cast<const double*>(d_psi), h->npw, static_cast<double*>(d_hpsi), h->npw,
-                                    nvec, precision);
 }
 
 q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* hpsi, int nvec, int precision) {
-    if (!h || !h->kp) {
-        set_error("apply: prepare_kpoint() must be called before apply (serial.hpp:487-492)");
-        return Q3X_ERR_STATE;
+    try {
+        if (!h || !h->kp) {
+            set_error("apply: prepare_kpoint() must be called before apply (serial.hpp:487-492)");
+            return Q3X_ERR_STATE;
+        }
+        if (nvec < 0 || (nvec > 0 && (!psi || !hpsi))) {
+            set_error("apply: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        if (nvec == 0) return Q3X_OK;
+        hipStream_t st = h->ctx->stream;
+        const size_t bytes = static_cast<size_t>(h->npw) * nvec * 16;
+        if (!h->io_psi.ensure(bytes) || !h->io_hpsi.ensure(bytes)) return Q3X_ERR_DEVICE;
+        if (!hip_ok(hipMemcpyAsync(h->io_psi.get(), psi, bytes, hipMemcpyHostToDevice, st), "H2D psi"))
+            return Q3X_ERR_DEVICE;
+        q3x_status s = hamiltonian_apply_device(h, h->io_psi.as<double>(), h->npw, h->io_hpsi.as<double>(), h->npw,
+                                                nvec,
+                                                precision);
+        if (s != Q3X_OK) return s;
+        if (!hip_ok(hipMemcpyAsync(hpsi, h->io_hpsi.get(), bytes, hipMemcpyDeviceToHost, st), "D2H hpsi") ||
+            !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+            return Q3X_ERR_DEVICE;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_apply: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    if (nvec < 0 || (nvec > 0 && (!psi || !hpsi))) {
-        set_error("apply: invalid argument");
-        return Q3X_ERR_ARGUMENT;
-    }
-    if (nvec == 0) return Q3X_OK;
-    hipStream_t st = h->ctx->stream;
-    const size_t bytes = static_cast<size_t>(h->npw) * nvec * 16;
-    if (!h->io_psi.ensure(bytes) || !h->io_hpsi.ensure(bytes)) return Q3X_ERR_DEVICE;
-    if (!hip_ok(hipMemcpyAsync(h->io_psi.get(), psi, bytes, hipMemcpyHostToDevice, st), "H2D psi"))
-        return Q3X_ERR_DEVICE;
-    q3x_status s = hamiltonian_apply_device(h, h->io_psi.as<double>(), h->npw, h->io_hpsi.as<double>(), h->npw, nvec,
-                                            precision);
-    if (s != Q3X_OK) return s;
-    if (!hip_ok(hipMemcpyAsync(hpsi, h->io_hpsi.get(), bytes, hipMemcpyDeviceToHost, st), "D2H hpsi") ||
-        !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
-        return Q3X_ERR_DEVICE;
-    return Q3X_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <atomic>
 #include <mutex>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -31,101 +32,123 @@ using namespace q3x;
 extern "C" {
 
 q3x_status q3x_batch_create(int device, int workers, q3x_batch* out) {
-    if (!out || workers < 1 || workers > 64) {
-        set_error("q3x_batch_create: need 1 <= workers <= 64");
-        return Q3X_ERR_ARGUMENT;
-    }
-    auto* b = new q3x_batch_s;
-    b->device = device;
-    for (int w = 0; w < workers; ++w) {
-        q3x_context c = nullptr;
-        const q3x_status s = q3x_context_create(device, &c);
-        if (s != Q3X_OK) {
-            q3x_batch_destroy(b);
-            return s;
+    try {
+        if (!out || workers < 1 || workers > 64) {
+            set_error("q3x_batch_create: need 1 <= workers <= 64");
+            return Q3X_ERR_ARGUMENT;
         }
-        b->workers.push_back(c);
+        auto* b = new q3x_batch_s;
+        b->device = device;
+        for (int w = 0; w < workers; ++w) {
+            q3x_context c = nullptr;
+            const q3x_status s = q3x_context_create(device, &c);
+            if (s != Q3X_OK) {
+                q3x_batch_destroy(b);
+                return s;
+            }
+            b->workers.push_back(c);
+        }
+        *out = b;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_batch_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    *out = b;
-    return Q3X_OK;
 }
 
 q3x_status q3x_batch_destroy(q3x_batch b) {
-    if (!b) return Q3X_OK;
-    for (q3x_hamiltonian h : b->hams) q3x_hamiltonian_destroy(h);
-    for (q3x_context c : b->workers) q3x_context_destroy(c);
-    delete b;
-    return Q3X_OK;
+    try {
+        if (!b) return Q3X_OK;
+        for (q3x_hamiltonian h : b->hams) q3x_hamiltonian_destroy(h);
+        for (q3x_context c : b->workers) q3x_context_destroy(c);
+        delete b;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_batch_destroy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_batch_add(q3x_batch b, q3x_basis basis, q3x_kpoint kp, const double* v_r, long n,
                          const double* beta, int nproj, int natoms, const int* block_sizes,
                          const double* d_blocks, int* index) {
-    if (!b || !basis || !kp || !v_r) {
-        set_error("q3x_batch_add: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!b || !basis || !kp || !v_r) {
+            set_error("q3x_batch_add: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        const int i = static_cast<int>(b->hams.size());
+        q3x_context ctx = b->workers[i % b->workers.size()];
+        q3x_hamiltonian h = nullptr;
+        q3x_status s = q3x_hamiltonian_create(ctx, basis, &h);
+        if (s != Q3X_OK) return s;
+        if ((s = q3x_hamiltonian_set_potential(h, v_r, n)) != Q3X_OK ||
+            (s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
+            (nproj > 0 &&
+             (s = q3x_hamiltonian_set_projectors(h, beta, nproj, natoms, block_sizes, d_blocks)) != Q3X_OK)) {
+            q3x_hamiltonian_destroy(h);
+            return s;
+        }
+        b->hams.push_back(h);
+        b->npw.push_back(h->npw);
+        if (index) *index = i;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_batch_add: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    const int i = static_cast<int>(b->hams.size());
-    q3x_context ctx = b->workers[i % b->workers.size()];
-    q3x_hamiltonian h = nullptr;
-    q3x_status s = q3x_hamiltonian_create(ctx, basis, &h);
-    if (s != Q3X_OK) return s;
-    if ((s = q3x_hamiltonian_set_potential(h, v_r, n)) != Q3X_OK || (s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK ||
-        (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, beta, nproj, natoms, block_sizes, d_blocks)) != Q3X_OK)) {
-        q3x_hamiltonian_destroy(h);
-        return s;
-    }
-    b->hams.push_back(h);
-    b->npw.push_back(h->npw);
-    if (index) *index = i;
-    return Q3X_OK;
 }
 
 int q3x_batch_size(q3x_batch b) { return b ? static_cast<int>(b->hams.size()) : 0; }
 
 q3x_status q3x_batch_solve(q3x_batch b, int nb, const q3x_davidson_options* opt, double* const* psi_inout,
                            double* eig_out, q3x_davidson_report* reports) {
-    if (!b || !opt || !psi_inout || !eig_out || nb <= 0) {
-        set_error("q3x_batch_solve: invalid argument");
-        return Q3X_ERR_ARGUMENT;
-    }
-    const int nk = static_cast<int>(b->hams.size());
-    const int nw = static_cast<int>(b->workers.size());
-    std::mutex mu;
-    q3x_status first = Q3X_OK;
-    std::string first_msg;
-    std::atomic<bool> failed{false};
-    auto work = [&](int w) {
-        if (hipSetDevice(b->device) != hipSuccess) {
-            std::lock_guard<std::mutex> g(mu);
-            if (first == Q3X_OK) {
-                first = Q3X_ERR_DEVICE;
-                first_msg = "q3x_batch_solve: hipSetDevice failed in a worker";
-            }
-            failed = true;
-            return;
+    try {
+        if (!b || !opt || !psi_inout || !eig_out || nb <= 0) {
+            set_error("q3x_batch_solve: invalid argument");
+            return Q3X_ERR_ARGUMENT;
         }
-        for (int i = w; i < nk && !failed; i += nw) {
-            q3x_davidson_report rep{};
-            const q3x_status s = q3x_davidson_solve(b->hams[i], nb, opt, psi_inout[i],
-                                                    eig_out + static_cast<size_t>(i) * nb, &rep);
-            if (reports) reports[i] = rep;
-            if (s != Q3X_OK) {
+        const int nk = static_cast<int>(b->hams.size());
+        const int nw = static_cast<int>(b->workers.size());
+        std::mutex mu;
+        q3x_status first = Q3X_OK;
+        std::string first_msg;
+        std::atomic<bool> failed{false};
+        auto work = [&](int w) {
+            if (hipSetDevice(b->device) != hipSuccess) {
                 std::lock_guard<std::mutex> g(mu);
                 if (first == Q3X_OK) {
-                    first = s;
-                    first_msg = "q3x_batch_solve: k-point " + std::to_string(i) + ": " + q3x_last_error();
+                    first = Q3X_ERR_DEVICE;
+                    first_msg = "q3x_batch_solve: hipSetDevice failed in a worker";
                 }
                 failed = true;
+                return;
             }
-        }
-    };
-    std::vector<std::thread> threads;
-    for (int w = 1; w < nw; ++w) threads.emplace_back(work, w);
-    work(0);
-    for (auto& t : threads) t.join();
-    if (first != Q3X_OK) set_error(first_msg);
-    return first;
+            for (int i = w; i < nk && !failed; i += nw) {
+                q3x_davidson_report rep{};
+                const q3x_status s = q3x_davidson_solve(b->hams[i], nb, opt, psi_inout[i],
+                                                        eig_out + static_cast<size_t>(i) * nb, &rep);
+                if (reports) reports[i] = rep;
+                if (s != Q3X_OK) {
+                    std::lock_guard<std::mutex> g(mu);
+                    if (first == Q3X_OK) {
+                        first = s;
+                        first_msg = "q3x_batch_solve: k-point " + std::to_string(i) + ": " + q3x_last_error();
+                    }
+                    failed = true;
+                }
+            }
+        };
+        std::vector<std::thread> threads;
+        for (int w = 1; w < nw; ++w) threads.emplace_back(work, w);
+        work(0);
+        for (auto& t : threads) t.join();
+        if (first != Q3X_OK) set_error(first_msg);
+        return first;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_batch_solve: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 }  // extern "C"

@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 
 #include "aux_kernels.h"
@@ -488,10 +489,20 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
 
 extern "C" q3x_status q3x_davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
                                          double* psi_inout, double* eig_out, q3x_davidson_report* report) {
-    return davidson_solve(h, nb, opt, psi_inout, false, eig_out, report);
+    try {
+        return davidson_solve(h, nb, opt, psi_inout, false, eig_out, report);
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_davidson_solve: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 extern "C" q3x_status q3x_davidson_solve_device(q3x_hamiltonian h, int nb, const q3x_davidson_options* opt,
                                                 void* d_psi_inout, double* eig_out, q3x_davidson_report* report) {
-    return davidson_solve(h, nb, opt, static_cast<double*>(d_psi_inout), true, eig_out, report);
+    try {
+        return davidson_solve(h, nb, opt, static_cast<double*>(d_psi_inout), true, eig_out, report);
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_davidson_solve_device: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }

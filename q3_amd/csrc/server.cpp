@@ -24,6 +24,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -139,7 +140,16 @@ struct q3x_server_s {
                 if (r->next_task >= r->nk) running.pop_front();
                 ++preparing;
             }
-            std::unique_ptr<Prepared> p = prepare(r, k);
+            std::unique_ptr<Prepared> p;
+            try {
+                p = prepare(r, k);
+            } catch (const std::exception& ex) {  // e.g. an allocation: the k-point fails, the thread lives
+                p = std::make_unique<Prepared>();
+                p->r = r;
+                p->k = k;
+                p->status = Q3X_ERR_INTERNAL;
+                p->err = std::string("preparing the k-point: ") + ex.what();
+            }
             std::lock_guard<std::mutex> g(mu);
             --preparing;
             ready.push_back(std::move(p));
@@ -292,170 +302,196 @@ double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int n
 extern "C" {
 
 q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out) {
-    if (!out || workers < 1 || workers > 64) {
-        set_error("q3x_server_create: need 1 <= workers <= 64");
-        return Q3X_ERR_ARGUMENT;
-    }
-    auto* s = new q3x_server_s;
-    s->device = device;
-    s->t0 = clk::now();
-    for (int w = 0; w < workers; ++w) {
-        q3x_context c = nullptr;
-        const q3x_status st = q3x_context_create(device, &c);
-        if (st != Q3X_OK) {
-            for (q3x_context x : s->workers) q3x_context_destroy(x);
-            delete s;
-            return st;
+    try {
+        if (!out || workers < 1 || workers > 64) {
+            set_error("q3x_server_create: need 1 <= workers <= 64");
+            return Q3X_ERR_ARGUMENT;
         }
-        s->workers.push_back(c);
+        auto* s = new q3x_server_s;
+        s->device = device;
+        s->t0 = clk::now();
+        for (int w = 0; w < workers; ++w) {
+            q3x_context c = nullptr;
+            const q3x_status st = q3x_context_create(device, &c);
+            if (st != Q3X_OK) {
+                for (q3x_context x : s->workers) q3x_context_destroy(x);
+                delete s;
+                return st;
+            }
+            s->workers.push_back(c);
+        }
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
+        s->hams.assign(workers, nullptr);
+        s->ham_ticket.assign(workers, 0);
+        // host preparation runs ahead on half as many threads as workers (its share of a k-point's
+        // time is about a third of the device part at the cfg5 cell sizes)
+        const int nprep = std::max(2, (workers + 1) / 2);
+        s->max_ready = static_cast<size_t>(2 * workers);
+        for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
+        for (int i = 0; i < nprep; ++i) s->prep_threads.emplace_back(&q3x_server_s::preparer, s);
+        *out = s;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_create: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    size_t free_b = 0, total_b = 0;
-    (void)hipMemGetInfo(&free_b, &total_b);
-    s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
-    s->hams.assign(workers, nullptr);
-    s->ham_ticket.assign(workers, 0);
-    // host preparation runs ahead on half as many threads as workers (its share of a k-point's
-    // time is about a third of the device part at the cfg5 cell sizes)
-    const int nprep = std::max(2, (workers + 1) / 2);
-    s->max_ready = static_cast<size_t>(2 * workers);
-    for (int w = 0; w < workers; ++w) s->threads.emplace_back(&q3x_server_s::worker, s, w);
-    for (int i = 0; i < nprep; ++i) s->prep_threads.emplace_back(&q3x_server_s::preparer, s);
-    *out = s;
-    return Q3X_OK;
 }
 
 q3x_status q3x_server_destroy(q3x_server s) {
-    if (!s) return Q3X_OK;
-    {
-        std::unique_lock<std::mutex> g(s->mu);
-        s->cv_done.wait(g, [&] {
-            for (auto& kv : s->all)
-                if (!kv.second->done) return false;
-            return true;
-        });
-        s->stop = true;
-        s->cv_task.notify_all();
-        s->cv_ready.notify_all();
+    try {
+        if (!s) return Q3X_OK;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            s->cv_done.wait(g, [&] {
+                for (auto& kv : s->all)
+                    if (!kv.second->done) return false;
+                return true;
+            });
+            s->stop = true;
+            s->cv_task.notify_all();
+            s->cv_ready.notify_all();
+        }
+        for (auto& t : s->prep_threads) t.join();
+        for (auto& t : s->threads) t.join();
+        for (auto& kv : s->all)
+            for (auto& cp : kv.second->psi)
+                if (cp.second) q3x_device_free(cp.first, cp.second);
+        for (auto& kv : s->psi_pool) q3x_device_free(s->workers[0], kv.second);
+        for (q3x_hamiltonian h : s->hams)
+            if (h) q3x_hamiltonian_destroy(h);
+        for (q3x_context c : s->workers) q3x_context_destroy(c);
+        delete s;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_destroy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    for (auto& t : s->prep_threads) t.join();
-    for (auto& t : s->threads) t.join();
-    for (auto& kv : s->all)
-        for (auto& cp : kv.second->psi)
-            if (cp.second) q3x_device_free(cp.first, cp.second);
-    for (auto& kv : s->psi_pool) q3x_device_free(s->workers[0], kv.second);
-    for (q3x_hamiltonian h : s->hams)
-        if (h) q3x_hamiltonian_destroy(h);
-    for (q3x_context c : s->workers) q3x_context_destroy(c);
-    delete s;
-    return Q3X_OK;
 }
 
 q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* bytes) {
-    if (!s || !q || !q->basis || !bytes || q->nk < 1) {
-        set_error("q3x_server_estimate: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!s || !q || !q->basis || !bytes || q->nk < 1) {
+            set_error("q3x_server_estimate: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        q3x_kpoint kp = nullptr;
+        int npw = 0;
+        const q3x_status st = q3x_kpoint_create(q->basis, q->kpoints, &kp, &npw);
+        if (st != Q3X_OK) return st;
+        q3x_kpoint_destroy(kp);
+        int rho[3], wave[3], ng = 0;
+        double vol = 0, tpa2 = 0;
+        q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
+        int nproj = 0;
+        for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
+        *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2],
+                           static_cast<long>(wave[0]) * wave[1] * wave[2],
+                           nproj, static_cast<int>(s->workers.size()));
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_estimate: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    q3x_kpoint kp = nullptr;
-    int npw = 0;
-    const q3x_status st = q3x_kpoint_create(q->basis, q->kpoints, &kp, &npw);
-    if (st != Q3X_OK) return st;
-    q3x_kpoint_destroy(kp);
-    int rho[3], wave[3], ng = 0;
-    double vol = 0, tpa2 = 0;
-    q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
-    int nproj = 0;
-    for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
-    *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2], static_cast<long>(wave[0]) * wave[1] * wave[2],
-                       nproj, static_cast<int>(s->workers.size()));
-    return Q3X_OK;
 }
 
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* ticket) {
-    if (!s || !q || !q->basis || !q->v_r || !q->kpoints || q->nk < 1 || q->nbands < 1 || !ticket) {
-        set_error("q3x_server_submit: invalid argument");
-        return Q3X_ERR_ARGUMENT;
+    try {
+        if (!s || !q || !q->basis || !q->v_r || !q->kpoints || q->nk < 1 || q->nbands < 1 || !ticket) {
+            set_error("q3x_server_submit: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        auto r = std::make_shared<Request>();
+        const q3x_status st = q3x_server_estimate(s, q, &r->bytes);
+        if (st != Q3X_OK) return st;
+        r->basis = q->basis;
+        r->v = copy_of(q->v_r, static_cast<size_t>(q->nv));
+        r->kpts = copy_of(q->kpoints, static_cast<size_t>(q->nk) * 3);
+        r->nk = q->nk;
+        r->natoms = q->natoms;
+        r->nspecies = q->nspecies;
+        r->atom_pos = copy_of(q->atom_pos, static_cast<size_t>(q->natoms) * 3);
+        r->atom_species = copy_of(q->atom_species, static_cast<size_t>(q->natoms));
+        r->nmesh = copy_of(q->nmesh, static_cast<size_t>(q->nspecies));
+        r->nbeta = copy_of(q->nbeta, static_cast<size_t>(q->nspecies));
+        const size_t mesh_total = std::accumulate(r->nmesh.begin(), r->nmesh.end(), size_t(0));
+        size_t beta_total = 0, beta_values = 0;
+        for (int sp = 0; sp < q->nspecies; ++sp) {
+            beta_total += r->nbeta[sp];
+            beta_values += static_cast<size_t>(r->nbeta[sp]) * r->nmesh[sp];
+        }
+        r->r_all = copy_of(q->r_all, mesh_total);
+        r->rab_all = copy_of(q->rab_all, mesh_total);
+        r->l_all = copy_of(q->l_all, beta_total);
+        r->cutoff_all = copy_of(q->cutoff_all, beta_total);
+        r->beta_r_all = copy_of(q->beta_r_all, beta_values);
+        r->block_sizes = copy_of(q->block_sizes, static_cast<size_t>(q->natoms));
+        size_t dvals = 0;
+        for (int b : r->block_sizes) dvals += static_cast<size_t>(b) * b;
+        r->d_blocks = copy_of(q->d_blocks, dvals);
+        r->nb = q->nbands;
+        r->nocc = q->noccupied;
+        r->opt = q->options;
+        r->opt.n_occupied = q->noccupied;
+        r->remaining = q->nk;
+        r->eig.assign(static_cast<size_t>(q->nk) * q->nbands, 0.0);
+        r->psi.assign(q->nk, {nullptr, nullptr});
+        r->psi_bytes.assign(q->nk, 0);
+        std::lock_guard<std::mutex> g(s->mu);
+        r->ticket = s->next_ticket++;
+        r->t_submit = now_s(s->t0);
+        s->all[r->ticket] = r;
+        s->waiting.push_back(r);
+        s->admit_locked();
+        *ticket = r->ticket;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_submit: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    auto r = std::make_shared<Request>();
-    const q3x_status st = q3x_server_estimate(s, q, &r->bytes);
-    if (st != Q3X_OK) return st;
-    r->basis = q->basis;
-    r->v = copy_of(q->v_r, static_cast<size_t>(q->nv));
-    r->kpts = copy_of(q->kpoints, static_cast<size_t>(q->nk) * 3);
-    r->nk = q->nk;
-    r->natoms = q->natoms;
-    r->nspecies = q->nspecies;
-    r->atom_pos = copy_of(q->atom_pos, static_cast<size_t>(q->natoms) * 3);
-    r->atom_species = copy_of(q->atom_species, static_cast<size_t>(q->natoms));
-    r->nmesh = copy_of(q->nmesh, static_cast<size_t>(q->nspecies));
-    r->nbeta = copy_of(q->nbeta, static_cast<size_t>(q->nspecies));
-    const size_t mesh_total = std::accumulate(r->nmesh.begin(), r->nmesh.end(), size_t(0));
-    size_t beta_total = 0, beta_values = 0;
-    for (int sp = 0; sp < q->nspecies; ++sp) {
-        beta_total += r->nbeta[sp];
-        beta_values += static_cast<size_t>(r->nbeta[sp]) * r->nmesh[sp];
-    }
-    r->r_all = copy_of(q->r_all, mesh_total);
-    r->rab_all = copy_of(q->rab_all, mesh_total);
-    r->l_all = copy_of(q->l_all, beta_total);
-    r->cutoff_all = copy_of(q->cutoff_all, beta_total);
-    r->beta_r_all = copy_of(q->beta_r_all, beta_values);
-    r->block_sizes = copy_of(q->block_sizes, static_cast<size_t>(q->natoms));
-    size_t dvals = 0;
-    for (int b : r->block_sizes) dvals += static_cast<size_t>(b) * b;
-    r->d_blocks = copy_of(q->d_blocks, dvals);
-    r->nb = q->nbands;
-    r->nocc = q->noccupied;
-    r->opt = q->options;
-    r->opt.n_occupied = q->noccupied;
-    r->remaining = q->nk;
-    r->eig.assign(static_cast<size_t>(q->nk) * q->nbands, 0.0);
-    r->psi.assign(q->nk, {nullptr, nullptr});
-    r->psi_bytes.assign(q->nk, 0);
-    std::lock_guard<std::mutex> g(s->mu);
-    r->ticket = s->next_ticket++;
-    r->t_submit = now_s(s->t0);
-    s->all[r->ticket] = r;
-    s->waiting.push_back(r);
-    s->admit_locked();
-    *ticket = r->ticket;
-    return Q3X_OK;
 }
 
 q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_request_stats* stats) {
-    if (!s) {
-        set_error("q3x_server_wait: null server");
-        return Q3X_ERR_ARGUMENT;
-    }
-    std::shared_ptr<Request> r;
-    {
-        std::unique_lock<std::mutex> g(s->mu);
-        auto it = s->all.find(ticket);
-        if (it == s->all.end()) {
-            set_error("q3x_server_wait: unknown ticket");
+    try {
+        if (!s) {
+            set_error("q3x_server_wait: null server");
             return Q3X_ERR_ARGUMENT;
         }
-        r = it->second;
-        s->cv_done.wait(g, [&] { return r->done; });
-        s->all.erase(it);
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            auto it = s->all.find(ticket);
+            if (it == s->all.end()) {
+                set_error("q3x_server_wait: unknown ticket");
+                return Q3X_ERR_ARGUMENT;
+            }
+            r = it->second;
+            s->cv_done.wait(g, [&] { return r->done; });
+            s->all.erase(it);
+        }
+        if (eig_out) std::memcpy(eig_out, r->eig.data(), r->eig.size() * sizeof(double));
+        if (stats) {
+            stats->submit_s = r->t_submit;
+            stats->admit_s = r->t_admit;
+            stats->done_s = r->t_done;
+            stats->footprint_bytes = r->bytes;
+            stats->iterations = r->iterations;
+            stats->setup_s = r->setup_s;
+            stats->solve_s = r->solve_s;
+        }
+        {
+            std::lock_guard<std::mutex> g(s->mu);  // the request's wavefunction blocks go back to the pool
+            for (size_t k = 0; k < r->psi.size(); ++k)
+                if (r->psi[k].second) s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
+        }
+        r->psi.clear();
+        if (r->status != Q3X_OK) set_error(r->error);
+        return r->status;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_wait: ") + e.what());
+        return Q3X_ERR_INTERNAL;
     }
-    if (eig_out) std::memcpy(eig_out, r->eig.data(), r->eig.size() * sizeof(double));
-    if (stats) {
-        stats->submit_s = r->t_submit;
-        stats->admit_s = r->t_admit;
-        stats->done_s = r->t_done;
-        stats->footprint_bytes = r->bytes;
-        stats->iterations = r->iterations;
-        stats->setup_s = r->setup_s;
-        stats->solve_s = r->solve_s;
-    }
-    {
-        std::lock_guard<std::mutex> g(s->mu);  // the request's wavefunction blocks go back to the pool
-        for (size_t k = 0; k < r->psi.size(); ++k)
-            if (r->psi[k].second) s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
-    }
-    r->psi.clear();
-    if (r->status != Q3X_OK) set_error(r->error);
-    return r->status;
 }
 
 double q3x_server_in_flight_bytes(q3x_server s) {
