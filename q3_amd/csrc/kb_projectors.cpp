@@ -9,10 +9,12 @@
 // m = 0, +1, -1, +2, -2, ... (harmonics.hpp:40-69).
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <list>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "q3x_internal.h"
@@ -68,17 +70,29 @@ double simpson_rab(const std::vector<double>& f, const double* rab, int n) {
     return s / 3.0;
 }
 
+// Entries are independent (the same arithmetic whatever the thread), so the q points are spread
+// over a few threads joined at the end: the table is built once per cell, on the path of its
+// first k-point.
 std::vector<double> radial_table(const double* r, const double* rab, int cutoff, int l, const double* beta_r,
                                  int nq) {
-    std::vector<double> t(nq), f(cutoff);
-    for (int iq = 0; iq < nq; ++iq) {
-        const double q = iq * kDq;
-        for (int i = 0; i < cutoff; ++i) {
-            const double j = (std::abs(q) < 1e-8) ? (l == 0 ? 1.0 : 0.0) : jl(l, q * r[i]);
-            f[i] = beta_r[i] * r[i] * j;
+    std::vector<double> t(nq);
+    auto part = [&](int q0, int q1) {
+        std::vector<double> f(cutoff);
+        for (int iq = q0; iq < q1; ++iq) {
+            const double q = iq * kDq;
+            for (int i = 0; i < cutoff; ++i) {
+                const double j = (std::abs(q) < 1e-8) ? (l == 0 ? 1.0 : 0.0) : jl(l, q * r[i]);
+                f[i] = beta_r[i] * r[i] * j;
+            }
+            t[iq] = kFourPi * simpson_rab(f, rab, cutoff);
         }
-        t[iq] = kFourPi * simpson_rab(f, rab, cutoff);
-    }
+    };
+    const int nt = std::max(1, std::min({static_cast<int>(std::thread::hardware_concurrency()), 8, nq / 64}));
+    std::vector<std::thread> th;
+    for (int i = 1; i < nt; ++i) th.emplace_back(part, static_cast<int>(static_cast<long>(nq) * i / nt),
+                                                 static_cast<int>(static_cast<long>(nq) * (i + 1) / nt));
+    part(0, static_cast<int>(nq / nt));
+    for (auto& x : th) x.join();
     return t;
 }
 
@@ -145,32 +159,50 @@ namespace {
 using Tables = std::vector<std::vector<std::vector<double>>>;  // [species][beta][q]
 
 // Last few table sets with the inputs they came from (compared in full on lookup); shared by
-// all threads (server and batch workers build projectors concurrently).
+// all threads (server and batch workers build projectors concurrently).  A set being built is
+// listed in `building`, and a thread that needs the same set waits for it instead of building
+// it again (the k-points of one cell reach several threads at once).
 struct TableCache {
     std::mutex mu;
+    std::condition_variable cv;
     std::list<std::pair<std::vector<double>, std::shared_ptr<const Tables>>> entries;
+    std::list<std::vector<double>> building;
 };
 TableCache& table_cache() {
     static TableCache c;
     return c;
 }
 
-std::shared_ptr<const Tables> cached_tables(const std::vector<double>& key) {
+template <typename Build>
+std::shared_ptr<const Tables> tables_for(const std::vector<double>& key, Build build) {
     TableCache& c = table_cache();
-    std::lock_guard<std::mutex> g(c.mu);
-    for (auto it = c.entries.begin(); it != c.entries.end(); ++it)
-        if (it->first == key) {
-            c.entries.splice(c.entries.begin(), c.entries, it);  // most recently used first
-            return c.entries.front().second;
-        }
-    return nullptr;
-}
-
-void store_tables(std::vector<double> key, std::shared_ptr<const Tables> t) {
-    TableCache& c = table_cache();
-    std::lock_guard<std::mutex> g(c.mu);
-    c.entries.emplace_front(std::move(key), std::move(t));
+    std::unique_lock<std::mutex> g(c.mu);
+    for (;;) {
+        for (auto it = c.entries.begin(); it != c.entries.end(); ++it)
+            if (it->first == key) {
+                c.entries.splice(c.entries.begin(), c.entries, it);  // most recently used first
+                return c.entries.front().second;
+            }
+        if (std::find(c.building.begin(), c.building.end(), key) == c.building.end()) break;
+        c.cv.wait(g);
+    }
+    c.building.push_back(key);
+    g.unlock();
+    std::shared_ptr<const Tables> t;
+    try {
+        t = build();
+    } catch (...) {
+        g.lock();
+        c.building.remove(key);
+        c.cv.notify_all();
+        throw;
+    }
+    g.lock();
+    c.building.remove(key);
+    c.entries.emplace_front(key, t);
     while (c.entries.size() > 8) c.entries.pop_back();
+    c.cv.notify_all();
+    return t;
 }
 
 }  // namespace
@@ -241,16 +273,14 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
             key.insert(key.end(), sp[s].rab, sp[s].rab + sp[s].nmesh);
             key.insert(key.end(), sp[s].beta_r, sp[s].beta_r + static_cast<size_t>(sp[s].nbeta) * sp[s].nmesh);
         }
-        std::shared_ptr<const Tables> tab = cached_tables(key);
-        if (!tab) {
+        std::shared_ptr<const Tables> tab = tables_for(key, [&] {
             auto t = std::make_shared<Tables>(nspecies);
             for (int s = 0; s < nspecies; ++s)
                 for (int i = 0; i < sp[s].nbeta; ++i)
                     (*t)[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
                                                    sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
-            store_tables(std::move(key), t);
-            tab = t;
-        }
+            return std::shared_ptr<const Tables>(t);
+        });
         const Tables& table = *tab;
         double kc[3];
         for (int d = 0; d < 3; ++d) kc[d] = K.k[0] * L.b[0][d] + K.k[1] * L.b[1][d] + K.k[2] * L.b[2][d];
