@@ -24,6 +24,13 @@ namespace q3x {
 namespace {
 inline size_t ix(int i, int j, int ld) { return static_cast<size_t>(i) + static_cast<size_t>(j) * ld; }
 
+// Thread team of the Cholesky and triangular inverse of the Davidson orthogonalisation.  These
+// run once per iteration on every solver thread at once (batch and server workers); full-size
+// OpenMP teams from each of them oversubscribe the host, and their spinning threads starve the
+// threads that drive the GPU (64 cfg4 k-points on 8 workers: 0.40 s with one thread per team,
+// 0.74-0.80 s with full teams).  So: serial below n = 128, at most four threads above.
+inline int small_team() { return std::max(1, std::min(4, omp_get_max_threads())); }
+
 // Real and imaginary planes, column-major n x n, in one allocation.  The imaginary plane
 // starts 2 KiB (mod 4 KiB) after the real one: with equal 4 KiB offsets, a load of im[i]
 // right after a store to re[i] is taken by x86 cores as a possible store-to-load forward and
@@ -58,7 +65,7 @@ int chol_split(Split& A) {
         A.cr(j)[j] = d;
         A.ci(j)[j] = 0.0;
         const double inv = 1.0 / d;
-#pragma omp parallel for schedule(static) if (n - j > 96)
+#pragma omp parallel for schedule(static) num_threads(small_team()) if (n - j > 96)
         for (int i = j + 1; i < n; ++i) {
             const double* br = A.cr(i);
             const double* bi = A.ci(i);
@@ -328,7 +335,7 @@ void tri_inv_upper(int n, zc* U, int ldu) {
             Ui[ix(i, j, n)] = U[ix(i, j, ldu)].imag();
         }
     std::vector<double> Xr(static_cast<size_t>(n) * n, 0.0), Xi(static_cast<size_t>(n) * n, 0.0);
-#pragma omp parallel for schedule(dynamic, 4)
+#pragma omp parallel for schedule(dynamic, 4) num_threads(small_team()) if (n >= 128)
     for (int j = 0; j < n; ++j) {
         double* xr = Xr.data() + static_cast<size_t>(j) * n;
         double* xi = Xi.data() + static_cast<size_t>(j) * n;
