@@ -48,3 +48,25 @@ def test_zgemm_beta_zero_ignores_garbage(gpu_context):
     zgemm(gpu_context, "C", 8, 4, 64, 1.0, dA, 64, dB, 64, 0.0, dC, 8)
     got = dC.download((8, 4))
     assert np.allclose(got, A.conj().T @ B, rtol=0, atol=1e-12)
+
+
+def test_split_k_products_in_sequence_repeat_bit_exactly(gpu_context):
+    # Deep-k products whose split-K partials are summed by the last workgroup of each tile:
+    # tile counters are left at zero by every product, so products of different tile counts
+    # run back to back on one stream give the same bits every time, and the right values.
+    shapes = [(96, 96, 1500), (320, 160, 3001), (17, 3, 100000), (96, 96, 1500), (48, 48, 2000)]
+    firsts = {}
+    for rep in range(3):
+        for i, (m, n, k) in enumerate(shapes):
+            A = _rand((k, m), 100 + i)
+            B = _rand((k, n), 200 + i)
+            dA, dB = DeviceArray.from_host(gpu_context, A), DeviceArray.from_host(gpu_context, B)
+            dC = DeviceArray.from_host(gpu_context, np.zeros((m, n), np.complex128, order="F"))
+            zgemm(gpu_context, "C", m, n, k, 1.0, dA, k, dB, k, 0.0, dC, m)
+            got = dC.download((m, n))
+            if i not in firsts:
+                want = A.conj().T @ B
+                assert np.abs(got - want).max() <= 1e-13 * np.abs(A).max() * np.abs(B).max() * np.sqrt(k)
+                firsts[i] = got
+            else:
+                np.testing.assert_array_equal(got, firsts[i])
