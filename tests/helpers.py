@@ -57,3 +57,31 @@ def check_eigenvalues(eig, exact, nocc, nb, j, occupied_bar=2e-8):
         assert d[:nocc - 1].max() <= occupied_bar, d
         assert d[nocc - 1] <= 1e-6, d
     assert d.max() <= 1e-4, d
+
+
+def sine_potential(dims, m0):
+    """V(r) = sin(G0 . r) on the box, r_a = i_a / N_a, G0 with Miller indices m0 (C order)."""
+    i = np.meshgrid(*[np.arange(n) / n for n in dims], indexing="ij")
+    return np.sin(2 * np.pi * sum(m0[a] * i[a] for a in range(3)))
+
+
+def sine_h_closed_form(miller, fft_map, dims, m0, ekin, psi):
+    """T psi + V psi for V = sin(G0 . r) without the FFT: sin = (e^{iG0.r} - e^{-iG0.r}) / 2i has
+    Fourier coefficients V(G0) = -i/2 and V(-G0) = +i/2, so (V psi)(G) = -i/2 psi(G - G0) +
+    i/2 psi(G + G0), where G -+ G0 is taken cyclically on the box (the FFT's own aliasing) and
+    psi is zero off the plane-wave list.  Checks first that fft_map is the C-order box index of
+    the Miller indices, the convention the lookup relies on."""
+    d = np.asarray(dims)
+
+    def box(mm):
+        w = np.mod(mm, d)
+        return (w[:, 0] * d[1] + w[:, 1]) * d[2] + w[:, 2]
+
+    assert np.array_equal(box(miller), fft_map)
+    where = np.full(int(np.prod(d)), -1)
+    where[fft_map] = np.arange(len(fft_map))
+    lo, hi = where[box(miller - np.asarray(m0))], where[box(miller + np.asarray(m0))]
+    out = ekin[:, None] * psi
+    out[lo >= 0] += -0.5j * psi[lo[lo >= 0]]
+    out[hi >= 0] += 0.5j * psi[hi[hi >= 0]]
+    return out

@@ -258,6 +258,20 @@ def test_hamiltonian_apply_constant_potential():
     np.testing.assert_allclose(got, (ek + 0.25)[:, None] * psi, atol=1e-13)
 
 
+@pytest.mark.parametrize("m0", [(1, -2, 3), (0, 3, -1)])
+def test_hamiltonian_apply_single_harmonic_potential(m0):
+    # V = sin(G0 . r) couples G only to G -+ G0 with weights -+ i/2: the sign pins the FFT
+    # directions (FFTW_BACKWARD to r, FFTW_FORWARD back, serial.hpp:456-478), the asymmetric G0
+    # the axis order of the box
+    from tests.helpers import sine_h_closed_form, sine_potential
+    b, gi, ek, fm, m, V, beta, bs, db = _small_case()
+    rng = np.random.default_rng(3)
+    psi = np.asfortranarray(rng.standard_normal((len(gi), 3)) + 1j * rng.standard_normal((len(gi), 3)))
+    got = O.hamiltonian_apply(b.wave_dims, fm, ek, sine_potential(b.wave_dims, m0), None, None, None, psi)
+    want = sine_h_closed_form(m[gi], fm, b.wave_dims, m0, ek, psi)
+    assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.norm(want)
+
+
 def test_davidson_matches_dense_eigh():
     """Davidson vs dense diagonalisation; reference bounds (test_scf_integration.cpp:504-516)
     are occupied <= 1e-6 Ry, empty <= 1e-4 Ry; the oracle is held to 2e-8 Ry (1e-8 Ha) occupied."""
