@@ -116,3 +116,17 @@ def test_cfg2_davidson_fp64_residuals(gpu_context):
     G = psi.conj().T @ psi
     assert np.abs(G - np.eye(160)).max() <= 1e-9
     assert rep["max_residual_norm"] < 1e-2
+
+
+def test_one_band_and_oversized_requests(gpu_context):
+    """nb = 1 converges to the lowest dense eigenvalue; nb > npw is refused with a message."""
+    from q3_amd._lib import Q3XError
+    case = kcase(W.cfg1(), 1)
+    exact = np.linalg.eigvalsh(_dense(case))
+    h, d = _solver(gpu_context, case)
+    d.set_tolerance(1e-10)
+    d.set_maximum_iterations(200)
+    e, _ = d.solve(1, 1)
+    assert abs(e[0] - exact[0]) <= 1e-8 * HA
+    with pytest.raises(Q3XError, match="nb <= npw"):
+        d.solve(case.kp.npw + 1, 1)

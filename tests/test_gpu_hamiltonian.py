@@ -52,6 +52,23 @@ def test_local_part_fp64_cfg1(gpu_context, kindex):
     assert rel_err(got, ref) <= 1e-10
 
 
+def test_ragged_and_empty_band_counts(gpu_context):
+    """Band counts off every internal tile and batch size agree with the oracle column by
+    column; nvec = 0 is a no-op and a negative count is refused."""
+    from q3_amd._lib import Q3XError, lib
+    case = kcase(W.cfg1(), 9)
+    h = _applicator(gpu_context, case)
+    wide = W.random_psi(case.kp.kinetic_energy_squared, 65, seed=21)
+    ref = _oracle(case, wide)
+    for nvec in (1, 2, 7, 33, 65):
+        assert rel_err(h.apply_batch(wide[:, :nvec], FP64), ref[:, :nvec]) <= 1e-12
+        assert rel_err(h.apply_batch(wide[:, :nvec], MIXED), ref[:, :nvec]) <= 2e-6
+    assert h.apply_batch(np.zeros((case.kp.npw, 0), np.complex128)).shape == (case.kp.npw, 0)
+    with pytest.raises(Q3XError):
+        from q3_amd._lib import check
+        check(lib().q3x_hamiltonian_apply(h.h, None, None, -1, FP64))
+
+
 @pytest.mark.parametrize("kindex", [0, 21])
 def test_full_h_fp64_cfg1(gpu_context, kindex):
     case = kcase(W.cfg1(), kindex)
