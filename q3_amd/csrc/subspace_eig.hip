@@ -464,8 +464,16 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
         double tk = 0.0, tn = tnext;
         z2* v = vb1;   // reflector k (not yet defined at k = -1)
         z2* vn = vb0;  // reflector k+1
+        // Once the trailing block has at most 64 rows, one row per lane covers it: the passes and
+        // column steps drop the second row set, which would only add masked zeros.
         for (int k = -1; k <= n - 3; ++k) {
-            if (tk != 0.0 || tn != 0.0) fused_pass<RPL, CB, LM>(M, n, k, tk, v, q, tn, vn, p);
+            const bool one_row = RPL > 1 && n - k - 2 <= 64;
+            if (tk != 0.0 || tn != 0.0) {
+                if (one_row)
+                    fused_pass<1, 4, LM>(M, n, k, tk, v, q, tn, vn, p);
+                else
+                    fused_pass<RPL, CB, LM>(M, n, k, tk, v, q, tn, vn, p);
+            }
             __syncthreads();
             const int len = n - k - 2;
             if (wv == 0) {  // f = tn vn^H p / 2, q = p - f vn, next column and reflector: one wavefront
@@ -477,7 +485,8 @@ __global__ __launch_bounds__(kThreads) void tridiag_kernel(z2* A, int n, double*
                     f = 0.5 * tn * wave_sum(part);
                     for (int i = lane; i < len; i += 64) q[i] = {p[i].x - f * vn[i].x, p[i].y - f * vn[i].y};
                 }
-                const double tj = column_step<RPL, LM>(M, n, k + 2, tn, vn, p, f, v, sd, sbeta, stau);
+                const double tj = one_row ? column_step<1, LM>(M, n, k + 2, tn, vn, p, f, v, sd, sbeta, stau)
+                                          : column_step<RPL, LM>(M, n, k + 2, tn, vn, p, f, v, sd, sbeta, stau);
                 if (lane == 0) tnext = tj;
             }
             __syncthreads();
