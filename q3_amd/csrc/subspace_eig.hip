@@ -88,8 +88,12 @@ __device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
         const bool up = (lane & o) != 0;
 #pragma unroll
         for (int i = 0; i < h; ++i) {
-            const double send = up ? x[i] : x[i + h];
-            const double keep = up ? x[i + h] : x[i];
+            // both values in registers before the select: a select between two loads of x is
+            // otherwise folded into one load at a lane-dependent index, which moves x to scratch
+            double lo = x[i], hi = x[i + h];
+            asm volatile("" : "+v"(lo), "+v"(hi));
+            const double send = up ? lo : hi;
+            const double keep = up ? hi : lo;
             x[i] = keep + __shfl_xor(send, o, 64);
         }
     }
