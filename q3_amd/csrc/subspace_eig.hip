@@ -97,9 +97,16 @@ __device__ __forceinline__ double wave_multi_sum(double (&x)[V], int lane) {
             x[i] = keep + __shfl_xor(send, o, 64);
         }
     }
+    // the remaining plain sums: across rows by shuffles, within a row of 16 by DPP moves (row
+    // mirror, half-row mirror, quad permutes: each pairs the lanes of the group still to be
+    // summed, in the VALU, without an LDS round trip)
     double s = x[0];
 #pragma unroll
-    for (; o >= 1; o /= 2) s += __shfl_xor(s, o, 64);
+    for (; o >= 16; o /= 2) s += __shfl_xor(s, o, 64);
+    if (o >= 8) s += dpp_mov<0x140>(s);
+    if (o >= 4) s += dpp_mov<0x141>(s);
+    if (o >= 2) s += dpp_mov<0x4E>(s);
+    if (o >= 1) s += dpp_mov<0xB1>(s);
     return s;
 }
 __device__ __forceinline__ double wave_max(double v) {
