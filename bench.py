@@ -444,6 +444,13 @@ def main():
             davidson = {"error": str(e)}
             log(f"davidson failed: {e}")
 
+    # the concurrent sections below run on contexts of their own: release this one (its stream,
+    # applicator tables and device buffers) so that they start from the same state as when run alone
+    L.q3x_device_free(ctx.h, d_psi)
+    L.q3x_device_free(ctx.h, d_hpsi)
+    d = h = None
+    ctx.close()
+
     batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args)}
     serve = None if args.no_serve else serving(args)
 
@@ -491,8 +498,6 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    L.q3x_device_free(ctx.h, d_psi)
-    L.q3x_device_free(ctx.h, d_hpsi)
     return 0
 
 
