@@ -280,7 +280,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     if (!s.V.ensure(col * mmax) || !s.HV.ensure(col * mmax) || !s.R.ensure(col * std::max(nb, 1)) ||
         !s.C.ensure(mm16) || !s.G.ensure(mm16) || !s.Hs_d.ensure(mm16) || !s.Ss_d.ensure(mm16) ||
         !s.X_d.ensure(mm16) || !s.Xc_d.ensure(static_cast<size_t>(mmax) * nb * 32) ||
-        !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) || !s.negeps_d.ensure(static_cast<size_t>(mmax) * 8) ||
+        !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) || !s.negeps_d.ensure(static_cast<size_t>(mmax) * 20) ||
         !s.norms_d.ensure(static_cast<size_t>(mmax) * 8) || !s.idx_d.ensure(static_cast<size_t>(mmax) * 4) ||
         !s.work.ensure(s.work_bytes) || !s.eigw.ensure(subspace_eig_workspace(mmax, mmax)))
         return Q3X_ERR_DEVICE;
@@ -390,27 +390,31 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                 }
                 double* corr = s.Vc(basis);
                 double* xc = s.Xc_d.as<double>();
-                if (!s.ok(s.pin().h2d(s.idx_d.get(), src.data(), nc * 4, s.st), "H2D idx") ||
-                    !s.ok(s.pin().h2d(s.eps_d.get(), ec.data(), nc * 8, s.st), "H2D eps"))
+                // e, -e and the band indices of the corrections in one upload: [e | -e | idx]
+                std::vector<char> packed(static_cast<size_t>(nc) * 20);
+                std::memcpy(packed.data(), ec.data(), static_cast<size_t>(nc) * 8);
+                std::memcpy(packed.data() + static_cast<size_t>(nc) * 8, nec.data(), static_cast<size_t>(nc) * 8);
+                std::memcpy(packed.data() + static_cast<size_t>(nc) * 16, src.data(), static_cast<size_t>(nc) * 4);
+                double* eps_c = s.negeps_d.as<double>();
+                double* negeps_c = eps_c + nc;
+                const int* idx_c = reinterpret_cast<const int*>(eps_c + 2L * nc);
+                if (!s.ok(s.pin().h2d(s.negeps_d.get(), packed.data(), packed.size(), s.st), "H2D e, -e, idx"))
                     return s.status;
                 if (x_identity) {
-                    if (!s.ok(launch_residual_columns_teter(npw, nc, s.idx_d.as<int>(), s.eps_d.as<double>(),
-                                                            h->ekin.as<double>(), s.Vc(0), npw, s.HVc(0), npw, corr,
-                                                            npw, s.st),
+                    if (!s.ok(launch_residual_columns_teter(npw, nc, idx_c, eps_c, h->ekin.as<double>(), s.Vc(0), npw,
+                                                            s.HVc(0), npw, corr, npw, s.st),
                               "corrections"))
                         return s.status;
-                } else if (!s.ok(s.pin().h2d(s.negeps_d.get(), nec.data(), nc * 8, s.st), "H2D -eps") ||
-                           !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), nullptr, s.X_d.as<double>(),
-                                                       basis, xc, basis, s.st),
+                } else if (!s.ok(launch_gather_columns(basis, nc, idx_c, nullptr, s.X_d.as<double>(), basis, xc, basis,
+                                                       s.st),
                                  "gather X") ||
-                           !s.ok(launch_gather_columns(basis, nc, s.idx_d.as<int>(), s.negeps_d.as<double>(),
-                                                       s.X_d.as<double>(), basis, xc + 2L * basis * nc, basis, s.st),
+                           !s.ok(launch_gather_columns(basis, nc, idx_c, negeps_c, s.X_d.as<double>(), basis,
+                                                       xc + 2L * basis * nc, basis, s.st),
                                  "gather -e X") ||
                            !s.gemm(0, npw, nc, basis, 1.0, s.Vc(0), npw, xc + 2L * basis * nc, basis, 0.0, corr,
                                    npw) ||
                            !s.gemm(0, npw, nc, basis, 1.0, s.HVc(0), npw, xc, basis, 1.0, corr, npw) ||
-                           !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), s.eps_d.as<double>(), corr, npw, s.st),
-                                 "Teter")) {
+                           !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), eps_c, corr, npw, s.st), "Teter")) {
                     return s.status;
                 }
             }
