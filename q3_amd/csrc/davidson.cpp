@@ -67,8 +67,10 @@ struct Solver {
     bool stage_sync() { return !timing || sync(); }
     q3x_status status = Q3X_OK;
 
-    double* Vc(int c) { return V.as<double>() + 2L * c * npw; }
-    double* HVc(int c) { return HV.as<double>() + 2L * c * npw; }
+    // Current basis and its H image, and the other buffers of the ping-pong (rotate() swaps them)
+    double *vbase = nullptr, *hvbase = nullptr, *valt = nullptr, *hvalt = nullptr;
+    double* Vc(int c) { return vbase + 2L * c * npw; }
+    double* HVc(int c) { return hvbase + 2L * c * npw; }
 
     bool ok(hipError_t e, const char* what) {
         if (hip_ok(e, what)) return true;
@@ -187,14 +189,14 @@ struct Solver {
         return accepted;
     }
 
-    // V[:, :ncols] <- V[:, :n] X[:, :ncols], same for HV (serial.hpp:825-945); X = X_d (ld n)
+    // V[:, :ncols] <- V[:, :n] X[:, :ncols], same for HV (serial.hpp:825-945); X = X_d (ld n).
+    // The products go to the other buffers of the ping-pong, which then become the basis.
     bool rotate(int n, int ncols) {
-        for (double* base : {Vc(0), HVc(0)}) {
-            if (!gemm(0, npw, ncols, n, 1.0, base, npw, X_d.get(), n, 0.0, R.get(), npw)) return false;
-            if (!ok(hipMemcpyAsync(base, R.get(), static_cast<size_t>(npw) * ncols * 16, hipMemcpyDeviceToDevice, st),
-                    "D2D rotated"))
-                return false;
-        }
+        if (!gemm(0, npw, ncols, n, 1.0, vbase, npw, X_d.get(), n, 0.0, valt, npw) ||
+            !gemm(0, npw, ncols, n, 1.0, hvbase, npw, X_d.get(), n, 0.0, hvalt, npw))
+            return false;
+        std::swap(vbase, valt);
+        std::swap(hvbase, hvalt);
         return stage_sync();
     }
 
@@ -277,13 +279,18 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     const double t_start = s.now_ms();
     const size_t col = static_cast<size_t>(npw) * 16;
     const size_t mm16 = static_cast<size_t>(mmax) * mmax * 16;
-    if (!s.V.ensure(col * mmax) || !s.HV.ensure(col * mmax) || !s.R.ensure(col * std::max(nb, 1)) ||
+    if (!s.V.ensure(col * mmax) || !s.HV.ensure(col * mmax) || !s.ws.V2.ensure(col * mmax) ||
+        !s.ws.HV2.ensure(col * mmax) || !s.R.ensure(col * std::max(nb, 1)) ||
         !s.C.ensure(mm16) || !s.G.ensure(mm16) || !s.Hs_d.ensure(mm16) || !s.Ss_d.ensure(mm16) ||
         !s.X_d.ensure(mm16) || !s.Xc_d.ensure(static_cast<size_t>(mmax) * nb * 32) ||
         !s.eps_d.ensure(static_cast<size_t>(mmax) * 8) || !s.negeps_d.ensure(static_cast<size_t>(mmax) * 20) ||
         !s.norms_d.ensure(static_cast<size_t>(mmax) * 8) || !s.idx_d.ensure(static_cast<size_t>(mmax) * 4) ||
         !s.work.ensure(s.work_bytes) || !s.eigw.ensure(subspace_eig_workspace(mmax, mmax)))
         return Q3X_ERR_DEVICE;
+    s.vbase = s.V.as<double>();
+    s.hvbase = s.HV.as<double>();
+    s.valt = s.ws.V2.as<double>();
+    s.hvalt = s.ws.HV2.as<double>();
     int prec = opt->precision;
 
     // ---- seed (davidson.hpp:1000-1101)

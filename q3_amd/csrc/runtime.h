@@ -94,7 +94,8 @@ struct FftTables {
 // davidson.hpp:120-221), kept by the context and grown on demand, so that repeated solves on a
 // context allocate nothing once the largest size has been seen.
 struct DavidsonWorkspace {
-    DevBuf V, HV, R, C, G, Hs, Ss, X, Xc, eps, negeps, work, norms, idx, eigw;
+    // V2 / HV2: the other half of the basis ping-pong (a rotation writes V X there and swaps)
+    DevBuf V, HV, V2, HV2, R, C, G, Hs, Ss, X, Xc, eps, negeps, work, norms, idx, eigw;
 };
 
 }  // namespace q3x

@@ -284,7 +284,8 @@ std::vector<T> copy_of(const T* p, size_t n) {
 }
 
 // Footprint of one request in bytes: its wavefunctions (kept for all k-points) plus, for each
-// k-point that may run at once, the Davidson basis V and HV, the rotation buffer, the subspace
+// k-point that may run at once, the Davidson basis V and HV twice (the rotation ping-pongs
+// between two copies), the orthogonalisation buffer, the subspace
 // matrices and eigensolver workspace, the projectors (complex128 + complex64), the z-stick
 // buffer and the potential -- the terms of PlanewaveMemoryEstimate::compute
 // (memory_estimate.hpp:75-215) for this path.
@@ -292,7 +293,7 @@ double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int n
     const double c = 16.0, nb = q->nbands, m = std::max(std::min(2.0 * nb, static_cast<double>(npw)), nb);
     const double psi = static_cast<double>(q->nk) * nb * npw * c;
     const double sticks = std::max(1.0, static_cast<double>(npw) / std::max(1, n3)) * 1.6;  // z-sticks
-    const double per_k = 2 * m * npw * c + nb * npw * c + 30.0 * m * m * c + 1.5 * nproj * npw * c +
+    const double per_k = 4 * m * npw * c + nb * npw * c + 30.0 * m * m * c + 1.5 * nproj * npw * c +
                          nb * n3 * sticks * c + 12.0 * npoints;
     return psi + per_k * std::min(q->nk, workers);
 }
