@@ -769,13 +769,24 @@ __global__ void zero_words(unsigned* a, int n) {
 // ---------------------------------------------------------------- 4. eigenvalues ----
 // Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x (Sturm count of the LDL^T
 // pivots; pivots smaller than pivmin are replaced by -pivmin, as LAPACK's dstebz does).
+// a / b to about an ulp: the hardware reciprocal refined by two Newton steps, then one product.
+// The Sturm recurrence's backward error bound needs each operation's relative error to be
+// O(eps), not a correctly rounded quotient, and this chain is shorter than the full division
+// sequence on the serial path of the count.
+__device__ __forceinline__ double div_ulp(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return a * r;
+}
+
 __device__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
     int c = 0;
     double q = d[0] - x;
     if (fabs(q) <= pivmin) q = -pivmin;
     c += q < 0.0;
     for (int i = 1; i < n; ++i) {
-        q = (d[i] - x) - e2[i] / q;
+        q = (d[i] - x) - div_ulp(e2[i], q);
         if (fabs(q) <= pivmin) q = -pivmin;
         c += q < 0.0;
     }
