@@ -924,10 +924,12 @@ __global__ __launch_bounds__(kInvitThreads) void invit_kernel(int n, const doubl
                 // smallest, then z_r = 1 and the two one-term recurrences outwards.  One pass
                 // each way instead of repeated LU solves; for an eigenvalue apart from the others
                 // the vector is accurate to about eps |T| / gap.  U1 holds L+, U2 D+, U3 U-.
+                // The divisions are div_ulp's: the recurrences need O(eps) relative error per
+                // operation, and they are the serial chain of this kernel.
                 double dp = d[0] - mu;
                 for (int i = 0; i + 1 < n; ++i) {
                     if (fabs(dp) < pivmin) dp = dp >= 0.0 ? pivmin : -pivmin;
-                    const double l = e[i + 1] / dp;
+                    const double l = div_ulp(e[i + 1], dp);
                     U1[at(i)] = l;
                     U2[at(i)] = dp;
                     dp = (d[i + 1] - mu) - l * e[i + 1];
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kInvitThreads) void invit_kernel(int n, const doubl
                         const int i = i0 - k;
                         if (i < 1) break;
                         if (fabs(dm) < pivmin) dm = dm >= 0.0 ? pivmin : -pivmin;
-                        const double u = e[i] / dm;
+                        const double u = div_ulp(e[i], dm);
                         U3[at(i)] = u;
                         dm = (d[i - 1] - mu) - u * e[i];
                         const double g = fabs(dpb[k] + dm - (d[i - 1] - mu));
