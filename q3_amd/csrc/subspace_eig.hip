@@ -54,6 +54,17 @@ __device__ __forceinline__ double dpp_mov(double v) {
     const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
     return __hiloint2double(hi, lo);
 }
+// a / b to about an ulp: the hardware reciprocal refined by two Newton steps, then one product.
+// The recurrences that use it (Householder reflectors, Sturm counts, twisted factorisations)
+// need each operation's relative error to be O(eps), not a correctly rounded quotient, and
+// this chain is shorter than the full division sequence on their serial paths.
+__device__ __forceinline__ double div_ulp(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(r, fma(-b, r, 1.0), r);
+    r = fma(r, fma(-b, r, 1.0), r);
+    return a * r;
+}
+
 __device__ __forceinline__ double read_lane(double v, int lane) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
                             __builtin_amdgcn_readlane(__double2loint(v), lane));
@@ -319,11 +330,12 @@ __device__ double column_step(z2* M, int n, int j, double tn, const z2* vn, cons
     } else {
         const double a0 = hypot(x0.x, x0.y);
         const double xnorm = sqrt(tail + a0 * a0);
-        const z2 ph = a0 == 0.0 ? z2{1.0, 0.0} : z2{x0.x / a0, x0.y / a0};
+        const double ia0 = a0 == 0.0 ? 0.0 : div_ulp(1.0, a0);
+        const z2 ph = a0 == 0.0 ? z2{1.0, 0.0} : z2{x0.x * ia0, x0.y * ia0};
         const z2 b = {-ph.x * xnorm, -ph.y * xnorm};
         const z2 u0 = {x0.x - b.x, x0.y - b.y};
-        const double iu2 = 1.0 / (u0.x * u0.x + u0.y * u0.y);
-        tj = 2.0 / (1.0 + tail * iu2);
+        const double iu2 = div_ulp(1.0, u0.x * u0.x + u0.y * u0.y);
+        tj = div_ulp(2.0, 1.0 + tail * iu2);
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             const int i = lane + 64 * t;
@@ -769,17 +781,6 @@ __global__ void zero_words(unsigned* a, int n) {
 // ---------------------------------------------------------------- 4. eigenvalues ----
 // Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x (Sturm count of the LDL^T
 // pivots; pivots smaller than pivmin are replaced by -pivmin, as LAPACK's dstebz does).
-// a / b to about an ulp: the hardware reciprocal refined by two Newton steps, then one product.
-// The Sturm recurrence's backward error bound needs each operation's relative error to be
-// O(eps), not a correctly rounded quotient, and this chain is shorter than the full division
-// sequence on the serial path of the count.
-__device__ __forceinline__ double div_ulp(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    r = fma(r, fma(-b, r, 1.0), r);
-    r = fma(r, fma(-b, r, 1.0), r);
-    return a * r;
-}
-
 __device__ int sturm_count(int n, const double* d, const double* e2, double x, double pivmin) {
     int c = 0;
     double q = d[0] - x;
