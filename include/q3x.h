@@ -98,13 +98,31 @@ q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, const double
                              const int* l_all, const int* cutoff_all, const double* beta_r_all,
                              double* beta_out, int* nproj);
 
+/* ---- per-atom D blocks of the nonlocal term, in the layout q3x_hamiltonian_set_projectors
+ *      takes (NormConservingPlanewaveKohnShamQuantumSystem::d_matrix_element,
+ *      src/layer_3_representation/quantum_system/kohn_sham/planewave/norm_conserving/init.hpp:84-100,
+ *      packed per atom as hamiltonian_applicator/serial.hpp:603-640 packs them).  Channels of a
+ *      species follow the projector order of q3x_kb_projectors (beta, then m = 0, +1, -1, ...);
+ *      D_ab = dion[beta_a][beta_b] when channels a and b have the same (l, m), else 0, dropped when
+ *      |D| <= 1e-15 (tolerances.hpp:99).  dion_all: per species nbeta[s] x nbeta[s] in Ry,
+ *      row-major, concatenated.  block_sizes (natoms) and d_blocks (column-major blocks packed in
+ *      atom order) may be NULL; *packed_len returns the packed length.  Host-only. */
+q3x_status q3x_kb_d_blocks(int natoms, const int* atom_species, int nspecies, const int* nbeta,
+                           const int* l_all, const double* dion_all, int* block_sizes, double* d_blocks,
+                           long* packed_len);
+
 /* ---- Hamiltonian applicator (PlanewaveHamiltonianApplicator,
  *      src/layer_4_computation/diagonalizer/planewave/internal/hamiltonian_applicator/init.hpp,
  *      serial semantics of .../serial.hpp:64-155) */
 q3x_status q3x_hamiltonian_create(q3x_context ctx, q3x_basis b, q3x_hamiltonian* out);
 q3x_status q3x_hamiltonian_destroy(q3x_hamiltonian h);
-/* update_potential (serial.hpp:64-78): V(r) on the wave grid, Ry, C order (i1,i2,i3), n points.
- * The library keeps V/N, as the reference does. */
+/* update_potential (serial.hpp:64-78): V(r) in Ry, C order (i1,i2,i3), on the density grid
+ * (n = its n1*n2*n3 points, the reference's real_space_potential).  When the wavefunction grid is
+ * smaller, V is restricted to it in G space on the device (compute_smooth_grid_potential,
+ * serial.hpp:852-872, and planewave_potential_utils.hpp:33-130: forward FFT / N_dense, keep the
+ * smooth box's G, backward FFT, real part); when the grids coincide V is taken as is.  V already
+ * on the wavefunction grid (n = its point count) is accepted too.  The library keeps V_smooth/N,
+ * as the reference does. */
 q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v_r, long n);
 /* prepare_kpoint (serial.hpp:84-103) */
 q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k);

@@ -58,6 +58,9 @@ def lib():
         L.orc_sph_bessel_batch.argtypes = [_i, _d, _p, _i, _p]
         L.orc_kb_projectors.restype = _i
         L.orc_kb_projectors.argtypes = [_p, _p, _i, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p]
+        L.orc_restrict_potential.argtypes = [_p, _p, _p, _p]
+        L.orc_d_blocks.restype = ctypes.c_long
+        L.orc_d_blocks.argtypes = [_i, _p, _i, _p, _p, _p, _p, _p]
         L.orc_zhegv.restype = _i
         L.orc_zhegv.argtypes = [_i, _p, _p, _p, _p]
         L.orc_davidson.restype = _i
@@ -228,6 +231,32 @@ def hamiltonian_apply(dims, fftmap, ekin, V, beta, block_sizes, dblocks, psi):
     lib().orc_hamiltonian_apply(_ptr(dims), npw, _ptr(fm), _ptr(ek), _ptr(Vr), nproj, _ptr(b), len(bs),
                                 _ptr(bs), _ptr(db), nvec, _ptr(psi), _ptr(out))
     return out
+
+
+def restrict_potential(v_dense, smooth_dims):
+    """V on the density grid (C order) restricted to the smooth box in G space
+    (planewave_potential_utils.hpp:33-130)."""
+    v = _c(v_dense, np.float64)
+    dd = _c(v.shape, np.int32)
+    sd = _c(smooth_dims, np.int32)
+    out = np.zeros(tuple(int(x) for x in smooth_dims))
+    lib().orc_restrict_potential(_ptr(dd), _ptr(v), _ptr(sd), _ptr(out))
+    return out
+
+
+def d_blocks(atom_species, species):
+    """Per-atom D blocks (sizes, packed column-major blocks) of the nonlocal term
+    (d_matrix_element, norm_conserving/init.hpp:84-100)."""
+    asp = _c(atom_species, np.int32)
+    nbeta = _c([len(s["l"]) for s in species], np.int32)
+    l_all = _c(np.concatenate([np.asarray(s["l"]) for s in species]), np.int32)
+    dion = _c(np.concatenate([np.asarray(s["dion_ry"], np.float64).ravel() for s in species]), np.float64)
+    n = lib().orc_d_blocks(len(asp), _ptr(asp), len(species), _ptr(nbeta), _ptr(l_all), _ptr(dion), None, None)
+    sizes = np.zeros(len(asp), np.int32)
+    out = np.zeros(max(n, 1))
+    lib().orc_d_blocks(len(asp), _ptr(asp), len(species), _ptr(nbeta), _ptr(l_all), _ptr(dion), _ptr(sizes),
+                       _ptr(out))
+    return sizes, out[:n]
 
 
 def zhegv(H, S):

@@ -1075,6 +1075,95 @@ int orc_kb_projectors(void* hb, const double* k, int npw, const int* gidx, int n
     return col;
 }
 
+// Per-atom D blocks of the nonlocal term.  Channels of a species are enumerated beta by beta,
+// m in the order 0, +1, -1, +2, -2, ... (projector_mapping_table.hpp build(), with
+// harmonic_order_sequence / harmonic_combined_degree_order_index, harmonics.hpp:40-69);
+// D_ab = dion[beta(a)][beta(b)] when the combined (l,m) indices of a and b agree, else 0
+// (d_matrix_element, quantum_system/.../norm_conserving/init.hpp:84-100), kept only when
+// |D_ab| > 1e-15 (serial.hpp:626-640, tolerances.hpp:99).  Blocks are column-major
+// (a + b * size) and packed in atom order (serial.hpp:603-624).
+// dion_all: per species nbeta[s] x nbeta[s], row-major (dion[i][j] at i * nbeta + j).
+// block_sizes (natoms) and d_out may be null; returns the packed length.
+long orc_d_blocks(int natoms, const int* atom_species, int nspecies, const int* nbeta, const int* lbeta_all,
+                  const double* dion_all, int* block_sizes, double* d_out) {
+    std::vector<std::vector<int>> beta_of(nspecies), lm_of(nspecies);
+    std::vector<const double*> dion(nspecies);
+    {
+        size_t bo = 0, dof = 0;
+        for (int s = 0; s < nspecies; s++) {
+            dion[s] = dion_all + dof;
+            for (int ib = 0; ib < nbeta[s]; ib++) {
+                int l = lbeta_all[bo + ib];
+                std::vector<int> ms{0};
+                for (int m = 1; m <= l; m++) {
+                    ms.push_back(m);
+                    ms.push_back(-m);
+                }
+                for (int m : ms) {
+                    beta_of[s].push_back(ib);
+                    lm_of[s].push_back(lm_index(l, m));
+                }
+            }
+            bo += nbeta[s];
+            dof += static_cast<size_t>(nbeta[s]) * nbeta[s];
+        }
+    }
+    long off = 0;
+    for (int a = 0; a < natoms; a++) {
+        int s = atom_species[a];
+        int bs = static_cast<int>(beta_of[s].size());
+        if (block_sizes) block_sizes[a] = bs;
+        for (int j = 0; j < bs; j++)
+            for (int i = 0; i < bs; i++) {
+                double v = 0.0;
+                if (lm_of[s][i] == lm_of[s][j]) v = dion[s][beta_of[s][i] * nbeta[s] + beta_of[s][j]];
+                if (!(std::abs(v) > kDThreshold)) v = 0.0;
+                if (d_out) d_out[off + i + static_cast<long>(j) * bs] = v;
+            }
+        off += static_cast<long>(bs) * bs;
+    }
+    return off;
+}
+
+// Restriction of a density-grid potential to the wavefunction grid
+// (compute_smooth_grid_potential_from_dense + inverse_fft_smooth_potential,
+// planewave_potential_utils.hpp:33-130): forward FFT of V on the dense box, times 1/N_dense;
+// every wrapped smooth index w takes the dense coefficient of Miller index m = w (w <= n/2) or
+// w - n; backward FFT (unnormalised) on the smooth box; real part.  Equal boxes: V itself
+// (serial.hpp:859-861).
+void orc_restrict_potential(const int* dd, const double* vd, const int* sd, double* vs) {
+    const long nd = static_cast<long>(dd[0]) * dd[1] * dd[2];
+    const long ns = static_cast<long>(sd[0]) * sd[1] * sd[2];
+    if (dd[0] == sd[0] && dd[1] == sd[1] && dd[2] == sd[2]) {
+        std::memcpy(vs, vd, sizeof(double) * nd);
+        return;
+    }
+    std::vector<cplx> dense(nd), smooth(ns, cplx(0, 0));
+    for (long i = 0; i < nd; i++) dense[i] = cplx(vd[i], 0.0);
+    Fft3d fd;
+    fd.init(dd);
+    fd.run(dense.data(), -1);
+    const double inv = 1.0 / static_cast<double>(nd);
+    for (long i = 0; i < nd; i++) dense[i] *= inv;
+    for (int w1 = 0; w1 < sd[0]; w1++) {
+        int m1 = w1 <= sd[0] / 2 ? w1 : w1 - sd[0];
+        for (int w2 = 0; w2 < sd[1]; w2++) {
+            int m2 = w2 <= sd[1] / 2 ? w2 : w2 - sd[1];
+            for (int w3 = 0; w3 < sd[2]; w3++) {
+                int m3 = w3 <= sd[2] / 2 ? w3 : w3 - sd[2];
+                // fft_grid_serial.hpp:301-307 on the dense box
+                long di = (static_cast<long>(m1 >= 0 ? m1 : m1 + dd[0]) * dd[1] + (m2 >= 0 ? m2 : m2 + dd[1])) * dd[2] +
+                          (m3 >= 0 ? m3 : m3 + dd[2]);
+                smooth[(static_cast<long>(w1) * sd[1] + w2) * sd[2] + w3] = dense[di];
+            }
+        }
+    }
+    Fft3d fs;
+    fs.init(sd);
+    fs.run(smooth.data(), +1);
+    for (long i = 0; i < ns; i++) vs[i] = smooth[i].real();
+}
+
 // zhegv (see above).  H, S: n x n column-major interleaved complex.
 int orc_zhegv(int n, const double* H, const double* S, double* w, double* X) {
     return zhegv(n, reinterpret_cast<const cplx*>(H), n, reinterpret_cast<const cplx*>(S), n, w,

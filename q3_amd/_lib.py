@@ -58,6 +58,7 @@ SIGNATURES = {
     "q3x_kpoint_random_psi": (_i, [_p, _i, ctypes.c_ulonglong, _p]),
     "q3x_kpoint_destroy": (_i, [_p]),
     "q3x_kb_projectors": (_i, [_p, _p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "q3x_kb_d_blocks": (_i, [_i, _p, _i, _p, _p, _p, _p, _p, ctypes.POINTER(_l)]),
     "q3x_hamiltonian_create": (_i, [_p, _p, _pp]),
     "q3x_hamiltonian_destroy": (_i, [_p]),
     "q3x_hamiltonian_set_potential": (_i, [_p, _p, _l]),

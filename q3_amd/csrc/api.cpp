@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "aux_kernels.h"
+#include "potential.h"
 #include "runtime.h"
 #include "subspace_eig.h"
 #include "zgemm.h"
@@ -328,6 +329,26 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
 using namespace q3x;
 
 namespace q3x {
+double device_bytes_held(const q3x_context_s* c, const q3x_hamiltonian_s* h) {
+    double b = 0.0;
+    if (c) {
+        const DavidsonWorkspace& w = c->dav;
+        for (const DevBuf* d : {&c->gemm_work, &c->eig_work, &w.V, &w.HV, &w.V2, &w.HV2, &w.R, &w.C, &w.G, &w.Hs, &w.Ss,
+                                &w.X, &w.Xc, &w.eps, &w.negeps, &w.work, &w.norms, &w.idx, &w.eigw})
+            b += static_cast<double>(d->size());
+    }
+    if (h) {
+        for (const DevBuf* d : {&h->v64, &h->v32, &h->stick_off, &h->stick_i2, &h->stick_col, &h->col_i1, &h->ent_pw,
+                                &h->ent_pos, &h->ekin, &h->outside, &h->beta64, &h->beta32, &h->dblocks, &h->pstart,
+                                &h->psize, &h->poff, &h->zbuf, &h->P, &h->W, &h->gemm_work, &h->io_psi, &h->io_hpsi,
+                                &h->plane_slots})
+            b += static_cast<double>(d->size());
+        for (const FftTables& t : h->fft)
+            b += static_cast<double>(t.tw64.size() + t.tw32.size() + t.plain64.size() + t.plain32.size());
+    }
+    return b;
+}
+
 q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b) {
     if (!h || !b) {
         set_error("hamiltonian rebind: invalid argument");
@@ -616,9 +637,46 @@ q3x_status q3x_hamiltonian_destroy(q3x_hamiltonian h) {
 
 q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v, long n) {
     try {
-        if (!h || !v || n != h->npoints) {
-            set_error("set_potential: V must hold n1*n2*n3 wave-grid points");
+        if (!h || !v) {
+            set_error("set_potential: invalid argument");
             return Q3X_ERR_ARGUMENT;
+        }
+        const int* rd = h->basis->d.rho;
+        const long nrho = static_cast<long>(rd[0]) * rd[1] * rd[2];
+        const bool same_grid = rd[0] == h->dims[0] && rd[1] == h->dims[1] && rd[2] == h->dims[2];
+        if (n != nrho && n != h->npoints) {
+            set_error("set_potential: V must hold the density grid's n1*n2*n3 points (or the wavefunction grid's)");
+            return Q3X_ERR_ARGUMENT;
+        }
+        hipStream_t st = h->ctx->stream;
+        std::vector<double> restricted;
+        const double* vs = v;  // V on the wavefunction grid
+        if (n == nrho && !same_grid) {
+            // compute_smooth_grid_potential (serial.hpp:852-872): G-space restriction on the device
+            FftTables dense[3];
+            for (int d = 0; d < 3; ++d)
+                if (!build_tables(dense[d], rd[d], st, &h->ctx->pin)) return Q3X_ERR_DEVICE;
+            DevBuf vd, wd, ws, vo;
+            if (!vd.ensure(static_cast<size_t>(nrho) * 8) || !wd.ensure(static_cast<size_t>(nrho) * 16) ||
+                !ws.ensure(static_cast<size_t>(h->npoints) * 16) || !vo.ensure(static_cast<size_t>(h->npoints) * 8))
+                return Q3X_ERR_DEVICE;
+            if (!hip_ok(hipMemcpyAsync(vd.get(), v, static_cast<size_t>(nrho) * 8, hipMemcpyHostToDevice, st), "H2D V"))
+                return Q3X_ERR_DEVICE;
+            const FftPlan dp[3] = {dense[0].plan, dense[1].plan, dense[2].plan};
+            const FftPlan sp[3] = {h->fft[0].plan, h->fft[1].plan, h->fft[2].plan};
+            const void* dtw[3] = {dense[0].tw64.get(), dense[1].tw64.get(), dense[2].tw64.get()};
+            const void* stw[3] = {h->fft[0].tw64.get(), h->fft[1].tw64.get(), h->fft[2].tw64.get()};
+            if (!hip_ok(launch_restrict_potential(vd.as<double>(), rd, dp, dtw, h->dims, sp, stw, wd.get(), ws.get(),
+                                                  vo.as<double>(), st),
+                        "potential restriction kernels"))
+                return Q3X_ERR_DEVICE;
+            restricted.resize(h->npoints);
+            if (!hip_ok(hipMemcpyAsync(restricted.data(), vo.get(), static_cast<size_t>(h->npoints) * 8,
+                                       hipMemcpyDeviceToHost, st),
+                        "D2H smooth V") ||
+                !hip_ok(hipStreamSynchronize(st), "hipStreamSynchronize"))
+                return Q3X_ERR_DEVICE;
+            vs = restricted.data();
         }
         const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
         const double inv = 1.0 / static_cast<double>(h->npoints);  // serial.hpp:69-73
@@ -627,7 +685,7 @@ q3x_status q3x_hamiltonian_set_potential(q3x_hamiltonian h, const double* v, lon
         for (int i1 = 0; i1 < n1; ++i1)
             for (int i2 = 0; i2 < n2; ++i2)
                 for (int i3 = 0; i3 < n3; ++i3) {
-                    const double s = v[(static_cast<long>(i1) * n2 + i2) * n3 + i3] * inv;
+                    const double s = vs[(static_cast<long>(i1) * n2 + i2) * n3 + i3] * inv;
                     const long o = (static_cast<long>(i3) * n2 + i2) * n1 + i1;
                     d[o] = s;
                     f[o] = static_cast<float>(s);

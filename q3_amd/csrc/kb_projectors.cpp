@@ -331,3 +331,72 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
         return Q3X_ERR_INTERNAL;
     }
 }
+
+// Per-atom D blocks (d_matrix_element, quantum_system/kohn_sham/planewave/norm_conserving/
+// init.hpp:84-100, packed as serial.hpp:603-640 packs them).  A species' channels run beta by
+// beta with m = 0, +1, -1, ... (the projector order above); two channels couple through
+// dion[beta_a][beta_b] exactly when they carry the same (l, m), so within one l the coupling
+// block of two betas is dion[a][b] times the (2l+1) identity.  Entries with |D| <= 1e-15 are
+// dropped (d_matrix_element_threshold, tolerances.hpp:99).
+extern "C" q3x_status q3x_kb_d_blocks(int natoms, const int* atom_species, int nspecies, const int* nbeta,
+                                      const int* l_all, const double* dion_all, int* block_sizes, double* d_blocks,
+                                      long* packed_len) {
+    try {
+        if (natoms < 0 || nspecies < 1 || !nbeta || !l_all || !dion_all || (natoms > 0 && !atom_species)) {
+            set_error("q3x_kb_d_blocks: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        // channel tables per species: (l, m) slot and owning beta
+        std::vector<std::vector<int>> slot(nspecies), owner(nspecies);
+        std::vector<size_t> dion_off(nspecies);
+        size_t lo = 0, doff = 0;
+        for (int s = 0; s < nspecies; ++s) {
+            if (nbeta[s] < 0) {
+                set_error("q3x_kb_d_blocks: negative projector count");
+                return Q3X_ERR_ARGUMENT;
+            }
+            dion_off[s] = doff;
+            for (int i = 0; i < nbeta[s]; ++i) {
+                const int l = l_all[lo + i];
+                if (l < 0 || l > 3) {
+                    set_error("q3x_kb_d_blocks: projector l must be 0..3");
+                    return Q3X_ERR_ARGUMENT;
+                }
+                for (int mi = 0; mi < 2 * l + 1; ++mi) {
+                    const int m = mi == 0 ? 0 : ((mi & 1) ? (mi + 1) / 2 : -(mi / 2));
+                    slot[s].push_back(lm_slot(l, m));
+                    owner[s].push_back(i);
+                }
+            }
+            lo += nbeta[s];
+            doff += static_cast<size_t>(nbeta[s]) * nbeta[s];
+        }
+        long total = 0;
+        for (int a = 0; a < natoms; ++a) {
+            const int s = atom_species[a];
+            if (s < 0 || s >= nspecies) {
+                set_error("q3x_kb_d_blocks: atom species out of range");
+                return Q3X_ERR_ARGUMENT;
+            }
+            const int p = static_cast<int>(slot[s].size());
+            if (block_sizes) block_sizes[a] = p;
+            if (d_blocks) {
+                double* blk = d_blocks + total;
+                std::fill(blk, blk + static_cast<long>(p) * p, 0.0);
+                const double* dion = dion_all + dion_off[s];
+                for (int col = 0; col < p; ++col)
+                    for (int row = 0; row < p; ++row) {
+                        if (slot[s][row] != slot[s][col]) continue;
+                        const double v = dion[owner[s][row] * nbeta[s] + owner[s][col]];
+                        if (std::abs(v) > 1e-15) blk[row + static_cast<long>(col) * p] = v;
+                    }
+            }
+            total += static_cast<long>(p) * p;
+        }
+        if (packed_len) *packed_len = total;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_kb_d_blocks: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
+}

@@ -142,6 +142,8 @@ namespace q3x {
 // projectors cleared) keeping its device buffers, which only grow: a long-lived worker reuses one
 // Hamiltonian across requests without allocating or freeing device memory once warm.
 q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b);
+// Device bytes a context (its workspaces) and a Hamiltonian (may be null) hold.
+double device_bytes_held(const q3x_context_s* ctx, const q3x_hamiltonian_s* h);
 // Applies H to nvec device vectors (ld = npw) on the context stream.
 q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long ldpsi, double* d_hpsi, long ldh,
                                     int nvec, int precision);

@@ -11,8 +11,12 @@
 // k-points, upload them and solve, so the host setup of the next k-points overlaps the device
 // work of the current ones.  A request that arrives
 // while others are running is admitted as soon as the budget allows and its k-points interleave
-// with theirs on the device; when its last k-point finishes, its footprint is released and the
-// next waiting requests are admitted.
+// with theirs on the device.  What admission counts as resident: the wavefunctions of every
+// admitted request not yet collected, the blocks parked in the reuse pool, and the larger of the
+// running requests' per-k-point shares and the workers' own buffers (Davidson workspace and
+// Hamiltonian, which persist between k-points).  A request's per-k-point share is released when
+// its last k-point finishes, its wavefunction share when the client collects the result; when
+// the head of the queue does not fit, the pool is emptied first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,7 +46,8 @@ struct Request {
     std::vector<int> atom_species, nmesh, nbeta, l_all, cutoff_all, block_sizes;
     int nk = 0, natoms = 0, nspecies = 0, nb = 0, nocc = 0;
     q3x_davidson_options opt{};
-    double bytes = 0.0;
+    double bytes = 0.0;      // admission estimate: bytes_psi + per-k-point share
+    double bytes_psi = 0.0;  // wavefunctions of all k-points, held until the result is collected
     // progress
     int remaining = 0;
     int next_task = 0;
@@ -75,7 +80,20 @@ struct Prepared {
 
 struct q3x_server_s {
     int device = 0;
-    double budget = 0.0, in_flight = 0.0;
+    double budget = 0.0;
+    // resident-memory terms of admission (see the file comment)
+    double psi_held = 0.0, work_running = 0.0, pool_bytes = 0.0;
+    std::vector<double> worker_bytes;  // device bytes each worker's context and Hamiltonian hold
+    double workers_locked() const {
+        double ws = 0.0;
+        for (double b : worker_bytes) ws += b;
+        return ws;
+    }
+    double resident_locked() const { return psi_held + pool_bytes + std::max(workers_locked(), work_running); }
+    // resident memory once r is admitted: r's per-k-point work may reuse the workers' buffers
+    double resident_with_locked(const Request& r) const {
+        return psi_held + pool_bytes + r.bytes_psi + std::max(workers_locked(), work_running + (r.bytes - r.bytes_psi));
+    }
     clk::time_point t0;
     std::vector<q3x_context> workers;
     std::vector<std::thread> threads, prep_threads;
@@ -102,6 +120,7 @@ struct q3x_server_s {
             auto it = psi_pool.lower_bound(bytes);
             if (it != psi_pool.end()) {
                 void* p = it->second;
+                pool_bytes -= static_cast<double>(it->first);
                 psi_pool.erase(it);
                 return p;
             }
@@ -110,15 +129,23 @@ struct q3x_server_s {
         return q3x_device_alloc(ctx, static_cast<unsigned long>(bytes), &p) == Q3X_OK ? p : nullptr;
     }
 
-    // admission in arrival order: the head waits until it fits (or nothing else is in flight)
+    // admission in arrival order: the head waits until it fits, or until no other request holds
+    // memory; parked wavefunction blocks are freed before the head is made to wait for them
     void admit_locked() {
         while (!waiting.empty()) {
             auto r = waiting.front();
-            if (in_flight > 0.0 && in_flight + r->bytes > budget) break;
+            if (resident_with_locked(*r) > budget && !psi_pool.empty()) {
+                for (auto& kv : psi_pool) (void)hipFree(kv.second);
+                psi_pool.clear();
+                pool_bytes = 0.0;
+            }
+            const bool others = psi_held > 0.0 || work_running > 0.0;
+            if (others && resident_with_locked(*r) > budget) break;
             waiting.pop_front();
             r->admitted = true;
             r->t_admit = now_s(t0);
-            in_flight += r->bytes;
+            psi_held += r->bytes_psi;
+            work_running += r->bytes - r->bytes_psi;
             running.push_back(r);
         }
         cv_task.notify_all();
@@ -192,7 +219,9 @@ struct q3x_server_s {
 
     void worker(int w) {
         q3x_context ctx = workers[w];
-        if (hipSetDevice(device) != hipSuccess) return;
+        // a worker that cannot select its device still takes k-points, and fails them, so that
+        // their requests finish and waiters are released
+        const bool device_ok = hipSetDevice(device) == hipSuccess;
         for (;;) {
             std::unique_ptr<Prepared> p;
             {
@@ -205,10 +234,19 @@ struct q3x_server_s {
             }
             Request& r = *p->r;
             std::string err;
-            const q3x_status s = p->status != Q3X_OK ? p->status : solve_kpoint(w, ctx, *p, err);
-            if (p->status != Q3X_OK) err = p->err;
+            q3x_status s = p->status;
+            if (s != Q3X_OK) {
+                err = p->err;
+            } else if (!device_ok) {
+                s = Q3X_ERR_DEVICE;
+                err = "the worker could not select device " + std::to_string(device);
+            } else {
+                s = solve_kpoint(w, ctx, *p, err);
+            }
             if (p->kp) q3x_kpoint_destroy(p->kp);
+            const double held = device_ok ? q3x::device_bytes_held(ctx, hams[w]) : 0.0;
             std::lock_guard<std::mutex> g(mu);
+            worker_bytes[w] = held;
             if (s != Q3X_OK && r.status == Q3X_OK) {
                 r.status = s;
                 r.error = "k-point " + std::to_string(p->k) + ": " + err;
@@ -216,8 +254,8 @@ struct q3x_server_s {
             if (--r.remaining == 0) {
                 r.done = true;
                 r.t_done = now_s(t0);
-                in_flight -= r.bytes;
-                if (in_flight < 1.0) in_flight = 0.0;
+                work_running -= r.bytes - r.bytes_psi;
+                if (work_running < 1.0) work_running = 0.0;
                 admit_locked();
                 cv_done.notify_all();
             }
@@ -289,9 +327,11 @@ std::vector<T> copy_of(const T* p, size_t n) {
 // matrices and eigensolver workspace, the projectors (complex128 + complex64), the z-stick
 // buffer and the potential -- the terms of PlanewaveMemoryEstimate::compute
 // (memory_estimate.hpp:75-215) for this path.
-double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int nproj, int workers) {
+double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int nproj, int workers,
+                 double* psi_share = nullptr) {
     const double c = 16.0, nb = q->nbands, m = std::max(std::min(2.0 * nb, static_cast<double>(npw)), nb);
     const double psi = static_cast<double>(q->nk) * nb * npw * c;
+    if (psi_share) *psi_share = psi;
     const double sticks = std::max(1.0, static_cast<double>(npw) / std::max(1, n3)) * 1.6;  // z-sticks
     const double per_k = 4 * m * npw * c + nb * npw * c + 30.0 * m * m * c + 1.5 * nproj * npw * c +
                          nb * n3 * sticks * c + 12.0 * npoints;
@@ -325,6 +365,7 @@ q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q
         (void)hipMemGetInfo(&free_b, &total_b);
         s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : 0.8 * static_cast<double>(free_b);
         s->hams.assign(workers, nullptr);
+        s->worker_bytes.assign(workers, 0.0);
         s->ham_ticket.assign(workers, 0);
         // host preparation runs ahead on half as many threads as workers (its share of a k-point's
         // time is about a third of the device part at the cfg5 cell sizes)
@@ -371,7 +412,8 @@ q3x_status q3x_server_destroy(q3x_server s) {
     }
 }
 
-q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* bytes) {
+namespace {
+q3x_status estimate(q3x_server s, const q3x_band_request* q, double* bytes, double* psi_share) {
     try {
         if (!s || !q || !q->basis || !bytes || q->nk < 1) {
             set_error("q3x_server_estimate: invalid argument");
@@ -389,12 +431,17 @@ q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* 
         for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
         *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2],
                            static_cast<long>(wave[0]) * wave[1] * wave[2],
-                           nproj, static_cast<int>(s->workers.size()));
+                           nproj, static_cast<int>(s->workers.size()), psi_share);
         return Q3X_OK;
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_server_estimate: ") + e.what());
         return Q3X_ERR_INTERNAL;
     }
+}
+}  // namespace
+
+q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* bytes) {
+    return estimate(s, q, bytes, nullptr);
 }
 
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* ticket) {
@@ -404,7 +451,7 @@ q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* tick
             return Q3X_ERR_ARGUMENT;
         }
         auto r = std::make_shared<Request>();
-        const q3x_status st = q3x_server_estimate(s, q, &r->bytes);
+        const q3x_status st = estimate(s, q, &r->bytes, &r->bytes_psi);
         if (st != Q3X_OK) return st;
         r->basis = q->basis;
         r->v = copy_of(q->v_r, static_cast<size_t>(q->nv));
@@ -469,7 +516,13 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
             }
             r = it->second;
             s->cv_done.wait(g, [&] { return r->done; });
-            s->all.erase(it);
+            // erase by key: another wait on the same ticket may have collected it meanwhile
+            auto again = s->all.find(ticket);
+            if (again == s->all.end() || again->second != r) {
+                set_error("q3x_server_wait: the ticket was collected by another call");
+                return Q3X_ERR_ARGUMENT;
+            }
+            s->all.erase(again);
         }
         if (eig_out) std::memcpy(eig_out, r->eig.data(), r->eig.size() * sizeof(double));
         if (stats) {
@@ -482,9 +535,17 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
             stats->solve_s = r->solve_s;
         }
         {
-            std::lock_guard<std::mutex> g(s->mu);  // the request's wavefunction blocks go back to the pool
+            // the request's wavefunction blocks go back to the pool, and its share of the
+            // admission budget moves with them (the pool is freed when admission needs the room)
+            std::lock_guard<std::mutex> g(s->mu);
             for (size_t k = 0; k < r->psi.size(); ++k)
-                if (r->psi[k].second) s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
+                if (r->psi[k].second) {
+                    s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
+                    s->pool_bytes += static_cast<double>(r->psi_bytes[k]);
+                }
+            s->psi_held -= r->bytes_psi;
+            if (s->psi_held < 1.0) s->psi_held = 0.0;
+            s->admit_locked();
         }
         r->psi.clear();
         if (r->status != Q3X_OK) set_error(r->error);
@@ -498,7 +559,7 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
 double q3x_server_in_flight_bytes(q3x_server s) {
     if (!s) return 0.0;
     std::lock_guard<std::mutex> g(s->mu);
-    return s->in_flight;
+    return s->resident_locked();
 }
 
 }  // extern "C"

@@ -67,29 +67,32 @@ def write_synthetic_upf(path: str, seed: int = 0) -> None:
 
 
 def d_blocks_for(crystal: Crystal, species_list):
-    """Per-atom D blocks (column-major, packed) and their sizes: D_ij = dion[beta_i][beta_j]
-    when (l,m) of the two channels agree (norm_conserving/init.hpp:84-100)."""
-    sizes, blocks = [], []
-    for s in crystal.species:
-        sp = species_list[s]
-        chans = []
-        for ib, l in enumerate(sp["l"]):
-            for mi in range(2 * l + 1):
-                chans.append((ib, l, mi))
-        p = len(chans)
-        D = np.zeros((p, p))
-        for a, (ia, la, ma) in enumerate(chans):
-            for b, (ib, lb, mb) in enumerate(chans):
-                if la == lb and ma == mb:
-                    D[a, b] = sp["dion_ry"][ia, ib]
-        sizes.append(p)
-        blocks.append(D.ravel(order="F"))
-    return np.array(sizes, np.int32), (np.concatenate(blocks) if blocks else np.zeros(0))
+    """Per-atom D blocks (column-major, packed) and their sizes, built by the library
+    (q3x_kb_d_blocks): D_ij = dion[beta_i][beta_j] when (l,m) of the two channels agree
+    (norm_conserving/init.hpp:84-100)."""
+    import ctypes
+
+    from ._lib import check, lib
+    asp = np.ascontiguousarray(crystal.species, dtype=np.int32)
+    nbeta = np.ascontiguousarray([len(s["l"]) for s in species_list], dtype=np.int32)
+    l_all = np.ascontiguousarray(np.concatenate([np.asarray(s["l"], np.int32) for s in species_list]), np.int32)
+    dion = np.ascontiguousarray(np.concatenate([np.asarray(s["dion_ry"], np.float64).ravel()
+                                                for s in species_list]), np.float64)
+    n = ctypes.c_long()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    check(lib().q3x_kb_d_blocks(len(asp), p(asp), len(species_list), p(nbeta), p(l_all), p(dion), None, None,
+                                ctypes.byref(n)))
+    sizes = np.zeros(len(asp), np.int32)
+    blocks = np.zeros(max(n.value, 1))
+    check(lib().q3x_kb_d_blocks(len(asp), p(asp), len(species_list), p(nbeta), p(l_all), p(dion), p(sizes),
+                                p(blocks), ctypes.byref(n)))
+    return sizes, blocks[:n.value]
 
 
-def local_potential(crystal: Crystal, basis, species_list) -> np.ndarray:
-    """V(r) on the wave grid (C order i1,i2,i3), Ry.  Built in G space over the density sphere
-    and brought to real space with the FFTW_BACKWARD convention (numpy: N * ifftn)."""
+def local_potential(crystal: Crystal, basis, species_list, grid: str = "wave") -> np.ndarray:
+    """V(r) on the wave grid (grid="wave") or the density grid (grid="rho"), C order (i1,i2,i3),
+    Ry.  Built in G space over the density sphere and brought to real space with the
+    FFTW_BACKWARD convention (numpy: N * ifftn)."""
     m, g, g2 = basis.g_vectors()
     tpa = 2 * math.pi / crystal.alat
     gabs2 = g2 * tpa * tpa  # bohr^-2
@@ -104,7 +107,7 @@ def local_potential(crystal: Crystal, basis, species_list) -> np.ndarray:
         for a in np.nonzero(np.asarray(crystal.species) == s_idx)[0]:
             sf += np.exp(-2j * math.pi * (g @ tau[a]))
         vg += form * sf
-    dims = basis.wave_dims
+    dims = basis.wave_dims if grid == "wave" else basis.rho_dims
     box = np.zeros(dims, np.complex128)
     ok = np.all(np.abs(m) <= np.array(dims) // 2, axis=1)
     w = np.where(m < 0, m + np.array(dims), m)

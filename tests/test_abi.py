@@ -78,6 +78,26 @@ def test_kb_projectors_against_oracle(make, kidx):
     assert np.abs(beta - obeta).max() <= 1e-13 * np.abs(obeta).max()
 
 
+def test_d_blocks_against_oracle():
+    """Library D blocks (q3x_kb_d_blocks) equal the oracle's on a two-species cell whose species
+    carry two betas in one l channel with an off-diagonal dion, and on the serving species."""
+    rng = np.random.default_rng(11)
+    a = rng.uniform(1, 3, (4, 4))
+    sp0 = {"l": [0, 1, 1, 2], "dion_ry": a + a.T}
+    sp1 = {"l": [1, 0], "dion_ry": np.array([[1.5, 0.3], [0.3, -2.0]])}
+    c = W.Crystal(W.FCC.copy(), 10.0, rng.uniform(size=(5, 3)), np.array([0, 1, 1, 0, 1], np.int32))
+    bs, db = W.d_blocks_for(c, [sp0, sp1])
+    obs, odb = O.d_blocks(c.species, [sp0, sp1])
+    np.testing.assert_array_equal(bs, obs)
+    np.testing.assert_array_equal(db, odb)
+    assert list(bs) == [12, 4, 4, 12, 4]
+    w = W.cfg2()
+    bs, db = W.d_blocks_for(w.crystal, w.species)
+    obs, odb = O.d_blocks(w.crystal.species, w.species)
+    np.testing.assert_array_equal(bs, obs)
+    np.testing.assert_array_equal(db, odb)
+
+
 def _hegv(H, S, nvec):
     n = H.shape[0]
     H = np.asfortranarray(H, np.complex128)

@@ -211,3 +211,34 @@ def test_fft_kernels_agree_cfg2(gpu_context, monkeypatch):
     monkeypatch.delenv("Q3X_FFT_KERNELS")
     for prec, tol in ((FP64, 1e-13), (MIXED, 2e-6)):
         assert rel_err(h_tl.apply_batch(psi, prec), h_gen.apply_batch(psi, prec)) <= tol
+
+
+@pytest.mark.parametrize("make,kindex", [(W.cfg1, 3), (W.cfg4, 1)])
+def test_dense_grid_potential_is_restricted_to_the_wave_grid(gpu_context, make, kindex):
+    """update_potential with the density-grid V on the reference's default wavefunction grid
+    (2 x ecutwfc, smaller than the density grid): the library's G-space restriction on the device
+    gives the H|psi> of the oracle's restriction (planewave_potential_utils.hpp:33-130), FP64
+    <= 1e-10.  Feeding the dense V sampled at the coarse points instead is off by far more."""
+    from q3_amd.planewave import PlanewaveBasis
+    w = make()
+    b = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, False, w.fft_factors)
+    assert b.rho_dims != b.wave_dims
+    kp = b.compute_gk(w.kpoints[kindex])
+    v_dense = W.local_potential(w.crystal, b, w.species, grid="rho")
+    beta = kp.kb_projectors(w.crystal, w.species)
+    bs, db = W.d_blocks_for(w.crystal, w.species)
+    psi = W.random_psi(kp.kinetic_energy_squared, 6, seed=21)
+    print(f"{w.name}: rho {b.rho_dims} wave {b.wave_dims} npw {kp.npw}", flush=True)
+    h = PlanewaveHamiltonianApplicator(gpu_context, b)
+    h.update_potential(v_dense)
+    h.prepare_kpoint(kp)
+    h.set_projectors(beta, bs, db)
+    got = h.apply_batch(psi, FP64)
+    v_s = O.restrict_potential(v_dense, b.wave_dims)
+    ref = O.hamiltonian_apply(b.wave_dims, kp.fft_map, kp.kinetic_energy_squared, v_s, beta, bs, db, psi)
+    assert rel_err(got, ref) <= 1e-10
+    step = [r // s for r, s in zip(b.rho_dims, b.wave_dims)]
+    if all(r == s * t for r, s, t in zip(b.rho_dims, b.wave_dims, step)):
+        sampled = np.ascontiguousarray(v_dense[::step[0], ::step[1], ::step[2]])
+        wrong = O.hamiltonian_apply(b.wave_dims, kp.fft_map, kp.kinetic_energy_squared, sampled, beta, bs, db, psi)
+        assert rel_err(wrong, ref) > 1e-6

@@ -77,3 +77,27 @@ def test_server_equals_direct_solves(gpu_context):
         # the server seeds k-point i with the damped random block of seed i
         e, _ = d.solve(spec.nbands, spec.noccupied, kp.random_psi(spec.nbands, i))
         assert np.abs(e - eig[i]).max() <= 1e-10
+
+
+def test_uncollected_result_holds_its_wavefunctions_in_the_budget():
+    """A request whose k-points are done but whose result is not collected still holds its
+    wavefunctions in HBM: a second request that does not fit beside them waits until the first
+    is collected (then the parked blocks are freed for it)."""
+    import time
+    specs = _specs(2)
+    probe = BandStructureServer(workers=2)
+    sizes = [probe.estimate(s) for s in specs]
+    probe.close()
+    budget = max(sizes) * 1.01
+    srv = BandStructureServer(workers=2, hbm_budget_bytes=budget)
+    ta = srv.submit(specs[0])
+    tb = srv.submit(specs[1])
+    hold_s = 6.0
+    time.sleep(hold_s)  # the first request finishes well inside this window and stays uncollected
+    ea, sa = srv.wait(ta)
+    eb, sb = srv.wait(tb)
+    srv.close()
+    print(f"first: submit {sa['submit_s']:.3f} done {sa['done_s']:.3f}; second: admit {sb['admit_s']:.3f}", flush=True)
+    assert sa["done_s"] < sa["submit_s"] + hold_s  # done before it was collected
+    assert sb["admit_s"] >= sa["submit_s"] + hold_s - 0.05  # admitted only after the collection
+    assert np.isfinite(ea).all() and np.isfinite(eb).all()

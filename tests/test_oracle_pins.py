@@ -294,3 +294,117 @@ def test_davidson_capped_leaves_bands_unconverged():
     b, gi, ek, fm, m, V, beta, bs, db = _small_case(12.0)
     eig, psi, rep = O.davidson(b.wave_dims, fm, ek, V, beta, bs, db, 8, 4, None, tol=1e-13, max_iter=1)
     assert 0 < rep["unconverged"] <= 8
+
+
+# ---------------------------------------------------------------- KB projectors, D blocks ----
+def _gaussian_species(ls, sigmas, dion_ry):
+    """Species on the synthetic log mesh with beta_i(r) = r^l exp(-r^2 / 2 sigma^2) (stored as r beta)."""
+    sp = W.synthetic_species(0)
+    r = sp["r"]
+    beta_r = np.vstack([r * r ** l * np.exp(-r * r / (2 * s * s)) for l, s in zip(ls, sigmas)])
+    return {"r": r, "rab": sp["rab"], "l": list(ls), "cutoff": [sp["cutoff"]] * len(ls), "beta_r": beta_r,
+            "dion_ry": np.asarray(dion_ry, float)}
+
+
+def _numpy_projectors(lat, alat, k, m, g_cart, pos, species_of_atom, species):
+    """beta_p(k+G) assembled in numpy from closed forms alone (projector.hpp:95-199 convention):
+    Omega^-1/2 * 4 pi sqrt(pi/2) sigma^(2l+3) q^l exp(-q^2 sigma^2/2) * Y_lm(k+G) (harmonics.hpp:85-141
+    closed forms) * exp(-i 2 pi (k+G).tau) (phase_factor.hpp:17-19) * (-i)^l (:33-44); columns atom by
+    atom, beta by beta, m = 0, +1, -1, +2, -2 (harmonics.hpp:40-48)."""
+    lat = np.asarray(lat, float)
+    recip = np.linalg.inv(lat).T                       # rows b_i, 2 pi / alat units
+    omega = abs(np.linalg.det(lat)) * alat ** 3
+    kc = np.asarray(k) @ recip
+    v = kc[None, :] + g_cart                           # k+G, 2 pi / alat units
+    q = np.linalg.norm(v, axis=1) * 2 * math.pi / alat  # bohr^-1
+    cols = []
+    for a, s in enumerate(species_of_atom):
+        sp = species[s]
+        tau = np.asarray(pos[a]) @ lat                 # alat units
+        phase = np.exp(-2j * math.pi * (v @ tau))
+        for l, sig in zip(sp["l"], sp["sigma"]):
+            rad = 4 * math.pi * math.sqrt(math.pi / 2) * sig ** (2 * l + 3) * q ** l * np.exp(-q * q * sig * sig / 2)
+            for mm in [0] + [x for j in range(1, l + 1) for x in (j, -j)]:
+                y = np.array([(_ylm_closed(l, mm, vi) if np.linalg.norm(vi) > 1e-12
+                               else (1 / math.sqrt(4 * math.pi) if l == 0 else 0.0)) for vi in v])
+                cols.append(rad * y * phase * (-1j) ** l / math.sqrt(omega))
+    return np.array(cols).T
+
+
+@pytest.mark.parametrize("k", [(0.0, 0.0, 0.0), (0.125, -0.25, 0.375)])
+def test_kb_projector_assembly_against_numpy_closed_forms(k):
+    """Pins the oracle's projector assembly, not just its radial table: a dropped Omega^-1/2, a
+    flipped structure-factor sign, tau taken in the wrong frame, a wrong (-i)^l or m = +1/-1 swapped
+    each moves beta by O(1) of its norm; the bound is the Simpson + 4-point Lagrange error."""
+    ls, sig = (0, 1, 1, 2), (0.5, 0.6, 0.45, 0.55)
+    sp = _gaussian_species(ls, sig, np.eye(4))
+    lat, alat = FCC, 10.26
+    pos = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25], [0.61, 0.07, 0.33]])
+    spec = np.zeros(3, np.int32)
+    b = O.Basis(lat, alat, 24.0, 0.0, True, (1, 1, 1))
+    gidx, ek, fm = b.kpoint(k)
+    beta = b.kb_projectors(k, gidx, pos, spec, [sp])
+    m, g, g2 = b.gvectors()
+    want = _numpy_projectors(lat, alat, k, m[gidx], g[gidx], pos, spec, [dict(sp, sigma=sig)])
+    assert beta.shape == want.shape == (len(gidx), 3 * (1 + 3 + 3 + 5))
+    err = np.abs(beta - want).max() / np.abs(want).max()
+    assert err <= 1e-8, err
+    # the same comparison would fail with each plausible slip in the assembly
+    n_cols = 12
+    assert np.abs(beta - want * math.sqrt(b_omega(lat, alat))).max() > 0.1 * np.abs(want).max()  # no Omega^-1/2
+    swapped = want.copy()
+    for a in range(3):
+        c = a * n_cols + 2  # first p channel of atom a: columns m = 0, +1, -1
+        swapped[:, [c, c + 1]] = swapped[:, [c + 1, c]]
+    assert np.abs(beta - swapped).max() > 0.1 * np.abs(want).max()  # m = +1 / -1 swapped
+    assert np.abs(beta - np.conj(want)).max() > 0.1 * np.abs(want).max()  # phase and (-i)^l conjugated
+
+
+def b_omega(lat, alat):
+    return abs(np.linalg.det(np.asarray(lat, float))) * alat ** 3
+
+
+def test_d_blocks_two_betas_per_channel_hand_built():
+    """norm_conserving/init.hpp:84-100: D couples two channels when their (l, m) agree, with
+    dion[beta_i][beta_j]; s-p couplings vanish even when dion holds them; |D| <= 1e-15 is dropped."""
+    dion = np.array([[2.0, 0.7, 0.0],      # beta0 l=0, beta1 l=1, beta2 l=1
+                     [0.7, 3.0, -0.4],
+                     [0.0, -0.4, 5e-16]])  # below the 1e-15 threshold: dropped
+    sp = {"l": [0, 1, 1], "dion_ry": dion}
+    sizes, blocks = O.d_blocks(np.array([0, 0]), [sp])
+    assert list(sizes) == [7, 7]
+    D = blocks[:49].reshape(7, 7, order="F")
+    want = np.zeros((7, 7))
+    want[0, 0] = 2.0
+    for j in range(3):          # channels 1..3: beta1 m=0,+1,-1 ; 4..6: beta2 m=0,+1,-1
+        want[1 + j, 1 + j] = 3.0
+        want[1 + j, 4 + j] = want[4 + j, 1 + j] = -0.4
+        want[4 + j, 4 + j] = 0.0
+    assert np.array_equal(D, want)
+    assert np.array_equal(blocks[49:], blocks[:49])
+
+
+# ---------------------------------------------------------------- dense -> smooth potential ----
+def _cos_field(dims, comps):
+    x = [np.arange(d) / d for d in dims]
+    X = np.meshgrid(*x, indexing="ij")
+    v = np.zeros(dims)
+    for amp, m, ph in comps:
+        v += amp * np.cos(2 * math.pi * (m[0] * X[0] + m[1] * X[1] + m[2] * X[2]) + ph)
+    return v
+
+
+def test_restrict_potential_closed_form():
+    """planewave_potential_utils.hpp:33-130: components inside the smooth box are kept exactly,
+    components outside it are removed, and a component on the smooth box's +n/2 plane keeps half
+    its amplitude (the +n/2 slot takes the +n/2 coefficient only, the -n/2 one is dropped)."""
+    dd, sd = (36, 40, 30), (24, 25, 20)
+    inside = [(1.3, (2, -3, 1), 0.4), (-0.7, (0, 5, -4), 1.1), (0.25, (-11, 12, 9), -0.3), (2.0, (0, 0, 0), 0.0)]
+    outside = [(0.9, (13, 2, 0), 0.2), (0.6, (1, -14, 3), 0.0), (0.4, (0, 1, 11), 0.7)]
+    nyq = [(0.8, (12, 1, 0), 0.0)]
+    vd = _cos_field(dd, inside + outside + nyq)
+    vs = O.restrict_potential(vd, sd)
+    want = _cos_field(sd, inside) + _cos_field(sd, [(0.4, (12, 1, 0), 0.0)])
+    assert np.abs(vs - want).max() <= 1e-12 * np.abs(want).max()
+    # equal boxes: the dense potential itself (serial.hpp:859-861)
+    assert np.array_equal(O.restrict_potential(vd, dd), vd)
