@@ -139,6 +139,17 @@ q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* h
  * context stream, returns after queueing (call q3x_context_synchronize to wait). */
 q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi,
                                         int nvec, int precision);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * stream capture, serving loops that forbid hipMalloc): the product of
+ * q3x_hamiltonian_apply_device with every scratch buffer (stick buffer, plane slots, projector
+ * products, split-K workspace) in one fixed device array of q3x_noalloc_scratch_bytes() bytes
+ * (64 MiB) per GPU, held until the process ends.  Bands run in chunks that fit it; one band
+ * that does not fit is refused with Q3X_ERR_ARGUMENT.  Calls no hipMalloc/hipFree.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (two contexts or threads)
+ * race.  Nothing in the array needs to start at zero. */
+q3x_status q3x_hamiltonian_apply_noalloc(q3x_hamiltonian h, const void* d_psi, void* d_hpsi,
+                                         int nvec, int precision);
+unsigned long q3x_noalloc_scratch_bytes(void);
 /* complex GEMM on device buffers, stream-ordered (stands in for GPUBLAS::gemm,
  * src/layer_0_core/math/linear_algebra/gpu/blas/init.hpp, on MFMA):
  *   C (m x n) = alpha op(A) B + beta C,  transa 'N' (A m x k) or 'C' (A k x m, op = A^H),

@@ -66,6 +66,8 @@ SIGNATURES = {
     "q3x_hamiltonian_set_projectors": (_i, [_p, _p, _i, _i, _p, _p]),
     "q3x_hamiltonian_apply": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_hamiltonian_apply_noalloc": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_noalloc_scratch_bytes": (ctypes.c_ulong, []),
     "q3x_zgemm": (_i, [_p, ctypes.c_char, _i, _i, _i, _d, _p, _l, _p, _l, _d, _p, _l, _i]),
     "q3x_device_alloc": (_i, [_p, _ul, _pp]),
     "q3x_device_free": (_i, [_p, _p]),

@@ -324,6 +324,99 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
     return Q3X_OK;
 }
 
+namespace {
+size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Scratch bytes of one band chunk of nb bands carved from a fixed region: stick buffer, plane
+// slots (planes kept outside LDS), P and W of the projector chain, each 256-byte aligned.
+size_t fixed_chunk_bytes(const q3x_hamiltonian_s* h, const PlaneTlConfig& tl, int nb, size_t cs) {
+    size_t b = align256(static_cast<size_t>(nb) * h->dims[2] * h->nst * cs);
+    if (tl.ok && tl.plane_global) b += align256(static_cast<size_t>(nb) * h->dims[2] * tl.plane_bytes);
+    if (h->have_projectors && h->nproj > 0) b += 2 * align256(static_cast<size_t>(h->nproj) * nb * cs);
+    return b;
+}
+}  // namespace
+
+q3x_status hamiltonian_apply_fixed(q3x_hamiltonian h, const double* d_psi, double* d_hpsi, int nvec, int precision,
+                                   char* scratch, size_t scratch_bytes) {
+    if (!h || !h->kp) {
+        set_error("apply: prepare_kpoint() must be called before apply (serial.hpp:487-492)");
+        return Q3X_ERR_STATE;
+    }
+    if (!h->have_potential) {
+        set_error("apply: set_potential() must be called before apply");
+        return Q3X_ERR_STATE;
+    }
+    if (nvec < 0) {
+        set_error("apply: negative band count");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (nvec == 0) return Q3X_OK;
+    if (precision != Q3X_FP64 && precision != Q3X_MIXED) {
+        set_error("apply: precision must be Q3X_FP64 or Q3X_MIXED");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!scratch || !d_psi || !d_hpsi) {
+        set_error("apply: null device pointer");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const bool fp32 = precision == Q3X_MIXED;
+    const size_t cs = fp32 ? 8 : 16;
+    const PlaneTlConfig& tl = fp32 ? h->tl32 : h->tl64;
+    const size_t one = fixed_chunk_bytes(h, tl, 1, cs);
+    if (one > scratch_bytes) {
+        set_error("apply (no allocation): one band needs " + std::to_string(one) + " bytes of scratch (stick buffer" +
+                  std::string(tl.ok && tl.plane_global ? ", plane slots" : "") + ", projector products), more than the " +
+                  std::to_string(scratch_bytes) + " bytes of the fixed scratch buffer");
+        return Q3X_ERR_ARGUMENT;
+    }
+    // largest chunk of bands whose buffers fit, leaving the split-K workspace the rest
+    int chunk = static_cast<int>(std::min<size_t>(static_cast<size_t>(nvec), scratch_bytes / one));
+    while (chunk > 1 && fixed_chunk_bytes(h, tl, chunk, cs) > scratch_bytes) --chunk;
+    const size_t zb = align256(static_cast<size_t>(chunk) * h->dims[2] * h->nst * cs);
+    const size_t sb = tl.ok && tl.plane_global ? align256(static_cast<size_t>(chunk) * h->dims[2] * tl.plane_bytes) : 0;
+    const bool nonlocal = h->have_projectors && h->nproj > 0;
+    const size_t pb = nonlocal ? align256(static_cast<size_t>(h->nproj) * chunk * cs) : 0;
+    char* zbuf = scratch;
+    char* slots = sb ? scratch + zb : nullptr;
+    char* P = scratch + zb + sb;
+    char* W = P + pb;
+    char* work = W + pb;
+    const size_t used = zb + sb + 2 * pb;
+    const size_t ws = scratch_bytes > used ? std::min(scratch_bytes - used, static_cast<size_t>(64) << 20) : 0;
+    hipStream_t st = h->ctx->stream;
+    const HLocalArgs args = make_args(h, fp32);
+    const void* vplane = fp32 ? h->v32.get() : h->v64.get();
+    const long ld = h->npw;
+    for (int b0 = 0; b0 < nvec; b0 += chunk) {
+        const int nb = std::min(chunk, nvec - b0);
+        const double* psi = d_psi + 2L * b0 * ld;
+        double* hpsi = d_hpsi + 2L * b0 * ld;
+        if (!hip_ok(launch_hlocal(args, fp32, vplane, zbuf, psi, ld, hpsi, ld, nb, st, nullptr, &tl, slots,
+                                  fp32 ? &h->tz32 : &h->tz64),
+                    "H_local kernels"))
+            return Q3X_ERR_DEVICE;
+        if (!nonlocal) continue;
+        const int np = h->nproj, npw = h->npw;
+        void* wk = ws ? work : nullptr;
+        hipError_t e = fp32 ? zgemm(GemmTypes::F_D_F_F, 1, np, nb, npw, 1.0, h->beta32.get(), npw, psi, ld, 0.0, P, np,
+                                    wk, ws, st)
+                            : zgemm(GemmTypes::D_D_D_D, 1, np, nb, npw, 1.0, h->beta64.get(), npw, psi, ld, 0.0, P, np,
+                                    wk, ws, st);
+        if (!hip_ok(e, "projector overlap GEMM")) return Q3X_ERR_DEVICE;
+        if (!hip_ok(launch_dblock_apply(fp32, np, nb, h->pstart.as<int>(), h->psize.as<int>(), h->poff.as<long>(),
+                                        h->dblocks.as<double>(), P, W, st),
+                    "D block kernel"))
+            return Q3X_ERR_DEVICE;
+        e = fp32 ? zgemm(GemmTypes::F_F_F_D, 0, npw, nb, np, 1.0, h->beta32.get(), npw, W, np, 1.0, hpsi, ld, nullptr,
+                         0, st)
+                 : zgemm(GemmTypes::D_D_D_D, 0, npw, nb, np, 1.0, h->beta64.get(), npw, W, np, 1.0, hpsi, ld, nullptr,
+                         0, st);
+        if (!hip_ok(e, "projector update GEMM")) return Q3X_ERR_DEVICE;
+    }
+    return Q3X_OK;
+}
+
 }  // namespace q3x
 
 using namespace q3x;
@@ -391,6 +484,8 @@ q3x_status q3x_context_create(int device, q3x_context* out) {
             return Q3X_ERR_DEVICE;
         }
         if (!hip_ok(hipSetDevice(device), "hipSetDevice")) return Q3X_ERR_DEVICE;
+        // device address of the no-allocation scratch array, once per device (this one is current)
+        if (!noalloc_scratch_init(device)) return Q3X_ERR_DEVICE;
         auto* c = new q3x_context_s;
         c->device = device;
         if (!hip_ok(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) {

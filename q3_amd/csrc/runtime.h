@@ -147,4 +147,12 @@ double device_bytes_held(const q3x_context_s* ctx, const q3x_hamiltonian_s* h);
 // Applies H to nvec device vectors (ld = npw) on the context stream.
 q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long ldpsi, double* d_hpsi, long ldh,
                                     int nvec, int precision);
+// The same product with every scratch buffer (stick buffer, plane slots, projector products,
+// split-K workspace) carved from one caller-given device region: bands run in chunks that fit
+// it; nothing is allocated.  Fails with Q3X_ERR_ARGUMENT when one band does not fit.
+q3x_status hamiltonian_apply_fixed(q3x_hamiltonian h, const double* d_psi, double* d_hpsi, int nvec, int precision,
+                                   char* scratch, size_t scratch_bytes);
+// Looks up, once per device and with that device current, the device address of the fixed
+// scratch region of q3x_hamiltonian_apply_noalloc (noalloc.hip).
+bool noalloc_scratch_init(int device);
 }  // namespace q3x

@@ -254,6 +254,12 @@ class PlanewaveHamiltonianApplicator:
         check(lib().q3x_hamiltonian_apply_device(self.h, ctypes.c_void_p(d_psi), ctypes.c_void_p(d_hpsi), nvec,
                                                  precision))
 
+    def apply_noalloc(self, d_psi: int, d_hpsi: int, nvec: int, precision: int = FP64):
+        """apply_device with every scratch buffer in the library's fixed per-GPU array: no device
+        allocation (for callers that cannot allocate).  Calls overlapping on one GPU race."""
+        check(lib().q3x_hamiltonian_apply_noalloc(self.h, ctypes.c_void_p(d_psi), ctypes.c_void_p(d_hpsi), nvec,
+                                                  precision))
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().q3x_hamiltonian_destroy(self.h)
