@@ -103,6 +103,141 @@ def kernel_bytes(name, nvec, npw, nst, n3, npts, cs, sv):
     return None
 
 
+FIXTURE_CFG2 = os.path.join(ROOT, "tests", "golden", "cfg2_eigs.npz")
+HA_PER_RY = 0.5
+
+
+def _solve(ctx, h, nb, nocc, psi0, prec, tol, iters):
+    from q3_amd import PlanewaveDavidsonDiagonalizer
+    d = PlanewaveDavidsonDiagonalizer(h)
+    d.set_tolerance(tol)
+    d.set_maximum_iterations(iters)
+    ctx.synchronize()
+    t = time.perf_counter()
+    eig, _ = d.solve(nb, nocc, psi0, precision=prec)
+    dt = time.perf_counter() - t
+    rep = d.report_dict()
+    # without profiling the library's stage clock reads host enqueue time, not device time
+    rep["host_enqueue_stage_ms"] = rep.pop("stage_ms")
+    return eig, dt, rep
+
+
+def davidson_section(args, ctx, h, w, psi0):
+    """cfg2 Davidson solves from the fixed starting block (seed 2), measured against the FP64
+    oracle's eigenvalues (tests/golden/cfg2_eigs.npz: the oracle's block Davidson held to 1e-10 Ry
+    on every band).
+
+    reference_settings: the reference's convergence test as the SCF calls it, 128 occupied bands
+    to 1e-6 Ha (2e-6 Ry) and the 32 empty bands relaxed to max(5 tol, 1e-5 Ry)
+    (davidson.hpp:607-629).  all_bands_1e-6_ha: the time to all 160 bands within 1e-6 Ha of the
+    oracle; every band is held to the tolerance (n_occupied = nb) and the tolerance is taken from
+    the ladder 2e-6, 2e-7, 2e-8 Ry until the returned eigenvalues meet the bar (the |de| test can
+    stop a slowly converging top band short of it); every attempt is reported."""
+    from q3_amd import FP64, MIXED
+    nb, nocc = w.nbands, w.noccupied
+    fix = np.load(FIXTURE_CFG2)["eigenvalues_ry"]
+    for prec in (FP64, MIXED):  # untimed warm-up: first launches load kernels, workspaces grow
+        _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, 2)
+    out = {"oracle_fixture": "tests/golden/cfg2_eigs.npz (FP64 oracle Davidson, every band to 1e-10 Ry)"}
+    ref = {}
+    for prec, label in ((FP64, "fp64"), (MIXED, "mixed")):
+        eig, dt, rep = _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, args.davidson_iters)
+        err = np.abs(eig - fix) * HA_PER_RY
+        ref[label] = {"s_per_solve": dt, "tolerance_ry": 2e-6, "n_occupied": nocc, **rep,
+                      "max_abs_err_ha_occupied": float(err[:nocc].max()), "max_abs_err_ha_all": float(err.max())}
+        log(f"davidson {label} (reference settings): {dt:.3f}s {rep['iterations']} it, occupied err "
+            f"{err[:nocc].max():.2e} Ha, all {err.max():.2e} Ha")
+    ref["speedup_mixed_vs_fp64"] = ref["fp64"]["s_per_solve"] / ref["mixed"]["s_per_solve"]
+    out["reference_settings"] = ref
+    allb = {}
+    for prec, label in ((FP64, "fp64"), (MIXED, "mixed")):
+        attempts = []
+        for tol in (2e-6, 2e-7, 2e-8):
+            eig, dt, rep = _solve(ctx, h, nb, nb, psi0, prec, tol, 4 * args.davidson_iters)
+            err = float(np.abs(eig - fix).max() * HA_PER_RY)
+            attempts.append({"tolerance_ry": tol, "s_per_solve": dt, "iterations": rep["iterations"],
+                             "max_abs_err_ha_all": err})
+            log(f"davidson {label} all bands at {tol:g} Ry: {dt:.3f}s {rep['iterations']} it, max err {err:.2e} Ha")
+            if err <= 1e-6:
+                break
+        ok = attempts[-1]["max_abs_err_ha_all"] <= 1e-6
+        allb[label] = {"s_per_solve": attempts[-1]["s_per_solve"] if ok else None, "met_1e-6_ha": ok,
+                       "tolerance_ry": attempts[-1]["tolerance_ry"], "attempts": attempts}
+    if allb["fp64"]["met_1e-6_ha"] and allb["mixed"]["met_1e-6_ha"]:
+        allb["speedup_mixed_vs_fp64"] = allb["fp64"]["s_per_solve"] / allb["mixed"]["s_per_solve"]
+    out["all_bands_1e-6_ha"] = allb
+    return out
+
+
+def cpu_baseline(basis, kp, V, beta, bs, db, psi0):
+    """The FP64 CPU oracle (a restatement of the reference path, OpenMP over bands, own mixed-radix
+    FFT; not FFTW/MKL-tuned) on this host: cfg2 H|psi> on every host thread and on one (median of
+    5 repeats each), and a Davidson stage profile of one cfg4 k-point (48 bands, 1e-6 Ha) on every
+    host thread.  A reported baseline only."""
+    from oracle import oracle as O
+    from q3_amd import workloads as W
+    nthreads = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        nthreads = min(nthreads, int(env))
+
+    def hpsi_rate(nbands, threads, reps=5):
+        O.set_threads(threads)
+        ps = np.asfortranarray(psi0[:, :nbands])
+        O.hamiltonian_apply(basis.wave_dims, kp.fft_map, kp.kinetic_energy_squared, V, beta, bs, db, ps)  # warm
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            O.hamiltonian_apply(basis.wave_dims, kp.fft_map, kp.kinetic_energy_squared, V, beta, bs, db, ps)
+            ts.append(time.perf_counter() - t)
+        return nbands / float(np.median(ts)), ts
+
+    try:
+        all_rate, all_ts = hpsi_rate(16, nthreads)
+        one_rate, one_ts = hpsi_rate(4, 1)
+        log(f"cpu oracle: {all_rate:.1f} Hpsi/s on {nthreads} threads, {one_rate:.2f} on 1")
+        # Davidson stage profile of one cfg4 k-point (the CPU cannot run a cfg2 solve in a bench)
+        w4 = W.cfg4()
+        from q3_amd.planewave import PlanewaveBasis
+        b4 = PlanewaveBasis(w4.crystal, w4.ecutwfc_ry, 0.0, True, w4.fft_factors)
+        k4 = b4.compute_gk(w4.kpoints[1])
+        V4 = W.local_potential(w4.crystal, b4, w4.species)
+        beta4 = k4.kb_projectors(w4.crystal, w4.species)
+        bs4, db4 = W.d_blocks_for(w4.crystal, w4.species)
+        O.set_threads(nthreads)
+        t = time.perf_counter()
+        _, _, rep = O.davidson(b4.wave_dims, k4.fft_map, k4.kinetic_energy_squared, V4, beta4, bs4, db4, w4.nbands,
+                               w4.noccupied, W.random_psi(k4.kinetic_energy_squared, w4.nbands, seed=101), tol=2e-6,
+                               max_iter=60)
+        dav_s = time.perf_counter() - t
+        log(f"cpu oracle Davidson cfg4 k1: {dav_s:.2f}s {rep['iterations']} iterations")
+        return {"value": all_rate, "unit": "Hpsi/s", "cores": nthreads, "kind": "port",
+                "sample": f"cfg2 H|psi> of 16 of the 160 bands, median of 5 (FP64 oracle, OpenMP over bands, "
+                          f"{nthreads} threads)",
+                "one_thread": {"value": one_rate, "unit": "Hpsi/s", "cores": 1,
+                               "sample": "cfg2 H|psi> of 4 bands, median of 5"},
+                "repeats_s": {"all_threads": all_ts, "one_thread": one_ts},
+                "cpu_model": _cpu_model(),
+                "davidson_cfg4_kpoint": {"s_per_solve": dav_s, "threads": nthreads, "iterations": rep["iterations"],
+                                         "stage_ms": rep["stage_ms"],
+                                         "sample": "cfg4 k-point 1 (npw %d), 48 bands, 2e-6 Ry, seed-101 start"
+                                                   % k4.npw}}
+    except Exception as e:  # reported, not hidden
+        log(f"cpu baseline failed: {e}")
+        return {"error": str(e)}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def batched_kpoints(args):
     """64 random k-points of the 8-atom cubic Si cell (20 Ha, 48^3, 48 bands, seed 3) solved as
     one batch over worker streams, FP64 and mixed, beside the same batch on a single worker."""
@@ -149,16 +284,21 @@ def batched_kpoints(args):
         return {"error": str(e)}
 
 
-def batched_gamma(args):
-    """Throughput of independent solves of the 64-atom Gamma problem (cfg2, 1e-6 Ha): N copies
-    over W worker streams in FP64 and in mixed precision, against the same N solved one after
-    another in FP64 on one worker (the plain FP64 path).  A single solve leaves most CUs idle
-    in its FP64 subspace eigensolve, which concurrent solves fill."""
+def batched_gamma(args, tolerances=None):
+    """Throughput of independent solves of the 64-atom Gamma problem (cfg2) with every band held
+    to the tolerance (n_occupied = nb): N copies over W worker streams in FP64 and in mixed
+    precision, against the same N solved one after another in FP64 on one worker (the plain FP64
+    path).  Tolerances per precision are the ones the single-solve section found to bring all 160
+    bands within 1e-6 Ha of the oracle; every solve's eigenvalues are checked against the oracle
+    fixture.  A single solve leaves most CUs idle in its FP64 subspace eigensolve, which
+    concurrent solves fill."""
     from q3_amd import FP64, MIXED, KPointBatch
     from q3_amd import workloads as W
     from q3_amd.planewave import PlanewaveBasis
+    tolerances = tolerances or {}
     try:
         w = W.cfg2()
+        fix = np.load(FIXTURE_CFG2)["eigenvalues_ry"]
         basis = PlanewaveBasis(w.crystal, w.ecutwfc_ry, 0.0, True, w.fft_factors)
         kp = basis.compute_gk(w.kpoints[0])
         V = W.local_potential(w.crystal, basis, w.species)
@@ -167,24 +307,23 @@ def batched_gamma(args):
         n = args.gamma_solves
         psi0 = W.random_psi(kp.kinetic_energy_squared, w.nbands, seed=2)  # the cfg2 starting block
         out = {"workload": f"{n} independent cfg2 solves (Si64 Gamma, 25 Ha, 96^3, 160 bands, random damped "
-                           f"psi0 seed 2) at 2e-6 Ry", "solves": n}
-        eigs = {}
+                           f"psi0 seed 2), every band held to the tolerance", "solves": n}
         for label, prec, workers in (("fp64_one_worker", FP64, 1), ("fp64", FP64, args.gamma_workers),
                                      ("mixed", MIXED, args.gamma_workers)):
+            tol = tolerances.get("mixed" if prec == MIXED else "fp64", 2e-7)
             batch = KPointBatch(workers=workers)
             for _ in range(n):
                 batch.add(basis, kp, V, beta, bs, db)
-            batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=[psi0] * n)  # warm-up
+            batch.solve(w.nbands, w.nbands, tol, 4 * args.davidson_iters, prec, psi0=[psi0] * n)  # warm-up
             t = time.perf_counter()
-            e, _ = batch.solve(w.nbands, w.noccupied, 2e-6, args.davidson_iters, prec, psi0=[psi0] * n)
+            e, _ = batch.solve(w.nbands, w.nbands, tol, 4 * args.davidson_iters, prec, psi0=[psi0] * n)
             dt = time.perf_counter() - t
-            eigs[label] = e
-            out[label] = {"workers": workers, "s_all": dt, "solves_per_s": n / dt,
-                          "iterations_mean": float(np.mean([r.iterations for r in batch.reports]))}
-            log(f"batched cfg2 {label}: {dt:.3f}s for {n} solves ({workers} workers)")
+            err = float(np.abs(np.asarray(e) - fix[None, :]).max() * HA_PER_RY)
+            out[label] = {"workers": workers, "s_all": dt, "solves_per_s": n / dt, "tolerance_ry": tol,
+                          "iterations_mean": float(np.mean([r.iterations for r in batch.reports])),
+                          "max_abs_err_ha_all_vs_oracle": err}
+            log(f"batched cfg2 {label}: {dt:.3f}s for {n} solves ({workers} workers), max err {err:.2e} Ha")
             batch.close()
-        out["mixed"]["max_abs_err_ha_occupied_vs_fp64"] = float(
-            np.abs(eigs["mixed"] - eigs["fp64"])[:, :w.noccupied].max() / 2)
         base = out["fp64_one_worker"]["solves_per_s"]
         out["throughput_fp64_batched_vs_plain"] = out["fp64"]["solves_per_s"] / base
         out["throughput_mixed_batched_vs_plain_fp64"] = out["mixed"]["solves_per_s"] / base
@@ -410,36 +549,7 @@ def main():
     davidson = None
     if not args.no_davidson:
         try:
-            # timed solves at 1e-6 Ha (2e-6 Ry) on the eigenvalue-change test, and an untimed FP64
-            # reference converged to 1e-10 Ry that both are measured against
-            davidson = {}
-            for prec in (FP64, MIXED):  # untimed warm-up: first launches load kernels, workspaces grow
-                d = PlanewaveDavidsonDiagonalizer(h)
-                d.set_tolerance(2e-6)
-                d.set_maximum_iterations(2)
-                d.solve(nb, w.noccupied, psi0, precision=prec)
-            runs = ((FP64, "fp64", 2e-6, args.davidson_iters), (MIXED, "mixed", 2e-6, args.davidson_iters),
-                    (FP64, "fp64_reference", 1e-10, 4 * args.davidson_iters))
-            for prec, label, tol, iters in runs:
-                d = PlanewaveDavidsonDiagonalizer(h)
-                d.set_tolerance(tol)
-                d.set_maximum_iterations(iters)
-                ctx.synchronize()
-                t = time.perf_counter()
-                eig, _ = d.solve(nb, w.noccupied, psi0, precision=prec)
-                dt = time.perf_counter() - t
-                davidson[label] = {"s_per_solve": dt, "tolerance_ry": tol, **d.report_dict(),
-                                   "eigenvalues_ry": [float(x) for x in eig]}
-                log(f"davidson {label}: {dt:.3f}s {d.report_dict()['iterations']} iterations")
-            ref = np.array(davidson["fp64_reference"]["eigenvalues_ry"])
-            nocc = w.noccupied
-            for lab in ("fp64", "mixed"):
-                e = np.array(davidson[lab]["eigenvalues_ry"])
-                davidson[lab]["max_abs_err_ha_occupied"] = float(np.abs(e - ref)[:nocc].max() / 2)
-                davidson[lab]["max_abs_err_ha_all"] = float(np.abs(e - ref).max() / 2)
-            davidson["speedup_mixed_vs_fp64"] = davidson["fp64"]["s_per_solve"] / davidson["mixed"]["s_per_solve"]
-            for lab in ("fp64", "mixed", "fp64_reference"):
-                davidson[lab].pop("eigenvalues_ry")
+            davidson = davidson_section(args, ctx, h, w, psi0)
         except Exception as e:  # reported, not hidden
             davidson = {"error": str(e)}
             log(f"davidson failed: {e}")
@@ -451,26 +561,13 @@ def main():
     d = h = None
     ctx.close()
 
-    batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args)}
+    tols = {}
+    if isinstance(davidson, dict) and "all_bands_1e-6_ha" in davidson:
+        tols = {k: v["tolerance_ry"] for k, v in davidson["all_bands_1e-6_ha"].items() if isinstance(v, dict)}
+    batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args, tols)}
     serve = None if args.no_serve else serving(args)
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        nsamp = 16
-        ps = np.asfortranarray(psi0[:, :nsamp])
-        t = time.perf_counter()
-        reps = 0
-        while True:
-            O.hamiltonian_apply(basis.wave_dims, kp.fft_map, kp.kinetic_energy_squared, V, beta, bs, db, ps)
-            reps += 1
-            if time.perf_counter() - t > 10.0 or reps >= 20:
-                break
-        dt = time.perf_counter() - t
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": nsamp * reps / dt, "unit": "Hpsi/s", "cores": min(threads, os.cpu_count() or 1),
-               "kind": "port", "sample": f"{reps} x H|psi> of {nsamp} of the 160 bands (cfg2, FP64 oracle, "
-                                         f"OpenMP over bands)"}
+    cpu = None if args.no_cpu else cpu_baseline(basis, kp, V, beta, bs, db, psi0)
 
     out = {
         "metric": "Hpsi_per_s",

@@ -65,6 +65,16 @@ q3x_status q3x_context_synchronize(q3x_context ctx);
 q3x_status q3x_basis_create(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
                             double ecutrho_ry, int use_exact_wavefunction_grid,
                             const int fft_factors[3], q3x_basis* out);
+/* The same with the Gamma trick (gamma_only != 0): the G list keeps one of each +-G pair, the half
+ * sphere i3 > 0, or i3 = 0 and i2 > 0, or i3 = i2 = 0 and i1 >= 0 (GVectorsGrid::generate with
+ * is_gamma_only, g_vectors_grid.hpp:56-76).  The reference switches it on for a 1x1x1 unshifted
+ * k grid (iterator/kohn_sham/planewave/init.hpp:585-610).  Its only k-point is (0, 0, 0); band
+ * blocks on it hold the half-sphere coefficients of real functions (psi(-G) = conj(psi(G)),
+ * G = 0 real and first), and every product over plane waves is the full-sphere real one. */
+q3x_status q3x_basis_create_gamma(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
+                                  double ecutrho_ry, int use_exact_wavefunction_grid,
+                                  const int fft_factors[3], int gamma_only, q3x_basis* out);
+q3x_status q3x_basis_is_gamma_only(q3x_basis b, int* gamma_only);
 q3x_status q3x_basis_info(q3x_basis b, int rho_dims[3], int wave_dims[3], int* number_of_g_vectors,
                           double* cell_volume, double* two_pi_over_alat_squared);
 /* miller: ng x 3 ints; g_cart: ng x 3 (2pi/alat units); g2: ng (2pi/alat)^2 units; any may be NULL */
@@ -78,6 +88,9 @@ q3x_status q3x_basis_destroy(q3x_basis b);
 q3x_status q3x_kpoint_create(q3x_basis b, const double k_crystal[3], q3x_kpoint* out, int* npw);
 /* g_index: npw ints into the basis G list; kinetic: |k+G|^2 in Ry; fft_map: wave-grid offsets */
 q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* fft_map);
+/* Gamma trick: wave-grid offsets of -G for each plane wave (the -G map of
+ * hamiltonian_applicator/serial.hpp:889-904); Q3X_ERR_STATE on a full-sphere k-point. */
+q3x_status q3x_kpoint_minus_map(q3x_kpoint k, int* minus_fft_map);
 /* Starting block for a solve: npw x nb complex128, column-major, real and imaginary parts
  * uniform in [-1, 1) (a splitmix64 hash of seed and the entry's position) damped by
  * 1 / (1 + |k+G|^2 [Ry]).  The band-structure server seeds k-point i of a request with seed i,
@@ -191,6 +204,18 @@ q3x_status q3x_hegv(int n, const double* H, const double* S, double* w, double* 
  * Q3X_ERR_DEVICE. */
 q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, const void* S, long lds,
                            int nvec, double* w, void* X, long ldx);
+/* No counterpart in the reference.  Test hook of the device eigensolver: the next `count`
+ * multi-workgroup tridiagonalisations (n >= 100) start with their grid-barrier give-up flag set,
+ * so each runs to its end without waiting and its solve is redone on one workgroup, as when
+ * another stream's work keeps a workgroup from being placed. */
+void q3x_debug_force_barrier_giveups(int count);
+/* No counterpart in the reference.  Test hook of the device eigensolver: bit i of mask set makes
+ * the i-th eigensolve from now (q3x_hegv_device or one inside a Davidson solve, counted in the
+ * process) report a non-positive-definite S (zhegv info > n) without solving, so that Davidson's
+ * recovery paths (forced collapse, previous eigenvalues kept) can be run on purpose. */
+void q3x_debug_fail_eigensolves(unsigned long long mask);
+/* Solves redone on one workgroup since the process started. */
+long q3x_eig_barrier_redos_total(void);
 
 /* ---- block Davidson (PlanewaveDavidsonDiagonalizer::solve,
  *      src/layer_4_computation/diagonalizer/planewave/davidson.hpp:324-841, with the subspace
@@ -220,6 +245,17 @@ typedef struct {
      * otherwise the solve synchronises only where the host needs device data and the stage
      * times are host-side. */
     double stage_ms[8];
+    /* subspace eigensolves redone on one workgroup because a grid barrier of the
+     * multi-workgroup tridiagonalisation gave up (other streams' work kept one of its
+     * workgroups from being placed) */
+    int eig_barrier_redos;
+    /* recovery paths taken (davidson.hpp:520-590, subspace_manager/serial.hpp:370-552):
+     * Cholesky-QR failures finished by modified Gram-Schmidt, subspace collapses forced by a
+     * failed subspace eigensolve, and iterations that kept the previous eigenvalues because the
+     * eigensolve failed again after such a collapse */
+    int mgs_fallbacks;
+    int recovery_collapses;
+    int previous_eigenvalue_keeps;
 } q3x_davidson_report;
 
 void q3x_davidson_default_options(q3x_davidson_options* opt);
@@ -290,6 +326,8 @@ typedef struct {
     int iterations;                    /* Davidson iterations summed over the k-points */
     double setup_s;                    /* worker time in per-k-point setup (G list, projectors, upload) */
     double solve_s;                    /* worker time in the Davidson solves */
+    int eig_barrier_redos;             /* subspace eigensolves redone on one workgroup (see
+                                          q3x_davidson_report), summed over the k-points */
 } q3x_request_stats;
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation */
 q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out);

@@ -66,6 +66,16 @@ def lib():
         L.orc_davidson.restype = _i
         L.orc_davidson.argtypes = [_p, _i, _p, _p, _p, _i, _p, _i, _p, _p, _i, _i, _p, _i, _p, _d,
                                    _i, _i, _p]
+        L.orc_davidson_stage_ms.argtypes = [_p]
+        L.orc_basis_create2.restype = _p
+        L.orc_basis_create2.argtypes = [_p, _d, _d, _d, _i, _p, _i]
+        L.orc_kpoint_minus_map.restype = _i
+        L.orc_kpoint_minus_map.argtypes = [_p, _p, _p, _i]
+        L.orc_hamiltonian_apply2.argtypes = [_p, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _i, _p, _p]
+        L.orc_davidson2.restype = _i
+        L.orc_davidson2.argtypes = [_p, _i, _p, _p, _p, _p, _i, _p, _i, _p, _p, _i, _i, _p, _i, _p, _d, _i, _i, _p]
+        L.orc_set_threads.argtypes = [_i]
+        L.orc_get_threads.restype = _i
         _lib = L
     return _lib
 
@@ -102,10 +112,12 @@ def grid_dims(lat, alat, ecut, factors=(1, 1, 1)):
 class Basis:
     """G sphere and FFT grids (PlanewaveBasis::generate)."""
 
-    def __init__(self, lat, alat, ecutwfc, ecutrho=0.0, use_exact=True, factors=(1, 1, 1)):
+    def __init__(self, lat, alat, ecutwfc, ecutrho=0.0, use_exact=True, factors=(1, 1, 1), gamma_only=False):
         self._lat = _c(lat, np.float64)
         f = _c(factors, np.int32)
-        self.h = lib().orc_basis_create(_ptr(self._lat), alat, ecutwfc, ecutrho, int(use_exact), _ptr(f))
+        self.gamma_only = bool(gamma_only)
+        self.h = lib().orc_basis_create2(_ptr(self._lat), alat, ecutwfc, ecutrho, int(use_exact), _ptr(f),
+                                         int(gamma_only))
         rho = np.zeros(3, np.int32)
         wave = np.zeros(3, np.int32)
         ng = np.zeros(1, np.int32)
@@ -129,6 +141,13 @@ class Basis:
         fm = np.zeros(cap, np.int32)
         n = lib().orc_kpoint(self.h, _ptr(k), _ptr(gidx), _ptr(ek), _ptr(fm), cap)
         return gidx[:n].copy(), ek[:n].copy(), fm[:n].copy()
+
+    def minus_map(self, k):
+        """Box offsets of -G for the k-point's plane waves (the Gamma trick's mirror map)."""
+        k = _c(k, np.float64)
+        mm = np.zeros(self.ng, np.int32)
+        n = lib().orc_kpoint_minus_map(self.h, _ptr(k), _ptr(mm), self.ng)
+        return mm[:n].copy()
 
     def kb_projectors(self, k, gidx, atom_pos, atom_species, species):
         """species: list of dicts with r, rab, l (list), cutoff (list), beta_r (nbeta x nmesh)."""
@@ -222,14 +241,16 @@ def _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks):
     return dims, fm, ek, Vr, b, nproj, bs, db
 
 
-def hamiltonian_apply(dims, fftmap, ekin, V, beta, block_sizes, dblocks, psi):
-    """psi: npw x nvec (column-major); returns H psi of the same shape."""
+def hamiltonian_apply(dims, fftmap, ekin, V, beta, block_sizes, dblocks, psi, minus_map=None):
+    """psi: npw x nvec (column-major); returns H psi of the same shape.  With minus_map (the -G box
+    offsets of a Gamma-trick basis) psi holds half-sphere vectors of real functions."""
     dims, fm, ek, Vr, b, nproj, bs, db = _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks)
     psi = np.asfortranarray(psi, dtype=np.complex128)
     npw, nvec = psi.shape
     out = np.zeros_like(psi, order="F")
-    lib().orc_hamiltonian_apply(_ptr(dims), npw, _ptr(fm), _ptr(ek), _ptr(Vr), nproj, _ptr(b), len(bs),
-                                _ptr(bs), _ptr(db), nvec, _ptr(psi), _ptr(out))
+    mm = None if minus_map is None else _c(minus_map, np.int32)
+    lib().orc_hamiltonian_apply2(_ptr(dims), npw, _ptr(fm), None if mm is None else _ptr(mm), _ptr(ek), _ptr(Vr),
+                                 nproj, _ptr(b), len(bs), _ptr(bs), _ptr(db), nvec, _ptr(psi), _ptr(out))
     return out
 
 
@@ -270,7 +291,7 @@ def zhegv(H, S):
 
 
 def davidson(dims, fftmap, ekin, V, beta, block_sizes, dblocks, nb, nocc, psi0=None, tol=1e-8,
-             max_iter=20, factor=2):
+             max_iter=20, factor=2, minus_map=None):
     """Returns (eigenvalues, psi (npw x nb), report dict)."""
     dims, fm, ek, Vr, b, nproj, bs, db = _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks)
     npw = len(ek)
@@ -282,6 +303,26 @@ def davidson(dims, fftmap, ekin, V, beta, block_sizes, dblocks, nb, nocc, psi0=N
         seed = 1
     eig = np.zeros(nb)
     rep = np.zeros(3, np.int32)
-    lib().orc_davidson(_ptr(dims), npw, _ptr(fm), _ptr(ek), _ptr(Vr), nproj, _ptr(b), len(bs), _ptr(bs),
-                       _ptr(db), nb, nocc, _ptr(psi), seed, _ptr(eig), tol, max_iter, factor, _ptr(rep))
-    return eig, psi, {"iterations": int(rep[0]), "unconverged": int(rep[1]), "h_applications": int(rep[2])}
+    mm = None if minus_map is None else _c(minus_map, np.int32)
+    lib().orc_davidson2(_ptr(dims), npw, _ptr(fm), None if mm is None else _ptr(mm), _ptr(ek), _ptr(Vr), nproj,
+                        _ptr(b), len(bs), _ptr(bs), _ptr(db), nb, nocc, _ptr(psi), seed, _ptr(eig), tol, max_iter,
+                        factor, _ptr(rep))
+    st = np.zeros(8)
+    lib().orc_davidson_stage_ms(_ptr(st))
+    stages = dict(zip(DAVIDSON_STAGES, (float(x) for x in st)))
+    return eig, psi, {"iterations": int(rep[0]), "unconverged": int(rep[1]), "h_applications": int(rep[2]),
+                      "stage_ms": stages}
+
+
+# stage names of DavidsonTimingAccumulator (counters.hpp:36-44), in the order orc_davidson_stage_ms fills
+DAVIDSON_STAGES = ("seed", "ritz_vectors", "residuals", "orthogonalize", "hamiltonian_application",
+                   "subspace_update", "collapse", "diagonalize")
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the oracle's parallel loops (bands of H|psi>, GEMM columns)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())

@@ -25,6 +25,7 @@
 //   orc_davidson                          dense eigh of the brute-force H (cfg1-sized)
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -163,6 +164,7 @@ struct GVec {
 
 struct Basis {
     Crystal c;
+    bool gamma = false;  // half sphere (g_vectors_grid.hpp:56-76)
     double ecutwfc = 0, ecutrho = 0;
     int rho[3] = {0, 0, 0}, wave[3] = {0, 0, 0};
     std::vector<GVec> gv;
@@ -177,6 +179,11 @@ void generate_gvectors(Basis& b) {
     for (int i1 = -mx1; i1 <= mx1; i1++)
         for (int i2 = -mx2; i2 <= mx2; i2++)
             for (int i3 = -mx3; i3 <= mx3; i3++) {
+                if (b.gamma) {  // g_vectors_grid.hpp:69-76: i3 > 0, or i3 = 0 and i2 > 0, or both 0 and i1 >= 0
+                    if (i3 < 0) continue;
+                    if (i3 == 0 && i2 < 0) continue;
+                    if (i3 == 0 && i2 == 0 && i1 < 0) continue;
+                }
                 V3 g;
                 for (int d = 0; d < 3; d++) g[d] = i1 * R.r[0][d] + i2 * R.r[1][d] + i3 * R.r[2][d];
                 double g2 = g.norm2();
@@ -622,6 +629,7 @@ struct Ham {
     std::vector<size_t> boff;
     std::vector<cplx> dblocks;    // per atom, column-major block (serial.hpp:603-628)
     Fft3d fft;
+    const int* minusmap = nullptr;  // Gamma trick: box offset of -G per plane wave (serial.hpp:889-904)
 };
 
 void ham_apply(const Ham& h, int nvec, const cplx* psi, cplx* hpsi) {
@@ -663,6 +671,98 @@ void ham_apply(const Ham& h, int nvec, const cplx* psi, cplx* hpsi) {
     }
     // serial.hpp:688-700  Hpsi += beta W
     zgemm('N', npw, nvec, h.nproj, 1.0, h.beta, npw, W.data(), h.nproj, 1.0, hpsi, npw);
+}
+
+// Gamma trick (hamiltonian_applicator/serial.hpp:283-418, 519-566): psi holds the half sphere of
+// a real function, psi(-G) = conj(psi(G)).  Two bands share one complex transform (psi_1 + i psi_2
+// at +G, conj(psi_1) + i conj(psi_2) at -G, the real parts of G = 0), separated after the forward
+// transform by the +-G symmetry; an odd last band goes alone.  The projector overlaps are
+// 2 Re(beta^H psi) less the G = 0 term.  G = 0 is plane wave 0 (it sorts first at k = 0).
+void ham_apply_gamma(const Ham& h, int nvec, const cplx* psi, cplx* hpsi) {
+    int npw = h.npw;
+    const int npairs = (nvec + 1) / 2;
+#pragma omp parallel
+    {
+        std::vector<cplx> box(h.N);
+#pragma omp for schedule(static)
+        for (int pi = 0; pi < npairs; pi++) {
+            const int b1 = 2 * pi, b2 = 2 * pi + 1;
+            const bool two = b2 < nvec;
+            const cplx* p1 = psi + static_cast<size_t>(b1) * npw;
+            cplx* h1 = hpsi + static_cast<size_t>(b1) * npw;
+            const cplx* p2 = two ? psi + static_cast<size_t>(b2) * npw : nullptr;
+            cplx* h2 = two ? hpsi + static_cast<size_t>(b2) * npw : nullptr;
+            for (int g = 0; g < npw; g++) h1[g] = h.ekin[g] * p1[g];
+            if (two)
+                for (int g = 0; g < npw; g++) h2[g] = h.ekin[g] * p2[g];
+            std::fill(box.begin(), box.end(), cplx(0, 0));
+            if (two) {
+                if (h.fftmap[0] >= 0) box[h.fftmap[0]] = cplx(p1[0].real(), p2[0].real());
+                for (int g = 1; g < npw; g++) {
+                    const int fp = h.fftmap[g], fm = h.minusmap[g];
+                    if (fp >= 0) {
+                        box[fp] = cplx(p1[g].real() - p2[g].imag(), p1[g].imag() + p2[g].real());
+                        box[fm] = cplx(p1[g].real() + p2[g].imag(), -p1[g].imag() + p2[g].real());
+                    }
+                }
+            } else {
+                for (int g = 0; g < npw; g++)
+                    if (h.fftmap[g] >= 0) box[h.fftmap[g]] = p1[g];
+                for (int g = 1; g < npw; g++) box[h.minusmap[g]] = std::conj(p1[g]);
+            }
+            h.fft.run(box.data(), +1);
+            for (int r = 0; r < h.N; r++) box[r] *= h.vscaled[r];
+            h.fft.run(box.data(), -1);
+            if (two) {
+                if (h.fftmap[0] >= 0) {
+                    const cplx f0 = box[h.fftmap[0]];
+                    h1[0] += cplx(f0.real(), 0.0);
+                    h2[0] += cplx(f0.imag(), 0.0);
+                }
+                for (int g = 1; g < npw; g++) {
+                    const int fp = h.fftmap[g], fm = h.minusmap[g];
+                    if (fp >= 0) {
+                        const cplx plus = box[fp], mconj = std::conj(box[fm]);
+                        h1[g] += (plus + mconj) * 0.5;
+                        const cplx diff = (plus - mconj) * 0.5;
+                        h2[g] += cplx(diff.imag(), -diff.real());
+                    }
+                }
+            } else {
+                for (int g = 0; g < npw; g++)
+                    if (h.fftmap[g] >= 0) h1[g] += box[h.fftmap[g]];
+            }
+        }
+    }
+    if (h.nproj == 0) return;
+    std::vector<cplx> P(static_cast<size_t>(h.nproj) * nvec), W(static_cast<size_t>(h.nproj) * nvec, cplx(0, 0));
+    zgemm('C', h.nproj, nvec, npw, 1.0, h.beta, npw, psi, npw, 0.0, P.data(), h.nproj);
+    for (int v = 0; v < nvec; v++)  // serial.hpp:519-556: 2 Re(.) less the G = 0 term, real
+        for (int p = 0; p < h.nproj; p++) {
+            const size_t i = p + static_cast<size_t>(v) * h.nproj;
+            const double g0 = (std::conj(h.beta[static_cast<size_t>(p) * npw]) * psi[static_cast<size_t>(v) * npw]).real();
+            P[i] = cplx(2.0 * P[i].real() - g0, 0.0);
+        }
+    for (int a = 0; a < h.natoms; a++) {
+        int bs = h.bsize[a];
+        if (bs == 0) continue;
+        const cplx* D = h.dblocks.data() + h.boff[a];
+        int s0 = h.bstart[a];
+        for (int v = 0; v < nvec; v++)
+            for (int i = 0; i < bs; i++) {
+                cplx s(0, 0);
+                for (int j = 0; j < bs; j++) s += D[i + static_cast<size_t>(j) * bs] * P[s0 + j + static_cast<size_t>(v) * h.nproj];
+                W[s0 + i + static_cast<size_t>(v) * h.nproj] = s;
+            }
+    }
+    zgemm('N', npw, nvec, h.nproj, 1.0, h.beta, npw, W.data(), h.nproj, 1.0, hpsi, npw);
+}
+
+void ham_apply_any(const Ham& h, int nvec, const cplx* psi, cplx* hpsi) {
+    if (h.minusmap)
+        ham_apply_gamma(h, nvec, psi, hpsi);
+    else
+        ham_apply(h, nvec, psi, hpsi);
 }
 
 // ---------------------------------------------------------------------------
@@ -850,9 +950,10 @@ void orc_grid_dims(const double* lat9, double alat, double ecut, const int* fact
 }
 
 // pair.hpp:24-100 + init.hpp:60-96 + g_vectors_grid.hpp:23-104
-void* orc_basis_create(const double* lat9, double alat, double ecutwfc, double ecutrho, int use_exact,
-                       const int* factors) {
+void* orc_basis_create2(const double* lat9, double alat, double ecutwfc, double ecutrho, int use_exact,
+                        const int* factors, int gamma_only) {
     Basis* b = new Basis;
+    b->gamma = gamma_only != 0;
     b->c.init(lat9, alat);
     b->ecutwfc = ecutwfc;
     b->ecutrho = ecutrho <= 0.0 ? 4.0 * ecutwfc : ecutrho;
@@ -861,6 +962,33 @@ void* orc_basis_create(const double* lat9, double alat, double ecutwfc, double e
     grid_dimensions(smooth, b->c.tpa2, b->c.lat, b->c.recip, factors, b->wave);
     generate_gvectors(*b);
     return b;
+}
+
+void* orc_basis_create(const double* lat9, double alat, double ecutwfc, double ecutrho, int use_exact,
+                       const int* factors) {
+    return orc_basis_create2(lat9, alat, ecutwfc, ecutrho, use_exact, factors, 0);
+}
+
+// Box offset of -G for each plane wave of the k-point list (serial.hpp:889-904; -1 outside the box).
+int orc_kpoint_minus_map(void* h, const double* k, int* minusmap, int cap) {
+    Basis* b = static_cast<Basis*>(h);
+    KPoint kp;
+    select_kpoint(*b, k, kp);
+    int n = static_cast<int>(kp.gidx.size());
+    const int* d = b->wave;
+    for (int i = 0; i < n && i < cap; i++) {
+        const GVec& gv = b->gv[kp.gidx[i]];
+        int m1 = -gv.m[0], m2 = -gv.m[1], m3 = -gv.m[2];
+        if (std::abs(m1) <= d[0] / 2 && std::abs(m2) <= d[1] / 2 && std::abs(m3) <= d[2] / 2) {
+            int w1 = m1 >= 0 ? m1 : m1 + d[0];
+            int w2 = m2 >= 0 ? m2 : m2 + d[1];
+            int w3 = m3 >= 0 ? m3 : m3 + d[2];
+            minusmap[i] = (w1 * d[1] + w2) * d[2] + w3;
+        } else {
+            minusmap[i] = -1;
+        }
+    }
+    return n;
 }
 
 void orc_basis_destroy(void* h) { delete static_cast<Basis*>(h); }
@@ -908,11 +1036,12 @@ void orc_fft3d(const int* dims, double* data, int sign) {
 
 // H|psi> for nvec vectors. V is the unscaled smooth potential V(r) (Ry).
 // dblocks: per-atom square blocks, column-major, packed in atom order (real values).
-void orc_hamiltonian_apply(const int* dims, int npw, const int* fftmap, const double* ekin,
-                           const double* V, int nproj, const double* beta, int natoms,
-                           const int* block_sizes, const double* dblocks, int nvec,
-                           const double* psi, double* hpsi) {
+void orc_hamiltonian_apply2(const int* dims, int npw, const int* fftmap, const int* minusmap, const double* ekin,
+                            const double* V, int nproj, const double* beta, int natoms,
+                            const int* block_sizes, const double* dblocks, int nvec,
+                            const double* psi, double* hpsi) {
     Ham h;
+    h.minusmap = minusmap;
     for (int i = 0; i < 3; i++) h.dims[i] = dims[i];
     h.N = dims[0] * dims[1] * dims[2];
     h.npw = npw;
@@ -939,7 +1068,15 @@ void orc_hamiltonian_apply(const int* dims, int npw, const int* fftmap, const do
         start += bs;
     }
     h.fft.init(dims);
-    ham_apply(h, nvec, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(hpsi));
+    ham_apply_any(h, nvec, reinterpret_cast<const cplx*>(psi), reinterpret_cast<cplx*>(hpsi));
+}
+
+void orc_hamiltonian_apply(const int* dims, int npw, const int* fftmap, const double* ekin,
+                           const double* V, int nproj, const double* beta, int natoms,
+                           const int* block_sizes, const double* dblocks, int nvec,
+                           const double* psi, double* hpsi) {
+    orc_hamiltonian_apply2(dims, npw, fftmap, nullptr, ekin, V, nproj, beta, natoms, block_sizes, dblocks, nvec, psi,
+                           hpsi);
 }
 
 // Beta table for one species: nq points on dq=0.01 up to 1.1*sqrt(ecutrho)
@@ -1183,11 +1320,32 @@ namespace {
 struct Dav {
     const Ham* h;
     int npw, nb, nocc, mmax;
+    bool gamma = false;  // half-sphere vectors of real functions (subspace_manager/serial.hpp gamma paths)
     std::vector<cplx> V, HV, Hs, Ss;
     int happly = 0;
     void apply(int first, int n) {
-        ham_apply(*h, n, V.data() + static_cast<size_t>(first) * npw, HV.data() + static_cast<size_t>(first) * npw);
+        ham_apply_any(*h, n, V.data() + static_cast<size_t>(first) * npw, HV.data() + static_cast<size_t>(first) * npw);
         happly += n;
+    }
+    // C (m x n, ld ldc) = A^H B over the plane waves; at Gamma the full-sphere real product
+    // 2 Re(A^H B) less the G = 0 term (serial.hpp:575-640, 1280-1300), with zero imaginary parts.
+    void gram(int m, int n, const cplx* A, const cplx* B, cplx* C, int ldc) {
+        zgemm('C', m, n, npw, 1.0, A, npw, B, npw, 0.0, C, ldc);
+        if (!gamma) return;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < m; i++) {
+                cplx& c = C[i + static_cast<size_t>(j) * ldc];
+                const double g0 = (std::conj(A[static_cast<size_t>(i) * npw]) * B[static_cast<size_t>(j) * npw]).real();
+                c = cplx(2.0 * c.real() - g0, 0.0);
+            }
+    }
+    // enforce_gamma_g0_real (subspace_manager/serial.hpp:1189-1200)
+    void g0_real(cplx* X, int n) {
+        if (!gamma) return;
+        for (int v = 0; v < n; v++) {
+            cplx& g0 = X[static_cast<size_t>(v) * npw];
+            g0 = cplx(g0.real(), 0.0);
+        }
     }
     // serial.hpp:41-553 (S = I)
     int orthogonalize(int start, int ncand) {
@@ -1195,11 +1353,11 @@ struct Dav {
         cplx* cand = V.data() + static_cast<size_t>(start) * npw;
         if (start > 0) {
             std::vector<cplx> C(static_cast<size_t>(start) * ncand);
-            zgemm('C', start, ncand, npw, 1.0, V.data(), npw, cand, npw, 0.0, C.data(), start);
+            gram(start, ncand, V.data(), cand, C.data(), start);
             zgemm('N', npw, ncand, start, -1.0, V.data(), npw, C.data(), start, 1.0, cand, npw);
         }
         std::vector<cplx> G(static_cast<size_t>(ncand) * ncand);
-        zgemm('C', ncand, ncand, npw, 1.0, cand, npw, cand, npw, 0.0, G.data(), ncand);
+        gram(ncand, ncand, cand, cand, G.data(), ncand);
         if (zpotrf_upper(ncand, G.data(), ncand) == 0) {
             ztrsm_right_upper(npw, ncand, G.data(), ncand, cand, npw);
             return ncand;
@@ -1214,11 +1372,12 @@ struct Dav {
             int nex = dst - start;
             if (nex > 0) {
                 std::vector<cplx> co(nex);
-                zgemm('C', nex, 1, npw, 1.0, cand, npw, cur, npw, 0.0, co.data(), nex);
+                gram(nex, 1, cand, cur, co.data(), nex);
                 zgemm('N', npw, 1, nex, -1.0, cand, npw, co.data(), nex, 1.0, cur, npw);
             }
             double n2 = 0;
             for (int g = 0; g < npw; g++) n2 += std::norm(cur[g]);
+            if (gamma) n2 = 2.0 * n2 - std::norm(cur[0]);  // gamma_aware_real_dot (serial.hpp:1202-1226)
             if (n2 > kNormThreshold) {
                 double inv = 1.0 / std::sqrt(n2);
                 for (int g = 0; g < npw; g++) cur[g] *= inv;
@@ -1229,14 +1388,18 @@ struct Dav {
     }
     // serial.hpp:555-678
     void update(int start, int nnew, int total) {
-        zgemm('C', total, nnew, npw, 1.0, V.data(), npw, HV.data() + static_cast<size_t>(start) * npw, npw, 0.0,
-              Hs.data() + static_cast<size_t>(start) * mmax, mmax);
-        zgemm('C', total, nnew, npw, 1.0, V.data(), npw, V.data() + static_cast<size_t>(start) * npw, npw, 0.0,
-              Ss.data() + static_cast<size_t>(start) * mmax, mmax);
+        gram(total, nnew, V.data(), HV.data() + static_cast<size_t>(start) * npw, Hs.data() + static_cast<size_t>(start) * mmax,
+             mmax);
+        gram(total, nnew, V.data(), V.data() + static_cast<size_t>(start) * npw, Ss.data() + static_cast<size_t>(start) * mmax,
+             mmax);
     }
-    // serial.hpp:680-756 -> subspace/serial.hpp
+    // serial.hpp:680-756 -> subspace/serial.hpp (Gamma: the real symmetric pencil, real eigenvectors,
+    // subspace/serial.hpp:33-90)
     int diag(int n, double* w, cplx* X) {
-        return zhegv(n, Hs.data(), mmax, Ss.data(), mmax, w, X, n);
+        const int info = zhegv(n, Hs.data(), mmax, Ss.data(), mmax, w, X, n);
+        if (gamma)
+            for (size_t i = 0; i < static_cast<size_t>(n) * n; i++) X[i] = cplx(X[i].real(), 0.0);
+        return info;
     }
     // serial.hpp:825-945
     int collapse(int n, bool have_matrices) {
@@ -1254,17 +1417,36 @@ struct Dav {
         std::memcpy(V.data(), scr.data(), sizeof(cplx) * scr.size());
         zgemm('N', npw, nb, n, 1.0, HV.data(), npw, X.data(), n, 0.0, scr.data(), npw);
         std::memcpy(HV.data(), scr.data(), sizeof(cplx) * scr.size());
+        g0_real(V.data(), nb);  // serial.hpp:939-943
+        g0_real(HV.data(), nb);
         return nb;
     }
 };
 }  // namespace
 
-int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin, const double* Vr,
-                 int nproj, const double* beta, int natoms, const int* block_sizes,
-                 const double* dblocks, int nb, int nocc, double* psi_inout, int seed_from_psi,
-                 double* eig_out, double tol, int max_iter, int factor, int* report) {
+// Stage clock of the last orc_davidson call, in the order of DavidsonTimingAccumulator
+// (counters.hpp:36-44): seed, ritz_vectors, residuals, orthogonalize, hamiltonian_application,
+// subspace_update, collapse, diagonalize (milliseconds, steady clock).
+static double g_stage_ms[8];
+
+void orc_davidson_stage_ms(double* out) {
+    for (int i = 0; i < 8; i++) out[i] = g_stage_ms[i];
+}
+
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int orc_get_threads() { return omp_get_max_threads(); }
+
+// minusmap: NULL for the full sphere; the -G box offsets of a Gamma-trick basis otherwise.
+int orc_davidson2(const int* dims, int npw, const int* fftmap, const int* minusmap, const double* ekin,
+                  const double* Vr, int nproj, const double* beta, int natoms, const int* block_sizes,
+                  const double* dblocks, int nb, int nocc, double* psi_inout, int seed_from_psi,
+                  double* eig_out, double tol, int max_iter, int factor, int* report) {
     // Build the Hamiltonian once (shared with orc_hamiltonian_apply's setup).
     Ham h;
+    h.minusmap = minusmap;
     for (int i = 0; i < 3; i++) h.dims[i] = dims[i];
     h.N = dims[0] * dims[1] * dims[2];
     h.npw = npw;
@@ -1296,6 +1478,7 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
 
     Dav d;
     d.h = &h;
+    d.gamma = minusmap != nullptr;
     d.npw = npw;
     d.nb = nb;
     d.nocc = nocc;
@@ -1306,6 +1489,13 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
     d.Hs.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
     d.Ss.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
     cplx* psi = reinterpret_cast<cplx*>(psi_inout);
+    for (double& t : g_stage_ms) t = 0.0;
+    auto clk = std::chrono::steady_clock::now();
+    auto lap = [&clk](int stage) {  // charges the time since the last lap to a stage
+        const auto now = std::chrono::steady_clock::now();
+        g_stage_ms[stage] += std::chrono::duration<double, std::milli>(now - clk).count();
+        clk = now;
+    };
 
     int basis = 0;
     if (seed_from_psi) {  // davidson.hpp:1000-1045
@@ -1331,6 +1521,7 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
         if (nnew > 0) d.apply(start, nnew);
     }
     d.update(0, basis, basis);
+    lap(0);
     std::vector<double> prev(nb, 0.0);
     for (int b = 0; b < nb; b++) prev[b] = d.Hs[static_cast<size_t>(b) * mmax + b].real();  // davidson.hpp:488-493
     std::vector<double> w(mmax, 0.0);
@@ -1341,12 +1532,15 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
     for (it = 0; it < max_iter; it++) {
         X.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
         int info = d.diag(basis, w.data(), X.data());
+        lap(7);
         if (info != 0 && basis > nb) {  // davidson.hpp:520-590
             basis = d.collapse(basis, false);
             std::fill(d.Hs.begin(), d.Hs.end(), cplx(0, 0));
             std::fill(d.Ss.begin(), d.Ss.end(), cplx(0, 0));
             d.update(0, basis, basis);
+            lap(6);
             info = d.diag(basis, w.data(), X.data());
+            lap(7);
             if (info != 0)
                 for (int b = 0; b < nb; b++) w[b] = prev[b];
         }
@@ -1383,6 +1577,7 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
                             sizeof(cplx) * basis);
             }
             zgemm('N', npw, nc, basis, 1.0, d.V.data(), npw, pc.data(), basis, 0.0, corr, npw);
+            lap(1);
             for (int c = 0; c < nc; c++) {
                 double ne = -ce[c];
                 cplx* col = corr + static_cast<size_t>(c) * npw;
@@ -1398,31 +1593,48 @@ int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin
                     col[g] /= den;
                 }
             }
+            d.g0_real(corr, nc);  // serial.hpp:1129-1130
         }
+        lap(2);
         int accepted = d.orthogonalize(basis, nc);
+        lap(3);
         if (accepted > 0) d.apply(basis, accepted);
+        lap(4);
         int old = basis;
         basis += accepted;
         if (accepted > 0) d.update(old, accepted, basis);
+        lap(5);
         if (basis >= mmax) {
             basis = d.collapse(basis, true);
             std::fill(d.Hs.begin(), d.Hs.end(), cplx(0, 0));
             std::fill(d.Ss.begin(), d.Ss.end(), cplx(0, 0));
             d.update(0, basis, basis);
+            lap(6);
         }
         for (int b = 0; b < nb; b++) prev[b] = w[b];
     }
     if (!done) {  // davidson.hpp:812-841
         X.assign(static_cast<size_t>(mmax) * mmax, cplx(0, 0));
         d.diag(basis, w.data(), X.data());
+        lap(7);
     }
     // extract_eigenvectors (serial.hpp:947-993)
     zgemm('N', npw, nb, basis, 1.0, d.V.data(), npw, X.data(), basis, 0.0, psi, npw);
+    d.g0_real(psi, nb);  // serial.hpp:982-983
+    lap(1);
     for (int b = 0; b < nb; b++) eig_out[b] = w[b];
     report[0] = done ? it + 1 : it;
     report[1] = done ? 0 : unconv;
     report[2] = d.happly;
     return 0;
+}
+
+int orc_davidson(const int* dims, int npw, const int* fftmap, const double* ekin, const double* Vr,
+                 int nproj, const double* beta, int natoms, const int* block_sizes,
+                 const double* dblocks, int nb, int nocc, double* psi_inout, int seed_from_psi,
+                 double* eig_out, double tol, int max_iter, int factor, int* report) {
+    return orc_davidson2(dims, npw, fftmap, nullptr, ekin, Vr, nproj, beta, natoms, block_sizes, dblocks, nb, nocc,
+                         psi_inout, seed_from_psi, eig_out, tol, max_iter, factor, report);
 }
 
 }  // extern "C"

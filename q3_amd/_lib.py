@@ -26,7 +26,8 @@ class DavidsonOptions(ctypes.Structure):
 
 class DavidsonReport(ctypes.Structure):
     _fields_ = [("iterations", _i), ("unconverged", _i), ("h_applications", _i), ("fp64_refinements", _i),
-                ("max_residual_norm", _d), ("stage_ms", _d * 8)]
+                ("max_residual_norm", _d), ("stage_ms", _d * 8), ("eig_barrier_redos", _i), ("mgs_fallbacks", _i),
+                ("recovery_collapses", _i), ("previous_eigenvalue_keeps", _i)]
 
 
 class BandRequest(ctypes.Structure):
@@ -39,7 +40,7 @@ class BandRequest(ctypes.Structure):
 
 class RequestStats(ctypes.Structure):
     _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i),
-                ("setup_s", _d), ("solve_s", _d)]
+                ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/q3x.h
@@ -68,6 +69,12 @@ SIGNATURES = {
     "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_noalloc": (_i, [_p, _p, _p, _i, _i]),
     "q3x_noalloc_scratch_bytes": (ctypes.c_ulong, []),
+    "q3x_basis_create_gamma": (_i, [_p, _d, _d, _d, _i, _p, _i, _pp]),
+    "q3x_basis_is_gamma_only": (_i, [_p, _p]),
+    "q3x_kpoint_minus_map": (_i, [_p, _p]),
+    "q3x_debug_force_barrier_giveups": (None, [_i]),
+    "q3x_debug_fail_eigensolves": (None, [ctypes.c_ulonglong]),
+    "q3x_eig_barrier_redos_total": (_l, []),
     "q3x_zgemm": (_i, [_p, ctypes.c_char, _i, _i, _i, _d, _p, _l, _p, _l, _d, _p, _l, _i]),
     "q3x_device_alloc": (_i, [_p, _ul, _pp]),
     "q3x_device_free": (_i, [_p, _p]),

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "aux_kernels.h"
+#include "gamma_kernels.h"
 #include "potential.h"
 #include "runtime.h"
 #include "subspace_eig.h"
@@ -259,15 +260,21 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
     hipStream_t st = h->ctx->stream;
     const HLocalArgs args = make_args(h, fp32);
     const PlaneTlConfig& tl = fp32 ? h->tl32 : h->tl64;
-    const int chunk = band_chunk(h, tl, nvec);
+    // Gamma trick: the transforms run on packed pairs of bands (nvec_t columns of the extended list)
+    const int nvec_t = h->gamma ? (nvec + 1) / 2 : nvec;
+    const int chunk = band_chunk(h, tl, nvec_t);
     const size_t cs = fp32 ? 8 : 16;
     if (!h->zbuf.ensure(static_cast<size_t>(chunk) * h->dims[2] * h->nst * cs)) return Q3X_ERR_DEVICE;
     if (tl.ok && tl.plane_global && !h->plane_slots.ensure(static_cast<size_t>(chunk) * h->dims[2] * tl.plane_bytes))
         return Q3X_ERR_DEVICE;
     const void* vplane = fp32 ? h->v32.get() : h->v64.get();
     Profiler& pr = h->ctx->prof;
-    for (int b0 = 0; b0 < nvec; b0 += chunk) {
-        const int nb = std::min(chunk, nvec - b0);
+    if (h->gamma) {
+        const size_t gb = static_cast<size_t>(h->npw_ext) * chunk * 16;
+        if (!h->gpack.ensure(gb) || !h->gout.ensure(gb)) return Q3X_ERR_DEVICE;
+    }
+    for (int b0 = 0; b0 < nvec_t; b0 += chunk) {
+        const int nb = std::min(chunk, nvec_t - b0);
         hipEvent_t marks[4];
         int mi[4] = {-1, -1, -1, -1};
         if (pr.on) {
@@ -281,10 +288,28 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
                 marks[i] = pr.pool[mi[i]];
             }
         }
-        if (!hip_ok(launch_hlocal(args, fp32, vplane, h->zbuf.get(), d_psi + 2L * b0 * ldpsi, ldpsi,
-                                  d_hpsi + 2L * b0 * ldh, ldh, nb, st, pr.on ? marks : nullptr,
-                                  &tl, h->plane_slots.get(), fp32 ? &h->tz32 : &h->tz64),
+        const double* in = d_psi + 2L * b0 * ldpsi;
+        double* out = d_hpsi + 2L * b0 * ldh;
+        long ldin = ldpsi, ldout = ldh;
+        int nbands = 0;
+        if (h->gamma) {  // bands 2 b0 .. of this chunk packed in pairs (serial.hpp:283-418)
+            nbands = std::min(2 * nb, nvec - 2 * b0);
+            if (!hip_ok(launch_gamma_pack(h->npw, nbands, d_psi + 4L * b0 * ldpsi, ldpsi, h->gpack.as<double>(),
+                                          h->npw_ext, st),
+                        "Gamma pack"))
+                return Q3X_ERR_DEVICE;
+            in = h->gpack.as<double>();
+            out = h->gout.as<double>();
+            ldin = ldout = h->npw_ext;
+        }
+        if (!hip_ok(launch_hlocal(args, fp32, vplane, h->zbuf.get(), in, ldin, out, ldout, nb, st,
+                                  pr.on ? marks : nullptr, &tl, h->plane_slots.get(), fp32 ? &h->tz32 : &h->tz64),
                     "H_local kernels"))
+            return Q3X_ERR_DEVICE;
+        if (h->gamma &&
+            !hip_ok(launch_gamma_unpack(h->npw, nbands, h->gout.as<double>(), h->npw_ext, d_hpsi + 4L * b0 * ldh, ldh,
+                                        st),
+                    "Gamma unpack"))
             return Q3X_ERR_DEVICE;
         pr.span(K_ZINV, mi[0], mi[1]);
         pr.span(K_PLANE, mi[1], mi[2]);
@@ -303,6 +328,11 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
                             : zgemm(GemmTypes::D_D_D_D, 1, np, nvec, npw, 1.0, h->beta64.get(), npw, d_psi, ldpsi,
                                     0.0, h->P.get(), np, h->gemm_work.get(), ws, st);
         if (!hip_ok(e, "projector overlap GEMM")) return Q3X_ERR_DEVICE;
+        // Gamma: 2 Re(beta^H psi) less the G = 0 term (serial.hpp:519-556)
+        if (h->gamma && !hip_ok(launch_gamma_overlap_fix(fp32, np, nvec, fp32 ? h->beta32.get() : h->beta64.get(), npw,
+                                                         d_psi, ldpsi, h->P.get(), np, st),
+                                "Gamma projector overlaps"))
+            return Q3X_ERR_DEVICE;
         const int m1 = pr.mark(st);
         pr.span(K_KB_OVERLAP, m0, m1);
         // W = D P per atom block (serial.hpp:587-679)
@@ -358,6 +388,10 @@ q3x_status hamiltonian_apply_fixed(q3x_hamiltonian h, const double* d_psi, doubl
     }
     if (!scratch || !d_psi || !d_hpsi) {
         set_error("apply: null device pointer");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (h->gamma) {
+        set_error("apply (no allocation): not available for a Gamma-trick basis");
         return Q3X_ERR_ARGUMENT;
     }
     const bool fp32 = precision == Q3X_MIXED;
@@ -434,7 +468,7 @@ double device_bytes_held(const q3x_context_s* c, const q3x_hamiltonian_s* h) {
         for (const DevBuf* d : {&h->v64, &h->v32, &h->stick_off, &h->stick_i2, &h->stick_col, &h->col_i1, &h->ent_pw,
                                 &h->ent_pos, &h->ekin, &h->outside, &h->beta64, &h->beta32, &h->dblocks, &h->pstart,
                                 &h->psize, &h->poff, &h->zbuf, &h->P, &h->W, &h->gemm_work, &h->io_psi, &h->io_hpsi,
-                                &h->plane_slots})
+                                &h->plane_slots, &h->gpack, &h->gout})
             b += static_cast<double>(d->size());
         for (const FftTables& t : h->fft)
             b += static_cast<double>(t.tw64.size() + t.tw32.size() + t.plain64.size() + t.plain32.size());
@@ -457,6 +491,8 @@ q3x_status hamiltonian_rebind(q3x_hamiltonian h, q3x_basis b) {
     h->have_potential = false;
     h->kp = nullptr;
     h->npw = h->nst = h->ncol = h->n_outside = 0;
+    h->gamma = false;
+    h->npw_ext = 0;
     h->nproj = 0;
     h->have_projectors = false;
     if (!same_grid)
@@ -633,6 +669,12 @@ q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, cons
     }
 }
 
+void q3x_debug_force_barrier_giveups(int count) { force_barrier_giveups(count); }
+
+void q3x_debug_fail_eigensolves(unsigned long long mask) { fail_eigensolves(mask); }
+
+long q3x_eig_barrier_redos_total(void) { return barrier_redos_total(); }
+
 q3x_status q3x_device_alloc(q3x_context ctx, unsigned long bytes, void** out) {
     try {
         if (!ctx || !out) {
@@ -806,11 +848,16 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
         const StickLayout& S = K.sticks;
         hipStream_t st = h->ctx->stream;
         h->npw = static_cast<int>(K.gidx.size());
+        h->gamma = K.gamma;
+        // the list the transforms and the kinetic term run over (Gamma: half sphere + mirrors)
+        const std::vector<int>& tmap = K.gamma ? K.ext_fftmap : K.fftmap;
+        const std::vector<double>& tkin = K.gamma ? K.ext_ekin : K.ekin;
+        h->npw_ext = static_cast<int>(tmap.size());
         h->nst = S.nst;
         h->ncol = S.ncol;
         std::vector<int> out;
-        for (int p = 0; p < h->npw; ++p)
-            if (K.fftmap[p] < 0) out.push_back(p);
+        for (int p = 0; p < h->npw_ext; ++p)
+            if (tmap[p] < 0) out.push_back(p);
         h->n_outside = static_cast<int>(out.size());
         const int n1 = h->dims[0], n2 = h->dims[1], n3 = h->dims[2];
         h->sticks_per_block = std::max(4, std::min(64, 3072 / n3));
@@ -846,7 +893,7 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
                   h->col_i1.upload(S.col_i1.data(), S.col_i1.size() * sizeof(int), st, &h->ctx->pin) &&
                   h->ent_pw.upload(S.ent_pw.data(), S.ent_pw.size() * sizeof(int), st, &h->ctx->pin) &&
                   h->ent_pos.upload(S.ent_pos.data(), S.ent_pos.size() * sizeof(int), st, &h->ctx->pin) &&
-                  h->ekin.upload(K.ekin.data(), K.ekin.size() * sizeof(double), st, &h->ctx->pin) &&
+                  h->ekin.upload(tkin.data(), tkin.size() * sizeof(double), st, &h->ctx->pin) &&
                   h->outside.upload(out.data(), out.size() * sizeof(int), st, &h->ctx->pin);
         if (!ok) return Q3X_ERR_DEVICE;
         h->kp = k;

@@ -29,6 +29,7 @@
 
 #include "aux_kernels.h"
 #include "dense.h"
+#include "gamma_kernels.h"
 #include "runtime.h"
 #include "subspace_eig.h"
 #include "zgemm.h"
@@ -51,6 +52,9 @@ struct Solver {
     hipStream_t st;
     int npw = 0, nb = 0, nocc = 0, mmax = 0;
     int precision = Q3X_FP64;
+    // Gamma trick: half-sphere columns of real functions; every product over plane waves is the
+    // full-sphere real one and G = 0 stays real (the gamma paths of subspace_manager/serial.hpp)
+    bool gamma = false;
     // buffers of the context's workspace.  X_d: subspace eigenvectors (n x nvec, ld n) of the
     // last diag(); Xc_d: their columns for the corrections; idx_d / eps_d / negeps_d: band
     // indices and energies of the corrections
@@ -58,6 +62,8 @@ struct Solver {
     DevBuf &V, &HV, &R, &C, &G, &Hs_d, &Ss_d, &X_d, &Xc_d, &eps_d, &negeps_d, &work, &norms_d, &idx_d, &eigw;
     size_t work_bytes = size_t(256) << 20;
     int h_applications = 0;
+    SubspaceEigStats eig_stats;  // barrier_redos accumulates over the solve's eigensolves
+    int mgs_fallbacks = 0, recovery_collapses = 0, previous_eigenvalue_keeps = 0;
     double stage[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     // With profiling on, every stage ends with a stream synchronisation so that stage_ms are
     // device times; otherwise the solve synchronises only where the host needs device data
@@ -89,6 +95,16 @@ struct Solver {
                   "Davidson GEMM");
     }
 
+    // C (m x n) = A^H B over the plane waves; at Gamma 2 Re(A^H B) less the G = 0 term, real
+    // (subspace_manager/serial.hpp:73-110, 228-262, 575-640)
+    bool gram(int m, int n, const double* A, const double* B, void* Cm, long ldc) {
+        if (!gemm(1, m, n, npw, 1.0, A, npw, B, npw, 0.0, Cm, ldc)) return false;
+        return !gamma || ok(launch_gamma_gram_fix(m, n, A, npw, B, npw, static_cast<double*>(Cm), ldc, st),
+                            "Gamma Gram");
+    }
+    // enforce_gamma_g0_real (subspace_manager/serial.hpp:1189-1200) on ncol columns
+    bool g0_real(double* X, int ncol) { return !gamma || ok(launch_g0_real(ncol, X, npw, st), "Gamma G = 0"); }
+
     bool apply(int first, int n, int prec) {
         if (n <= 0) return true;
         const double t0 = now_ms();
@@ -110,8 +126,8 @@ struct Solver {
         const double t0 = now_ms();
         double* hs = Hs_d.as<double>() + 2L * start * mmax;
         double* ss = Ss_d.as<double>() + 2L * start * mmax;
-        if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, HVc(start), npw, 0.0, hs, mmax)) return false;
-        if (!gemm(1, total, nnew, npw, 1.0, Vc(0), npw, Vc(start), npw, 0.0, ss, mmax)) return false;
+        if (!gram(total, nnew, Vc(0), HVc(start), hs, mmax)) return false;
+        if (!gram(total, nnew, Vc(0), Vc(start), ss, mmax)) return false;
         if (!stage_sync()) return false;
         stage[3] += now_ms() - t0;
         return true;
@@ -126,11 +142,14 @@ struct Solver {
         hipError_t e = hipSuccess;
         const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
                                       X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
-                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, nullptr, &h->ctx->prof, &pin());
+                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, &eig_stats, &h->ctx->prof, &pin());
         if (info < 0) {
             ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
             return -1;
         }
+        // Gamma: the pencil is real symmetric and its eigenvectors real (dsygv, subspace/serial.hpp:33-90)
+        if (gamma && info == 0 && !ok(launch_zero_imag(n, nvec, X_d.as<double>(), n, st), "Gamma real eigenvectors"))
+            return -1;
         stage[4] += now_ms() - t0;
         return info;
     }
@@ -142,10 +161,10 @@ struct Solver {
         const double t0 = now_ms();
         double* cand = Vc(start);
         if (start > 0) {
-            if (!gemm(1, start, nc, npw, 1.0, Vc(0), npw, cand, npw, 0.0, C.get(), start)) return -1;
+            if (!gram(start, nc, Vc(0), cand, C.get(), start)) return -1;
             if (!gemm(0, npw, nc, start, -1.0, Vc(0), npw, C.get(), start, 1.0, cand, npw)) return -1;
         }
-        if (!gemm(1, nc, nc, npw, 1.0, cand, npw, cand, npw, 0.0, G.get(), nc)) return -1;
+        if (!gram(nc, nc, cand, cand, G.get(), nc)) return -1;
         std::vector<zc> Gh(static_cast<size_t>(nc) * nc);
         if (!ok(pin().d2h(Gh.data(), G.get(), Gh.size() * 16, st), "D2H Gram")) return -1;
         int accepted = nc;
@@ -158,6 +177,7 @@ struct Solver {
                 return -1;
         } else {
             // modified Gram-Schmidt fallback (serial.hpp:370-552)
+            ++mgs_fallbacks;
             accepted = 0;
             std::vector<double> n2(1);
             for (int c = 0; c < nc; ++c) {
@@ -168,10 +188,11 @@ struct Solver {
                     return -1;
                 const int nex = dst - start;
                 if (nex > 0) {
-                    if (!gemm(1, nex, 1, npw, 1.0, cand, npw, Vc(dst), npw, 0.0, C.get(), nex)) return -1;
+                    if (!gram(nex, 1, cand, Vc(dst), C.get(), nex)) return -1;
                     if (!gemm(0, npw, 1, nex, -1.0, cand, npw, C.get(), nex, 1.0, Vc(dst), npw)) return -1;
                 }
                 if (!ok(launch_column_norm2(npw, 1, Vc(dst), npw, norms_d.as<double>(), st), "norm") ||
+                    (gamma && !ok(launch_gamma_norm_fix(1, Vc(dst), npw, norms_d.as<double>(), st), "Gamma norm")) ||
                     !ok(pin().d2h(n2.data(), norms_d.get(), 8, st), "D2H norm"))
                     return -1;
                 if (n2[0] > kNormThreshold) {
@@ -197,6 +218,7 @@ struct Solver {
             return false;
         std::swap(vbase, valt);
         std::swap(hvbase, hvalt);
+        if (!g0_real(vbase, ncols) || !g0_real(hvbase, ncols)) return false;  // serial.hpp:939-943, 982-983
         return stage_sync();
     }
 
@@ -275,6 +297,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     s.nocc = std::max(0, std::min(opt->n_occupied, nb));
     s.mmax = std::max(std::min(opt->basis_factor * nb, s.npw), nb);
     s.precision = opt->precision;
+    s.gamma = h->gamma;
     const int npw = s.npw, mmax = s.mmax;
     const double t_start = s.now_ms();
     const size_t col = static_cast<size_t>(npw) * 16;
@@ -363,12 +386,19 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
             collapsed = false;
             if (info < 0) return s.status;
             if (info != 0 && basis > nb) {  // davidson.hpp:520-590
+                ++s.recovery_collapses;
                 basis = s.collapse(basis, false);
                 if (basis < 0) return s.status;
                 info = s.diag(basis, w, nb);
                 if (info < 0) return s.status;
-                if (info != 0)
+                if (info != 0) {
+                    ++s.previous_eigenvalue_keeps;
                     for (int b = 0; b < nb; ++b) w[b] = prev[b];
+                    // the collapsed basis is made of Ritz vectors: their coefficients are the
+                    // identity (the reference goes on with whatever the failed solve left)
+                    if (!s.ok(launch_identity(nb, s.X_d.as<double>(), nb, s.st), "identity eigenvectors"))
+                        return s.status;
+                }
             }
             t0 = s.now_ms();
             std::vector<int> src;
@@ -424,6 +454,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                            !s.ok(launch_teter(npw, nc, h->ekin.as<double>(), eps_c, corr, npw, s.st), "Teter")) {
                     return s.status;
                 }
+                if (!s.g0_real(corr, nc)) return s.status;  // serial.hpp:1129-1130
             }
             if (!s.stage_sync()) return s.status;
             s.stage[6] += s.now_ms() - t0;
@@ -475,6 +506,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
             !s.ok(launch_axpby(npw, nb, s.eps_d.as<double>(), s.Vc(0), npw, 0.0, res, npw, s.st), "r = -e psi") ||
             !s.ok(launch_axpby(npw, nb, nullptr, s.HVc(0), npw, 1.0, res, npw, s.st), "r += H psi") ||
             !s.ok(launch_column_norm2(npw, nb, res, npw, s.norms_d.as<double>(), s.st), "norms") ||
+            (s.gamma && !s.ok(launch_gamma_norm_fix(nb, res, npw, s.norms_d.as<double>(), s.st), "Gamma norms")) ||
             !s.ok(s.pin().d2h(n2.data(), s.norms_d.get(), nb * 8, s.st), "D2H norms"))
             return s.status;
         for (int b = 0; b < nb; ++b) maxres = std::max(maxres, std::sqrt(n2[b]));
@@ -492,6 +524,10 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
         report->h_applications = s.h_applications;
         report->fp64_refinements = refinements;
         report->max_residual_norm = maxres;
+        report->eig_barrier_redos = s.eig_stats.barrier_redos;
+        report->mgs_fallbacks = s.mgs_fallbacks;
+        report->recovery_collapses = s.recovery_collapses;
+        report->previous_eigenvalue_keeps = s.previous_eigenvalue_keeps;
         for (int i = 0; i < 8; ++i) report->stage_ms[i] = s.stage[i];
     }
     return Q3X_OK;

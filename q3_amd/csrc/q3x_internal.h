@@ -38,6 +38,7 @@ struct GEntry {
 
 struct BasisData {
     Lattice lat;
+    bool gamma = false;  // Gamma trick: half sphere (g_vectors_grid.hpp:56-76)
     double ecutwfc, ecutrho;
     int rho[3], wave[3];
     std::vector<GEntry> g;
@@ -62,6 +63,14 @@ struct KPointData {
     std::vector<int> gidx;
     std::vector<double> ekin;
     std::vector<int> fftmap;
+    // Gamma trick (k = 0 on a half-sphere basis): box offset of -G per plane wave
+    // (serial.hpp:889-904), and the extended list the transforms run over: the npw half-sphere
+    // entries, then the mirrors -G_j of j = 1..npw-1 (mirror j at npw + j - 1), with their box
+    // offsets and kinetic energies.  sticks index the extended list.
+    bool gamma = false;
+    std::vector<int> minusmap;
+    std::vector<int> ext_fftmap;
+    std::vector<double> ext_ekin;
     StickLayout sticks;
 };
 

@@ -132,6 +132,11 @@ struct q3x_hamiltonian_s {
     int nproj = 0;
     bool have_projectors = false;
     q3x::DevBuf beta64, beta32, dblocks, pstart, psize, poff;
+    // Gamma trick: transforms run over the extended list (half sphere, then mirrors) of npw_ext
+    // entries, two bands per column: gpack holds the packed pairs, gout their H images
+    bool gamma = false;
+    int npw_ext = 0;
+    q3x::DevBuf gpack, gout;
     // scratch
     q3x::DevBuf zbuf, P, W, gemm_work, io_psi, io_hpsi;
     q3x::DevBuf plane_slots;  // per-workgroup planes of the two-level kernel when a plane exceeds LDS

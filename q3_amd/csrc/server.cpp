@@ -58,6 +58,7 @@ struct Request {
     std::vector<std::pair<q3x_context, void*>> psi;  // device wavefunctions, one per k-point
     std::vector<size_t> psi_bytes;
     int iterations = 0;
+    int eig_barrier_redos = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     double setup_s = 0, solve_s = 0;
 };
@@ -305,6 +306,7 @@ struct q3x_server_s {
         {
             std::lock_guard<std::mutex> g(mu);
             r.iterations += rep.iterations;
+            r.eig_barrier_redos += rep.eig_barrier_redos;
             r.setup_s += p.host_s + std::chrono::duration<double>(tsolve - ts).count();
             r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
         }
@@ -533,6 +535,7 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
             stats->iterations = r->iterations;
             stats->setup_s = r->setup_s;
             stats->solve_s = r->solve_s;
+            stats->eig_barrier_redos = r->eig_barrier_redos;
         }
         {
             // the request's wavefunction blocks go back to the pool, and its share of the

@@ -99,6 +99,11 @@ static void build_sphere(BasisData& B) {
     for (int i1 = -h1; i1 <= h1; ++i1)
         for (int i2 = -h2; i2 <= h2; ++i2)
             for (int i3 = -h3; i3 <= h3; ++i3) {
+                if (B.gamma) {  // g_vectors_grid.hpp:69-76: keep i3 > 0, (0, i2 > 0), (0, 0, i1 >= 0)
+                    if (i3 < 0) continue;
+                    if (i3 == 0 && i2 < 0) continue;
+                    if (i3 == 0 && i2 == 0 && i1 < 0) continue;
+                }
                 GEntry e;
                 g_of_miller(L, i1, i2, i3, e.g);
                 e.g2 = norm2(e.g);
@@ -116,15 +121,17 @@ static int wrap(int m, int n) { return m >= 0 ? m : m + n; }
 
 // Stick layout for the pencil FFT.  Permutation-only bookkeeping: values are never reordered
 // in HBM, the kernels gather/scatter through ent_pw.
-static void build_sticks(const BasisData& B, KPointData& kp) {
-    const int n1 = B.wave[0], n2 = B.wave[1], n3 = B.wave[2];
-    StickLayout& S = kp.sticks;
-    const int npw = static_cast<int>(kp.gidx.size());
+// millers: 3 ints per plane wave of the list the transforms run over; fftmap: its box offsets
+// (-1 outside the box).
+static void build_sticks(const int wave[3], const std::vector<int>& millers, const std::vector<int>& fftmap,
+                         StickLayout& S) {
+    const int n1 = wave[0], n2 = wave[1], n3 = wave[2];
+    const int npw = static_cast<int>(fftmap.size());
     // distinct columns (wrapped i1) present
     std::vector<char> has_col(n1, 0);
     for (int p = 0; p < npw; ++p) {
-        if (kp.fftmap[p] < 0) continue;
-        has_col[wrap(B.g[kp.gidx[p]].m[0], n1)] = 1;
+        if (fftmap[p] < 0) continue;
+        has_col[wrap(millers[3 * p], n1)] = 1;
     }
     std::vector<int> col_of_i1(n1, -1);
     S.col_i1.clear();
@@ -138,13 +145,13 @@ static void build_sticks(const BasisData& B, KPointData& kp) {
     std::vector<int> stick_id(static_cast<size_t>(S.ncol) * n2, -1);
     std::vector<int> pw_stick(npw, -1), pw_z(npw, -1);
     for (int p = 0; p < npw; ++p) {
-        if (kp.fftmap[p] < 0) continue;
-        const GEntry& e = B.g[kp.gidx[p]];
-        int c = col_of_i1[wrap(e.m[0], n1)];
-        int key = c * n2 + wrap(e.m[1], n2);
+        if (fftmap[p] < 0) continue;
+        const int* m = &millers[3 * p];
+        int c = col_of_i1[wrap(m[0], n1)];
+        int key = c * n2 + wrap(m[1], n2);
         stick_id[key] = 0;
         pw_stick[p] = key;
-        pw_z[p] = wrap(e.m[2], n3);
+        pw_z[p] = wrap(m[2], n3);
     }
     S.stick_col.clear();
     S.stick_i2.clear();
@@ -204,16 +211,45 @@ void build_kpoint(const BasisData& B, const double k[3], KPointData& kp) {
     kp.ekin.resize(npw);
     kp.fftmap.resize(npw);
     const int n1 = B.wave[0], n2 = B.wave[1], n3 = B.wave[2];
+    auto box = [n1, n2, n3](int m1, int m2, int m3) {
+        if (std::abs(m1) <= n1 / 2 && std::abs(m2) <= n2 / 2 && std::abs(m3) <= n3 / 2)
+            return (wrap(m1, n1) * n2 + wrap(m2, n2)) * n3 + wrap(m3, n3);
+        return -1;
+    };
+    std::vector<int> millers(3 * static_cast<size_t>(npw));
     for (int p = 0; p < npw; ++p) {
         kp.gidx[p] = sel[order[p]];
         kp.ekin[p] = kin[order[p]];
         const int* m = B.g[kp.gidx[p]].m;
-        if (std::abs(m[0]) <= n1 / 2 && std::abs(m[1]) <= n2 / 2 && std::abs(m[2]) <= n3 / 2)
-            kp.fftmap[p] = (wrap(m[0], n1) * n2 + wrap(m[1], n2)) * n3 + wrap(m[2], n3);
-        else
-            kp.fftmap[p] = -1;
+        kp.fftmap[p] = box(m[0], m[1], m[2]);
+        for (int d = 0; d < 3; ++d) millers[3 * p + d] = m[d];
     }
-    build_sticks(B, kp);
+    kp.gamma = B.gamma;
+    kp.minusmap.clear();
+    kp.ext_fftmap.clear();
+    kp.ext_ekin.clear();
+    if (!B.gamma) {
+        build_sticks(B.wave, millers, kp.fftmap, kp.sticks);
+        return;
+    }
+    // Gamma trick: -G map (serial.hpp:889-904) and the extended list [half sphere | mirrors of 1..]
+    kp.minusmap.resize(npw);
+    for (int p = 0; p < npw; ++p) kp.minusmap[p] = box(-millers[3 * p], -millers[3 * p + 1], -millers[3 * p + 2]);
+    const int next = npw > 0 ? 2 * npw - 1 : 0;
+    std::vector<int> ext_m(3 * static_cast<size_t>(next));
+    kp.ext_fftmap.resize(next);
+    kp.ext_ekin.resize(next);
+    for (int p = 0; p < npw; ++p) {
+        for (int d = 0; d < 3; ++d) ext_m[3 * p + d] = millers[3 * p + d];
+        kp.ext_fftmap[p] = kp.fftmap[p];
+        kp.ext_ekin[p] = kp.ekin[p];
+        if (p == 0) continue;
+        const int q = npw + p - 1;
+        for (int d = 0; d < 3; ++d) ext_m[3 * q + d] = -millers[3 * p + d];
+        kp.ext_fftmap[q] = kp.minusmap[p];
+        kp.ext_ekin[q] = kp.ekin[p];
+    }
+    build_sticks(B.wave, ext_m, kp.ext_fftmap, kp.sticks);
 }
 
 }  // namespace q3x
@@ -226,6 +262,13 @@ extern "C" {
 q3x_status q3x_basis_create(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
                             double ecutrho_ry, int use_exact_wavefunction_grid,
                             const int fft_factors[3], q3x_basis* out) {
+    return q3x_basis_create_gamma(lattice_rows, alat_bohr, ecutwfc_ry, ecutrho_ry, use_exact_wavefunction_grid,
+                                  fft_factors, 0, out);
+}
+
+q3x_status q3x_basis_create_gamma(const double lattice_rows[9], double alat_bohr, double ecutwfc_ry,
+                                  double ecutrho_ry, int use_exact_wavefunction_grid, const int fft_factors[3],
+                                  int gamma_only, q3x_basis* out) {
     if (!lattice_rows || !out || !(alat_bohr > 0) || !(ecutwfc_ry > 0)) {
         set_error("q3x_basis_create: invalid argument");
         return Q3X_ERR_ARGUMENT;
@@ -233,6 +276,7 @@ q3x_status q3x_basis_create(const double lattice_rows[9], double alat_bohr, doub
     try {
         auto* b = new q3x_basis_s;
         BasisData& B = b->d;
+        B.gamma = gamma_only != 0;
         B.lat = make_lattice(lattice_rows, alat_bohr);
         if (!(B.lat.volume > 1e-10)) {
             delete b;
@@ -278,6 +322,15 @@ q3x_status q3x_basis_info(q3x_basis b, int rho_dims[3], int wave_dims[3], int* n
     return Q3X_OK;
 }
 
+q3x_status q3x_basis_is_gamma_only(q3x_basis b, int* gamma_only) {
+    if (!b || !gamma_only) {
+        set_error("q3x_basis_is_gamma_only: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    *gamma_only = b->d.gamma ? 1 : 0;
+    return Q3X_OK;
+}
+
 q3x_status q3x_basis_gvectors(q3x_basis b, int* miller, double* g_cart, double* g2) {
     if (!b) {
         set_error("q3x_basis_gvectors: null basis");
@@ -302,6 +355,10 @@ q3x_status q3x_basis_destroy(q3x_basis b) {
 q3x_status q3x_kpoint_create(q3x_basis b, const double k_crystal[3], q3x_kpoint* out, int* npw) {
     if (!b || !k_crystal || !out) {
         set_error("q3x_kpoint_create: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (b->d.gamma && (k_crystal[0] != 0.0 || k_crystal[1] != 0.0 || k_crystal[2] != 0.0)) {
+        set_error("q3x_kpoint_create: a Gamma-trick (half-sphere) basis has only the k-point (0, 0, 0)");
         return Q3X_ERR_ARGUMENT;
     }
     try {
@@ -331,6 +388,19 @@ q3x_status q3x_kpoint_maps(q3x_kpoint k, int* g_index, double* kinetic_ry, int* 
     return Q3X_OK;
 }
 
+q3x_status q3x_kpoint_minus_map(q3x_kpoint k, int* minus_fft_map) {
+    if (!k || !minus_fft_map) {
+        set_error("q3x_kpoint_minus_map: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    if (!k->d.gamma) {
+        set_error("q3x_kpoint_minus_map: the k-point is not on a Gamma-trick basis");
+        return Q3X_ERR_STATE;
+    }
+    std::copy(k->d.minusmap.begin(), k->d.minusmap.end(), minus_fft_map);
+    return Q3X_OK;
+}
+
 q3x_status q3x_kpoint_random_psi(q3x_kpoint k, int nb, unsigned long long seed, double* psi_out) {
     if (!k || nb < 0 || (!psi_out && nb > 0)) {
         set_error("q3x_kpoint_random_psi: invalid argument");
@@ -354,7 +424,8 @@ q3x_status q3x_kpoint_random_psi(q3x_kpoint k, int nb, unsigned long long seed, 
             const double re = static_cast<double>(mix(s0 + o) >> 11) * scale - 1.0;
             const double im = static_cast<double>(mix(s0 + o + 1) >> 11) * scale - 1.0;
             psi_out[o] = re * damp;
-            psi_out[o + 1] = im * damp;
+            // a Gamma-trick band is a real function: its G = 0 coefficient is real
+            psi_out[o + 1] = k->d.gamma && g == 0 ? 0.0 : im * damp;
         }
     return Q3X_OK;
 }

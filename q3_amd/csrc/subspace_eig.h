@@ -16,7 +16,18 @@ size_t subspace_eig_workspace(int n, int nvec);
 struct SubspaceEigStats {
     int newton_schulz_iterations = 0;  // 0: first-order Loewdin factor was accurate enough
     double s_deviation = 0.0;          // max |S - I| over the compacted matrix
+    // solves redone on one workgroup because a grid barrier of the multi-workgroup
+    // tridiagonalisation gave up (added to, never reset, by subspace_eig)
+    int barrier_redos = 0;
 };
+
+// Test hook: the next `count` multi-workgroup tridiagonalisations start with their give-up flag
+// set, so each is redone on one workgroup.
+void force_barrier_giveups(int count);
+// Test hook: bit i of mask makes the i-th subspace_eig call from now return n + 1 without solving.
+void fail_eigensolves(unsigned long long mask);
+// Redos on one workgroup since the process started.
+long barrier_redos_total();
 
 // Solves H x = w S x for the nvec lowest pairs.  H and S (device, complex128, column-major,
 // leading dimensions ldh / lds) are read from their upper triangles, as zhegv 'U' does.

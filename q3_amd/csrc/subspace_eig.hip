@@ -778,6 +778,13 @@ __global__ void zero_words(unsigned* a, int n) {
     if (i < n) a[i] = 0u;
 }
 
+// The give-up flag of a multi-workgroup launch set before it starts (test hook): every grid
+// barrier then returns at once, the kernel runs to its end on stale data and the host redoes
+// the solve on one workgroup.
+__global__ void set_word(int* a, int v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) a[0] = v;
+}
+
 // ---------------------------------------------------------------- 4. eigenvalues ----
 // Number of eigenvalues of the tridiagonal (d, e2 = e^2) below x (Sturm count of the LDL^T
 // pivots; pivots smaller than pivmin are replaced by -pivmin, as LAPACK's dstebz does).
@@ -1178,7 +1185,32 @@ Layout layout(int n, int nvec) {
 
 size_t tridiag_lds(int n, bool lm) { return (6 * static_cast<size_t>(n) + 17 + (lm ? static_cast<size_t>(n) * n : 0)) * 16; }
 
+// multi-workgroup launches still to be started with their give-up flag set (test hook), and
+// redos on one workgroup since the process started
+std::atomic<int> g_forced_giveups{0};
+std::atomic<long> g_barrier_redos{0};
+// eigensolves to fail on purpose (test hook): bit 0 is the next solve; shifted once per solve
+std::atomic<unsigned long long> g_fail_mask{0};
+
+bool take_forced_failure() {
+    unsigned long long m = g_fail_mask.load();
+    while (m != 0)
+        if (g_fail_mask.compare_exchange_weak(m, m >> 1)) return (m & 1ull) != 0;
+    return false;
+}
+
+bool take_forced_giveup() {
+    int v = g_forced_giveups.load();
+    while (v > 0)
+        if (g_forced_giveups.compare_exchange_weak(v, v - 1)) return true;
+    return false;
+}
+
 }  // namespace
+
+void force_barrier_giveups(int count) { g_forced_giveups.store(std::max(0, count)); }
+void fail_eigensolves(unsigned long long mask) { g_fail_mask.store(mask); }
+long barrier_redos_total() { return g_barrier_redos.load(); }
 
 size_t subspace_eig_workspace(int n, int nvec) { return layout(std::max(n, 1), std::max(nvec, 1)).total; }
 
@@ -1195,6 +1227,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     };
     *err = hipSuccess;
     if (n <= 0 || nvec <= 0) return 0;
+    if (take_forced_failure()) return n + 1;  // test hook: reported as a non-positive-definite S
     nvec = std::min(nvec, n);
     const Layout L = layout(n, nvec);
     if (work_bytes < L.total) {
@@ -1314,6 +1347,10 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
                     z2* xp = P(B_XP);
                     z2* xc = P(B_XC);
                     int* fail = reinterpret_cast<int*>(bar + 1);
+                    if (take_forced_giveup()) {
+                        hipLaunchKernelGGL(set_word, dim3(1), dim3(64), 0, st, fail, 1);
+                        if (!ok(hipGetLastError())) return -1;
+                    }
                     hipLaunchKernelGGL(tridiag_coop, dim3(W), dim3(kThreads), lds, st, Cc, n, R, d, beta, tau, ph, e,
                                        xp, xc, bar, fail);
                     if (!ok(hipGetLastError())) return -1;
@@ -1398,6 +1435,8 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     auto solve_once = [&](double* sdev_out) -> int {
         int rc = solve_from_zl(sdev_out);
         if (rc == -2) {
+            g_barrier_redos.fetch_add(1);
+            if (stats) ++stats->barrier_redos;
             multi_wg_ok = false;
             rc = solve_from_zl(sdev_out);
             multi_wg_ok = true;

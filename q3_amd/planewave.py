@@ -124,13 +124,17 @@ def hegv_device(context: Context, n: int, H: DeviceArray, ldh: int, S: DeviceArr
 
 class PlanewaveBasis:
     def __init__(self, crystal: Crystal, ecutwfc_ry: float, ecutrho_ry: float = 0.0,
-                 use_exact_wavefunction_grid: bool = True, fft_factors=(1, 1, 1)):
+                 use_exact_wavefunction_grid: bool = True, fft_factors=(1, 1, 1), gamma_only: bool = False):
+        """gamma_only: the Gamma trick (half G sphere, real bands at k = 0), as the reference uses for a
+        1x1x1 unshifted k grid (iterator/kohn_sham/planewave/init.hpp:585-610)."""
         self.crystal = crystal
+        self.gamma_only = bool(gamma_only)
         lat = np.ascontiguousarray(crystal.lattice, dtype=np.float64).ravel()
         fac = np.ascontiguousarray(fft_factors, dtype=np.int32)
         h = ctypes.c_void_p()
-        check(lib().q3x_basis_create(_ptr(lat), float(crystal.alat), float(ecutwfc_ry), float(ecutrho_ry),
-                                     int(use_exact_wavefunction_grid), _ptr(fac), ctypes.byref(h)))
+        check(lib().q3x_basis_create_gamma(_ptr(lat), float(crystal.alat), float(ecutwfc_ry), float(ecutrho_ry),
+                                           int(use_exact_wavefunction_grid), _ptr(fac), int(gamma_only),
+                                           ctypes.byref(h)))
         self.h = h
         rho = np.zeros(3, np.int32)
         wave = np.zeros(3, np.int32)
@@ -177,6 +181,10 @@ class KPoint:
         self.fft_map = np.zeros(self.npw, np.int32)
         check(lib().q3x_kpoint_maps(h, _ptr(self.g_vector_indices), _ptr(self.kinetic_energy_squared),
                                     _ptr(self.fft_map)))
+        self.minus_fft_map = None
+        if basis.gamma_only:
+            self.minus_fft_map = np.zeros(self.npw, np.int32)
+            check(lib().q3x_kpoint_minus_map(h, _ptr(self.minus_fft_map)))
 
     def random_psi(self, nbands: int, seed: int) -> np.ndarray:
         """Starting block (npw x nbands, Fortran order): entries uniform in the unit square damped
@@ -315,6 +323,8 @@ class PlanewaveDavidsonDiagonalizer:
                  "collapse", "residual", "total"]
         return {"iterations": r.iterations, "unconverged": r.unconverged, "h_applications": r.h_applications,
                 "fp64_refinements": r.fp64_refinements, "max_residual_norm": r.max_residual_norm,
+                "eig_barrier_redos": r.eig_barrier_redos, "mgs_fallbacks": r.mgs_fallbacks,
+                "recovery_collapses": r.recovery_collapses, "previous_eigenvalue_keeps": r.previous_eigenvalue_keeps,
                 "stage_ms": {n: r.stage_ms[i] for i, n in enumerate(names)}}
 
 
@@ -476,7 +486,7 @@ class BandStructureServer:
             del self._live[ticket]
         return eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
                      "footprint_bytes": st.footprint_bytes, "iterations": st.iterations,
-                     "setup_s": st.setup_s, "solve_s": st.solve_s}
+                     "setup_s": st.setup_s, "solve_s": st.solve_s, "eig_barrier_redos": st.eig_barrier_redos}
 
     def in_flight_bytes(self) -> float:
         return lib().q3x_server_in_flight_bytes(self.h)

@@ -94,6 +94,9 @@ def local_potential(crystal: Crystal, basis, species_list, grid: str = "wave") -
     Ry.  Built in G space over the density sphere and brought to real space with the
     FFTW_BACKWARD convention (numpy: N * ifftn)."""
     m, g, g2 = basis.g_vectors()
+    if getattr(basis, "gamma_only", False):  # half sphere: add the mirror of every G but G = 0
+        nz0 = np.any(m != 0, axis=1)
+        m, g, g2 = np.vstack([m, -m[nz0]]), np.vstack([g, -g[nz0]]), np.concatenate([g2, g2[nz0]])
     tpa = 2 * math.pi / crystal.alat
     gabs2 = g2 * tpa * tpa  # bohr^-2
     omega = basis.cell_volume

@@ -130,3 +130,62 @@ def test_one_band_and_oversized_requests(gpu_context):
     assert abs(e[0] - exact[0]) <= 1e-8 * HA
     with pytest.raises(Q3XError, match="nb <= npw"):
         d.solve(case.kp.npw + 1, 1)
+
+
+def test_dependent_starting_columns_take_the_mgs_path(gpu_context):
+    """A starting block with repeated columns makes the Gram matrix of the seed singular: the
+    Cholesky-QR fails, modified Gram-Schmidt drops the dependent columns (norm^2 <= 1e-14,
+    subspace_manager/serial.hpp:370-552), kinetic unit vectors fill the block
+    (davidson.hpp:1047-1101) and the solve still reaches the dense eigenvalues."""
+    case = kcase(W.cfg1(), 1)
+    exact = np.linalg.eigvalsh(_dense(case))
+    h, d = _solver(gpu_context, case)
+    d.set_tolerance(1e-10)
+    d.set_maximum_iterations(200)
+    psi0 = W.random_psi(case.kp.kinetic_energy_squared, 8, seed=12)
+    psi0[:, 1] = psi0[:, 0]
+    psi0[:, 5] = 2.0 * psi0[:, 4]
+    e, psi = d.solve(8, 8, psi0)
+    rep = d.report_dict()
+    assert rep["mgs_fallbacks"] >= 1 and rep["unconverged"] == 0
+    assert np.abs(e - exact[:8]).max() <= 1e-8 * HA
+    np.testing.assert_allclose(psi.conj().T @ psi, np.eye(8), atol=1e-10)
+
+
+def test_failed_subspace_eigensolves_are_recovered(gpu_context):
+    """The eigensolve of iteration 1 fails (the basis holds more than nb columns): the subspace is
+    collapsed and rebuilt from the vectors; the solve after the collapse fails too: the previous
+    eigenvalues are kept for that iteration (davidson.hpp:520-590).  The solve still converges to
+    the dense eigenvalues.  Failures are forced by the eigensolver's test hook (solves 1 and 3
+    from now: iteration 0, the failing iteration-1 solve, the collapse's rebuilt solve, then the
+    failing solve after it)."""
+    from q3_amd._lib import lib
+    case = kcase(W.cfg1(), 1)
+    exact = np.linalg.eigvalsh(_dense(case))
+    h, d = _solver(gpu_context, case)
+    d.set_tolerance(1e-10)
+    d.set_maximum_iterations(200)
+    psi0 = W.random_psi(case.kp.kinetic_energy_squared, 8, seed=13)
+    lib().q3x_debug_fail_eigensolves(0b1010)
+    try:
+        e, _ = d.solve(8, 8, psi0)
+    finally:
+        lib().q3x_debug_fail_eigensolves(0)
+    rep = d.report_dict()
+    assert rep["recovery_collapses"] == 1 and rep["previous_eigenvalue_keeps"] == 1
+    assert rep["unconverged"] == 0
+    assert np.abs(e - exact[:8]).max() <= 1e-8 * HA
+
+
+def test_hegv_device_reports_a_forced_failure(gpu_context):
+    from q3_amd._lib import Q3XError, lib
+    from q3_amd.planewave import DeviceArray, hegv_device
+    n = 8
+    H = np.eye(n, dtype=complex)
+    dH, dS = DeviceArray.from_host(gpu_context, H), DeviceArray.from_host(gpu_context, H)
+    dX = DeviceArray(gpu_context, n * n * 16)
+    lib().q3x_debug_fail_eigensolves(1)
+    with pytest.raises(Q3XError, match="not positive definite"):
+        hegv_device(gpu_context, n, dH, n, dS, n, n, dX, n)
+    w = hegv_device(gpu_context, n, dH, n, dS, n, n, dX, n)  # the hook fired once only
+    np.testing.assert_allclose(w, np.ones(n), atol=1e-12)

@@ -54,7 +54,7 @@ def _check(H, S, w, X, nvec, tol=1e-12):
 
 @pytest.mark.parametrize("n,nvec", [(1, 1), (2, 2), (3, 3), (5, 2), (17, 17), (64, 32), (96, 48), (99, 99),
                                     (100, 50), (160, 160), (250, 125), (320, 160), (450, 100),
-                                    (700, 64)])
+                                    (700, 64), (880, 440)])
 def test_hegv_device_near_identity_overlap(gpu_context, n, nvec):
     rng = np.random.default_rng(n)
     H = _herm(n, rng)
@@ -116,3 +116,21 @@ def test_hegv_device_indefinite_overlap(gpu_context):
     with pytest.raises(Q3XError) as e:
         _solve(gpu_context, H, S, 5)
     assert e.value.status == 4
+
+
+@pytest.mark.parametrize("n,nvec", [(320, 160), (880, 440)])
+def test_forced_barrier_giveup_is_redone_on_one_workgroup(gpu_context, n, nvec):
+    """The multi-workgroup tridiagonalisation started with its give-up flag set (the host-side
+    test hook) runs to its end without waiting; the solve is redone on one workgroup, counted,
+    and still meets scipy.  n = 880 is the subspace of a 216-atom, 440-band request."""
+    from q3_amd._lib import lib
+    rng = np.random.default_rng(n + 1)
+    H = _herm(n, rng)
+    E = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    S = np.eye(n) + 1e-9 * (E + E.conj().T)
+    before = lib().q3x_eig_barrier_redos_total()
+    lib().q3x_debug_force_barrier_giveups(1)
+    w, X = _solve(gpu_context, H, S, nvec)
+    lib().q3x_debug_force_barrier_giveups(0)
+    assert lib().q3x_eig_barrier_redos_total() == before + 1
+    _check(H, S, w, X, nvec)
