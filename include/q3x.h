@@ -145,6 +145,23 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k);
 q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta, int nproj,
                                           int natoms, const int* block_sizes,
                                           const double* d_blocks);
+/* The same projector set built on the device (GPUProjectorSetBuilder::build,
+ * hamiltonian_applicator/projector_set_builder_gpu.hpp:89-300): the species arrays are those of
+ * q3x_kb_projectors, the D blocks those of q3x_hamiltonian_set_projectors.  Only the beta(q)
+ * radial tables (host, cached per species set and cutoff), the k-point's G list and the atoms go
+ * to the device; form factors, structure factors and their products are formed there, in
+ * complex128 and complex64.  Equal to q3x_kb_projectors + q3x_hamiltonian_set_projectors to
+ * rounding (1e-13 relative). */
+q3x_status q3x_hamiltonian_build_projectors(q3x_hamiltonian h, int natoms, const double* atom_pos,
+                                            const int* atom_species, int nspecies, const int* nmesh,
+                                            const double* r_all, const double* rab_all, const int* nbeta,
+                                            const int* l_all, const int* cutoff_all,
+                                            const double* beta_r_all, const int* block_sizes,
+                                            const double* d_blocks);
+/* q3x_kpoint_random_psi of the prepared k-point written on the device (d_psi: npw x nb
+ * complex128), bit for bit the host generator's block; stream-ordered.  No counterpart in the
+ * reference's interface. */
+q3x_status q3x_hamiltonian_random_psi(q3x_hamiltonian h, int nb, unsigned long long seed, void* d_psi);
 /* apply_batch (serial.hpp:110-155): host buffers psi / hpsi, npw x nvec complex. */
 q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* hpsi, int nvec,
                                  int precision);

@@ -65,6 +65,8 @@ SIGNATURES = {
     "q3x_hamiltonian_set_potential": (_i, [_p, _p, _l]),
     "q3x_hamiltonian_prepare_kpoint": (_i, [_p, _p]),
     "q3x_hamiltonian_set_projectors": (_i, [_p, _p, _i, _i, _p, _p]),
+    "q3x_hamiltonian_build_projectors": (_i, [_p, _i, _p, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "q3x_hamiltonian_random_psi": (_i, [_p, _i, ctypes.c_ulonglong, _p]),
     "q3x_hamiltonian_apply": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_noalloc": (_i, [_p, _p, _p, _i, _i]),

@@ -468,7 +468,8 @@ double device_bytes_held(const q3x_context_s* c, const q3x_hamiltonian_s* h) {
         for (const DevBuf* d : {&h->v64, &h->v32, &h->stick_off, &h->stick_i2, &h->stick_col, &h->col_i1, &h->ent_pw,
                                 &h->ent_pos, &h->ekin, &h->outside, &h->beta64, &h->beta32, &h->dblocks, &h->pstart,
                                 &h->psize, &h->poff, &h->zbuf, &h->P, &h->W, &h->gemm_work, &h->io_psi, &h->io_hpsi,
-                                &h->plane_slots, &h->gpack, &h->gout})
+                                &h->plane_slots, &h->gpack, &h->gout, &h->kb_g, &h->kb_tables, &h->kb_form,
+                                &h->kb_ints, &h->kb_dbl})
             b += static_cast<double>(d->size());
         for (const FftTables& t : h->fft)
             b += static_cast<double>(t.tw64.size() + t.tw32.size() + t.plain64.size() + t.plain32.size());
@@ -906,6 +907,46 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
     }
 }
 
+}  // extern "C"
+
+namespace q3x {
+// Per-atom D blocks of a projector set (serial.hpp:587-679): the packed blocks, and per projector
+// its block's first projector, size and offset; marks the set complete.
+q3x_status upload_dblocks(q3x_hamiltonian h, int nproj, int natoms, const int* block_sizes, const double* d_blocks) {
+    hipStream_t st = h->ctx->stream;
+    std::vector<int> ps(nproj), pz(nproj);
+    std::vector<long> po(nproj);
+    std::vector<double> D;
+    int start = 0;
+    long off = 0;
+    for (int a = 0; a < natoms; ++a) {
+        const int bs = block_sizes[a];
+        for (int q = 0; q < bs * bs; ++q) {
+            const double dv = d_blocks[off + q];
+            D.push_back(std::abs(dv) > 1e-15 ? dv : 0.0);  // d_matrix_element_threshold
+        }
+        for (int i = 0; i < bs; ++i) {
+            ps[start + i] = start;
+            pz[start + i] = bs;
+            po[start + i] = off;
+        }
+        start += bs;
+        off += static_cast<long>(bs) * bs;
+    }
+    if (D.empty()) D.push_back(0.0);
+    const bool ok = h->dblocks.upload(D.data(), D.size() * sizeof(double), st, &h->ctx->pin) &&
+                    h->pstart.upload(ps.data(), ps.size() * sizeof(int), st, &h->ctx->pin) &&
+                    h->psize.upload(pz.data(), pz.size() * sizeof(int), st, &h->ctx->pin) &&
+                    h->poff.upload(po.data(), po.size() * sizeof(long), st, &h->ctx->pin);
+    if (!ok) return Q3X_ERR_DEVICE;
+    h->nproj = nproj;
+    h->have_projectors = true;
+    return Q3X_OK;
+}
+}  // namespace q3x
+
+extern "C" {
+
 q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta, int nproj, int natoms,
                                           const int* block_sizes, const double* d_blocks) {
     try {
@@ -927,36 +968,10 @@ q3x_status q3x_hamiltonian_set_projectors(q3x_hamiltonian h, const double* beta,
         const size_t n = static_cast<size_t>(h->npw) * nproj;
         std::vector<float> b32(2 * n);
         for (size_t i = 0; i < 2 * n; ++i) b32[i] = static_cast<float>(beta[i]);
-        std::vector<int> ps(nproj), pz(nproj);
-        std::vector<long> po(nproj);
-        std::vector<double> D;
-        int start = 0;
-        long off = 0;
-        for (int a = 0; a < natoms; ++a) {
-            const int bs = block_sizes[a];
-            for (int q = 0; q < bs * bs; ++q) {
-                const double dv = d_blocks[off + q];
-                D.push_back(std::abs(dv) > 1e-15 ? dv : 0.0);  // d_matrix_element_threshold
-            }
-            for (int i = 0; i < bs; ++i) {
-                ps[start + i] = start;
-                pz[start + i] = bs;
-                po[start + i] = off;
-            }
-            start += bs;
-            off += static_cast<long>(bs) * bs;
-        }
-        if (D.empty()) D.push_back(0.0);
         bool ok = h->beta64.upload(beta, n * 16, st, &h->ctx->pin) &&
-                  h->beta32.upload(b32.data(), n * 8, st, &h->ctx->pin) &&
-                  h->dblocks.upload(D.data(), D.size() * sizeof(double), st, &h->ctx->pin) &&
-                  h->pstart.upload(ps.data(), ps.size() * sizeof(int), st, &h->ctx->pin) &&
-                  h->psize.upload(pz.data(), pz.size() * sizeof(int), st, &h->ctx->pin) &&
-                  h->poff.upload(po.data(), po.size() * sizeof(long), st, &h->ctx->pin);
+                  h->beta32.upload(b32.data(), n * 8, st, &h->ctx->pin);
         if (!ok) return Q3X_ERR_DEVICE;
-        h->nproj = nproj;
-        h->have_projectors = true;
-        return Q3X_OK;
+        return upload_dblocks(h, nproj, natoms, block_sizes, d_blocks);
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_hamiltonian_set_projectors: ") + e.what());
         return Q3X_ERR_INTERNAL;

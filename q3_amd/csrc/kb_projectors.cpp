@@ -17,6 +17,7 @@
 #include <thread>
 #include <vector>
 
+#include "kb_plan.h"
 #include "q3x_internal.h"
 
 namespace q3x {
@@ -206,6 +207,93 @@ std::shared_ptr<const Tables> tables_for(const std::vector<double>& key, Build b
 }
 
 }  // namespace
+
+q3x_status kb_plan(const BasisData& B, const KPointData& K, int natoms, const double* atom_pos,
+                   const int* atom_species, int nspecies, const int* nmesh, const double* r_all,
+                   const double* rab_all, const int* nbeta, const int* l_all, const int* cutoff_all,
+                   const double* beta_r_all, KbPlan& plan) {
+    if (natoms < 0 || nspecies < 1 || !nmesh || !nbeta || (natoms > 0 && (!atom_pos || !atom_species))) {
+        set_error("kb projectors: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    const Lattice& L = B.lat;
+    struct Sp {
+        const double *r, *rab, *beta_r;
+        int nmesh, nbeta;
+        const int *l, *cut;
+    };
+    std::vector<Sp> sp(nspecies);
+    size_t ro = 0, bo = 0, bro = 0;
+    plan.lmax = 0;
+    plan.beta_l.assign(nspecies, {});
+    for (int s = 0; s < nspecies; ++s) {
+        sp[s] = {r_all + ro, rab_all + ro, beta_r_all + bro, nmesh[s], nbeta[s], l_all + bo, cutoff_all + bo};
+        if (nbeta[s] < 0 || nmesh[s] < 0) {
+            set_error("kb projectors: negative projector or mesh count");
+            return Q3X_ERR_ARGUMENT;
+        }
+        for (int i = 0; i < nbeta[s]; ++i) {
+            if (l_all[bo + i] < 0 || l_all[bo + i] > 3 || cutoff_all[bo + i] > nmesh[s]) {
+                set_error("kb projectors: projector l must be 0..3 and cutoff <= nmesh");
+                return Q3X_ERR_ARGUMENT;
+            }
+            plan.lmax = std::max(plan.lmax, l_all[bo + i]);
+            plan.beta_l[s].push_back(l_all[bo + i]);
+        }
+        ro += nmesh[s];
+        bo += nbeta[s];
+        bro += static_cast<size_t>(nbeta[s]) * nmesh[s];
+    }
+    plan.species_of_atom.assign(atom_species, atom_species + natoms);
+    plan.col0_of_atom.resize(natoms);
+    plan.nproj = 0;
+    for (int a = 0; a < natoms; ++a) {
+        const int s = atom_species[a];
+        if (s < 0 || s >= nspecies) {
+            set_error("kb projectors: atom species out of range");
+            return Q3X_ERR_ARGUMENT;
+        }
+        plan.col0_of_atom[a] = plan.nproj;
+        for (int i = 0; i < sp[s].nbeta; ++i) plan.nproj += 2 * sp[s].l[i] + 1;
+    }
+    plan.nq = static_cast<int>(std::sqrt(B.ecutrho) * 1.1 / kDq) + 4;
+    // The beta(q) tables depend on the species data and nq only, not on k: every k-point of a
+    // cell (a band path, a batch) reuses them from a small cache keyed by the exact inputs.
+    std::vector<double> key;
+    key.push_back(plan.nq);
+    key.push_back(nspecies);
+    for (int s = 0; s < nspecies; ++s) {
+        key.push_back(sp[s].nmesh);
+        key.push_back(sp[s].nbeta);
+        for (int i = 0; i < sp[s].nbeta; ++i) {
+            key.push_back(sp[s].l[i]);
+            key.push_back(sp[s].cut[i]);
+        }
+        key.insert(key.end(), sp[s].r, sp[s].r + sp[s].nmesh);
+        key.insert(key.end(), sp[s].rab, sp[s].rab + sp[s].nmesh);
+        key.insert(key.end(), sp[s].beta_r, sp[s].beta_r + static_cast<size_t>(sp[s].nbeta) * sp[s].nmesh);
+    }
+    const int nq = plan.nq;
+    plan.tables = tables_for(key, [&] {
+        auto t = std::make_shared<KbTables>(nspecies);
+        for (int s = 0; s < nspecies; ++s)
+            for (int i = 0; i < sp[s].nbeta; ++i)
+                (*t)[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
+                                               sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
+        return std::shared_ptr<const KbTables>(t);
+    });
+    for (int d = 0; d < 3; ++d) plan.kc[d] = K.k[0] * L.b[0][d] + K.k[1] * L.b[1][d] + K.k[2] * L.b[2][d];
+    plan.tau.resize(3 * static_cast<size_t>(natoms));
+    plan.kdt.resize(natoms);
+    for (int a = 0; a < natoms; ++a) {
+        double* tau = &plan.tau[3 * static_cast<size_t>(a)];
+        for (int d = 0; d < 3; ++d)
+            tau[d] = atom_pos[3 * a] * L.a[0][d] + atom_pos[3 * a + 1] * L.a[1][d] + atom_pos[3 * a + 2] * L.a[2][d];
+        plan.kdt[a] = plan.kc[0] * tau[0] + plan.kc[1] * tau[1] + plan.kc[2] * tau[2];
+    }
+    return Q3X_OK;
+}
+
 }  // namespace q3x
 
 extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, const double* atom_pos,
@@ -214,7 +302,7 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
                                         const int* l_all, const int* cutoff_all, const double* beta_r_all,
                                         double* beta_out, int* nproj_out) {
     try {
-        if (!b || !k || natoms < 0 || nspecies < 1 || !nmesh || !nbeta) {
+        if (!b || !k) {
             set_error("q3x_kb_projectors: invalid argument");
             return Q3X_ERR_ARGUMENT;
         }
@@ -222,68 +310,15 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
         const KPointData& K = k->d;
         const Lattice& L = B.lat;
         const int npw = static_cast<int>(K.gidx.size());
-        // species tables
-        struct Sp {
-            const double *r, *rab, *beta_r;
-            int nmesh, nbeta;
-            const int *l, *cut;
-        };
-        std::vector<Sp> sp(nspecies);
-        size_t ro = 0, bo = 0, bro = 0;
-        int lmax = 0;
-        for (int s = 0; s < nspecies; ++s) {
-            sp[s] = {r_all + ro, rab_all + ro, beta_r_all + bro, nmesh[s], nbeta[s], l_all + bo, cutoff_all + bo};
-            for (int i = 0; i < nbeta[s]; ++i) {
-                if (l_all[bo + i] < 0 || l_all[bo + i] > 3 || cutoff_all[bo + i] > nmesh[s]) {
-                    set_error("q3x_kb_projectors: projector l must be 0..3 and cutoff <= nmesh");
-                    return Q3X_ERR_ARGUMENT;
-                }
-                lmax = std::max(lmax, l_all[bo + i]);
-            }
-            ro += nmesh[s];
-            bo += nbeta[s];
-            bro += static_cast<size_t>(nbeta[s]) * nmesh[s];
-        }
-        int nproj = 0;
-        for (int a = 0; a < natoms; ++a) {
-            int s = atom_species[a];
-            if (s < 0 || s >= nspecies) {
-                set_error("q3x_kb_projectors: atom species out of range");
-                return Q3X_ERR_ARGUMENT;
-            }
-            for (int i = 0; i < sp[s].nbeta; ++i) nproj += 2 * sp[s].l[i] + 1;
-        }
-        if (nproj_out) *nproj_out = nproj;
+        KbPlan plan;
+        const q3x_status st = kb_plan(B, K, natoms, atom_pos, atom_species, nspecies, nmesh, r_all, rab_all, nbeta,
+                                      l_all, cutoff_all, beta_r_all, plan);
+        if (st != Q3X_OK) return st;
+        if (nproj_out) *nproj_out = plan.nproj;
         if (!beta_out) return Q3X_OK;
-
-        const int nq = static_cast<int>(std::sqrt(B.ecutrho) * 1.1 / kDq) + 4;
-        // The beta(q) tables depend on the species data and nq only, not on k: every k-point of a
-        // cell (a band path, a batch) reuses them from a small cache keyed by the exact inputs.
-        std::vector<double> key;
-        key.push_back(nq);
-        key.push_back(nspecies);
-        for (int s = 0; s < nspecies; ++s) {
-            key.push_back(sp[s].nmesh);
-            key.push_back(sp[s].nbeta);
-            for (int i = 0; i < sp[s].nbeta; ++i) {
-                key.push_back(sp[s].l[i]);
-                key.push_back(sp[s].cut[i]);
-            }
-            key.insert(key.end(), sp[s].r, sp[s].r + sp[s].nmesh);
-            key.insert(key.end(), sp[s].rab, sp[s].rab + sp[s].nmesh);
-            key.insert(key.end(), sp[s].beta_r, sp[s].beta_r + static_cast<size_t>(sp[s].nbeta) * sp[s].nmesh);
-        }
-        std::shared_ptr<const Tables> tab = tables_for(key, [&] {
-            auto t = std::make_shared<Tables>(nspecies);
-            for (int s = 0; s < nspecies; ++s)
-                for (int i = 0; i < sp[s].nbeta; ++i)
-                    (*t)[s].push_back(radial_table(sp[s].r, sp[s].rab, sp[s].cut[i], sp[s].l[i],
-                                                   sp[s].beta_r + static_cast<size_t>(i) * sp[s].nmesh, nq));
-            return std::shared_ptr<const Tables>(t);
-        });
-        const Tables& table = *tab;
-        double kc[3];
-        for (int d = 0; d < 3; ++d) kc[d] = K.k[0] * L.b[0][d] + K.k[1] * L.b[1][d] + K.k[2] * L.b[2][d];
+        const KbTables& table = *plan.tables;
+        const double* kc = plan.kc;
+        const int lmax = plan.lmax;
         const int nlm = (lmax + 1) * (lmax + 1);
         std::vector<double> ylm(static_cast<size_t>(npw) * nlm), qabs(npw);
         for (int p = 0; p < npw; ++p) {
@@ -299,25 +334,21 @@ extern "C" q3x_status q3x_kb_projectors(q3x_basis b, q3x_kpoint k, int natoms, c
         std::vector<zcplx> phase(npw);
         int col = 0;
         for (int a = 0; a < natoms; ++a) {
-            const int s = atom_species[a];
-            double tau[3];
-            for (int d = 0; d < 3; ++d)
-                tau[d] = atom_pos[3 * a] * L.a[0][d] + atom_pos[3 * a + 1] * L.a[1][d] +
-                         atom_pos[3 * a + 2] * L.a[2][d];
-            const double kdt = kc[0] * tau[0] + kc[1] * tau[1] + kc[2] * tau[2];
+            const int s = plan.species_of_atom[a];
+            const double* tau = &plan.tau[3 * static_cast<size_t>(a)];
+            const double kdt = plan.kdt[a];
             for (int p = 0; p < npw; ++p) {
                 const double* g = B.g[K.gidx[p]].g;
                 const double ph = kTwoPi * ((g[0] * tau[0] + g[1] * tau[1] + g[2] * tau[2]) + kdt);
                 phase[p] = zcplx(std::cos(ph), -std::sin(ph));
             }
-            for (int i = 0; i < sp[s].nbeta; ++i) {
-                const int l = sp[s].l[i];
+            for (size_t i = 0; i < plan.beta_l[s].size(); ++i) {
+                const int l = plan.beta_l[s][i];
                 for (int p = 0; p < npw; ++p) radial[p] = interp4(table[s][i], qabs[p]);
                 static const zcplx minus_i_pow[4] = {{1, 0}, {0, -1}, {-1, 0}, {0, 1}};
                 const zcplx pref = inv_sqrt_vol * minus_i_pow[l % 4];
                 for (int mi = 0; mi < 2 * l + 1; ++mi) {
-                    const int m = mi == 0 ? 0 : ((mi & 1) ? (mi + 1) / 2 : -(mi / 2));
-                    const int slot = lm_slot(l, m);
+                    const int slot = lm_slot(l, kb_channel_m(mi));
                     zcplx* dst = out + static_cast<size_t>(col) * npw;
                     for (int p = 0; p < npw; ++p)
                         dst[p] = radial[p] * ylm[static_cast<size_t>(p) * nlm + slot] * phase[p] * pref;

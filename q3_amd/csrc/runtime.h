@@ -132,6 +132,9 @@ struct q3x_hamiltonian_s {
     int nproj = 0;
     bool have_projectors = false;
     q3x::DevBuf beta64, beta32, dblocks, pstart, psize, poff;
+    // inputs of the device projector builder (q3x_hamiltonian_build_projectors): G list of the
+    // k-point, beta(q) tables, form factors, channel / atom tables
+    q3x::DevBuf kb_g, kb_tables, kb_form, kb_ints, kb_dbl;
     // Gamma trick: transforms run over the extended list (half sphere, then mirrors) of npw_ext
     // entries, two bands per column: gpack holds the packed pairs, gout their H images
     bool gamma = false;
@@ -157,6 +160,9 @@ q3x_status hamiltonian_apply_device(q3x_hamiltonian h, const double* d_psi, long
 // it; nothing is allocated.  Fails with Q3X_ERR_ARGUMENT when one band does not fit.
 q3x_status hamiltonian_apply_fixed(q3x_hamiltonian h, const double* d_psi, double* d_hpsi, int nvec, int precision,
                                    char* scratch, size_t scratch_bytes);
+// D blocks of the projector set of the prepared k-point (packed per atom, block_sizes summing to
+// nproj); marks the set complete (beta64 / beta32 must already hold the projectors).
+q3x_status upload_dblocks(q3x_hamiltonian h, int nproj, int natoms, const int* block_sizes, const double* d_blocks);
 // Looks up, once per device and with that device current, the device address of the fixed
 // scratch region of q3x_hamiltonian_apply_noalloc (noalloc.hip).
 bool noalloc_scratch_init(int device);

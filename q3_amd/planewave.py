@@ -241,6 +241,30 @@ class PlanewaveHamiltonianApplicator:
         self.kpoint = kpoint
         self.nproj = 0
 
+    def build_projectors(self, crystal: Crystal, species, block_sizes, d_blocks):
+        """The projector set of the prepared k-point built on the device from the species (the
+        arrays kb_projectors takes) and the D blocks; no host projector array exists."""
+        pos = np.ascontiguousarray(crystal.positions, dtype=np.float64)
+        asp = np.ascontiguousarray(crystal.species, dtype=np.int32)
+        nmesh = np.ascontiguousarray([len(s["r"]) for s in species], dtype=np.int32)
+        r_all = np.ascontiguousarray(np.concatenate([s["r"] for s in species]), dtype=np.float64)
+        rab_all = np.ascontiguousarray(np.concatenate([s["rab"] for s in species]), dtype=np.float64)
+        nbeta = np.ascontiguousarray([len(s["l"]) for s in species], dtype=np.int32)
+        l_all = np.ascontiguousarray(np.concatenate([s["l"] for s in species]), dtype=np.int32)
+        cut_all = np.ascontiguousarray(np.concatenate([s["cutoff"] for s in species]), dtype=np.int32)
+        br_all = np.ascontiguousarray(np.concatenate([np.asarray(s["beta_r"]).ravel() for s in species]),
+                                      dtype=np.float64)
+        bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+        db = np.ascontiguousarray(d_blocks, dtype=np.float64)
+        check(lib().q3x_hamiltonian_build_projectors(self.h, len(asp), _ptr(pos), _ptr(asp), len(species),
+                                                     _ptr(nmesh), _ptr(r_all), _ptr(rab_all), _ptr(nbeta),
+                                                     _ptr(l_all), _ptr(cut_all), _ptr(br_all), _ptr(bs), _ptr(db)))
+        self.nproj = int(bs.sum())
+
+    def random_psi_device(self, nbands: int, seed: int, d_psi: int):
+        """The starting block of KPoint.random_psi(nbands, seed) written on the device."""
+        check(lib().q3x_hamiltonian_random_psi(self.h, nbands, seed, ctypes.c_void_p(d_psi)))
+
     def set_projectors(self, beta: np.ndarray, block_sizes, d_blocks):
         b = np.asfortranarray(beta, dtype=np.complex128)
         bs = np.ascontiguousarray(block_sizes, dtype=np.int32)

@@ -152,29 +152,48 @@ def test_dependent_starting_columns_take_the_mgs_path(gpu_context):
     np.testing.assert_allclose(psi.conj().T @ psi, np.eye(8), atol=1e-10)
 
 
-def test_failed_subspace_eigensolves_are_recovered(gpu_context):
-    """The eigensolve of iteration 1 fails (the basis holds more than nb columns): the subspace is
-    collapsed and rebuilt from the vectors; the solve after the collapse fails too: the previous
-    eigenvalues are kept for that iteration (davidson.hpp:520-590).  The solve still converges to
-    the dense eigenvalues.  Failures are forced by the eigensolver's test hook (solves 1 and 3
-    from now: iteration 0, the failing iteration-1 solve, the collapse's rebuilt solve, then the
-    failing solve after it)."""
+def _forced_failure_solve(gpu_context, mask):
     from q3_amd._lib import lib
     case = kcase(W.cfg1(), 1)
     exact = np.linalg.eigvalsh(_dense(case))
     h, d = _solver(gpu_context, case)
     d.set_tolerance(1e-10)
     d.set_maximum_iterations(200)
+    # a basis of 4 nb keeps the first collapse out of the way (with the default 2 nb the basis is
+    # full after iteration 0 and every iteration collapses)
+    d.set_basis_expansion_factor(4)
     psi0 = W.random_psi(case.kp.kinetic_energy_squared, 8, seed=13)
-    lib().q3x_debug_fail_eigensolves(0b1010)
+    lib().q3x_debug_fail_eigensolves(mask)
     try:
         e, _ = d.solve(8, 8, psi0)
     finally:
         lib().q3x_debug_fail_eigensolves(0)
-    rep = d.report_dict()
-    assert rep["recovery_collapses"] == 1 and rep["previous_eigenvalue_keeps"] == 1
+    return e, d.report_dict(), exact
+
+
+def test_failed_subspace_eigensolve_forces_a_collapse(gpu_context):
+    """The eigensolve of iteration 1 fails while the basis holds more than nb columns: the
+    subspace is collapsed, rebuilt from the vectors and solved again (davidson.hpp:520-575); the
+    solve goes on to the dense eigenvalues.  The failure is forced by the eigensolver's test hook
+    (solve 1 from now: solve 0 is iteration 0)."""
+    e, rep, exact = _forced_failure_solve(gpu_context, 0b10)
+    assert rep["recovery_collapses"] == 1 and rep["previous_eigenvalue_keeps"] == 0
     assert rep["unconverged"] == 0
     assert np.abs(e - exact[:8]).max() <= 1e-8 * HA
+
+
+def test_second_failure_keeps_the_previous_eigenvalues(gpu_context):
+    """The solve after the recovery collapse fails too (solves 1 and 3 fail; solve 2 is the
+    collapse's rebuilt subspace): the previous iteration's eigenvalues are kept
+    (davidson.hpp:576-589).  As in the reference, the kept values then differ from the previous
+    ones by zero, so the eigenvalue-change test (davidson.hpp:607-629) passes for every band and
+    the solve stops at that iteration with them: Ritz values of the iteration-0 subspace, upper
+    bounds of the exact eigenvalues."""
+    e, rep, exact = _forced_failure_solve(gpu_context, 0b1010)
+    assert rep["recovery_collapses"] == 1 and rep["previous_eigenvalue_keeps"] == 1
+    assert rep["iterations"] == 2 and rep["unconverged"] == 0
+    assert np.all(np.isfinite(e)) and np.all(np.diff(e) >= 0)
+    assert np.all(e >= exact[:8] - 1e-10)
 
 
 def test_hegv_device_reports_a_forced_failure(gpu_context):
