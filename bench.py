@@ -45,8 +45,14 @@ def parse():
     p.add_argument("--gamma-solves", type=int, default=8)
     p.add_argument("--gamma-workers", type=int, default=4)
     p.add_argument("--no-serve", action="store_true")
+    p.add_argument("--no-stream", action="store_true", help="skip the full-distribution serving window")
     p.add_argument("--serve-requests", type=int, default=24)
     p.add_argument("--serve-max-atoms", type=int, default=16)
+    p.add_argument("--stream-window", type=float, default=5.0,
+                   help="full-distribution serving: seconds of the 20 req/s stream submitted")
+    p.add_argument("--stream-run", type=float, default=40.0,
+                   help="full-distribution serving: seconds after the first arrival at which unstarted work is "
+                        "cancelled")
     p.add_argument("--davidson-iters", type=int, default=60)
     p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "copy"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
@@ -333,6 +339,88 @@ def batched_gamma(args, tolerances=None):
         return {"error": str(e)}
 
 
+def serving_stream(args):
+    """The cfg5 stream as stated (Poisson 20 req/s, seed 4, Si/C cells of 2 to 216 atoms, 15-25 Ha,
+    8-40 k-points, bands 2 N + 8), over a bounded arrival window: the requests arriving in the
+    first --stream-window seconds are submitted at their arrival times; --stream-run seconds after
+    the first arrival the work not yet started is cancelled.  Reported on the server's clock:
+    requests completed by then (sustained req/s against the 20 offered), their p50/p99 latency,
+    the backlog when the arrival window closes and at the end of the run, and k-point solves/s.
+    The stream offers far more work than one GPU does in real time (a 216-atom, 440-band request
+    with tens of k-points alone takes tens of seconds), so the backlog grows; the figures say by
+    how much."""
+    import threading
+    from q3_amd import BandRequestSpec, BandStructureServer, FP64
+    from q3_amd import workloads as W
+    from q3_amd.planewave import PlanewaveBasis
+    try:
+        reqs = [r for r in W.cfg5_requests(2000, rate_per_s=20.0, seed=4) if r.arrival_s < args.stream_window]
+        sp = W.serving_species()
+        t0 = time.time()
+        specs = []
+        for r in reqs:
+            basis = PlanewaveBasis(r.crystal, r.ecutwfc_ry, 0.0, True, (1, 1, 1))
+            V = W.local_potential(r.crystal, basis, sp)
+            bs, db = W.d_blocks_for(r.crystal, sp)
+            specs.append(BandRequestSpec(basis, V, r.kpoints, sp, bs, db, r.nbands, r.noccupied,
+                                         tolerance_ry=2e-6, max_iterations=args.davidson_iters, precision=FP64))
+        log(f"stream setup {time.time() - t0:.1f}s: {len(specs)} requests in the first {args.stream_window:g} s")
+        srv = BandStructureServer(workers=args.batch_workers)
+        results = {}
+
+        def collect(tickets):
+            for i, t in tickets:
+                status, _, st = srv.wait_status(t)
+                results[i] = ("done" if status == 0 else "cancelled", st)
+
+        start = time.perf_counter()
+        tickets = []
+        for i, (r, spec) in enumerate(zip(reqs, specs)):
+            delay = start + r.arrival_s - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append((i, srv.submit(spec)))
+        window_end = time.perf_counter() - start
+        waiter = threading.Thread(target=collect, args=(tickets,))
+        waiter.start()
+        remaining = start + args.stream_run - time.perf_counter()
+        if remaining > 0:
+            time.sleep(remaining)
+        cancelled = srv.cancel_pending()
+        waiter.join()
+        srv.close()
+        done = [results[i][1] for i in range(len(reqs)) if results[i][0] == "done"]
+        # the first arrival on the server's clock
+        sub0 = min(results[i][1]["submit_s"] - reqs[i].arrival_s for i in range(len(reqs)))
+        # server-clock times relative to the first arrival
+        fin = sorted(st["done_s"] - sub0 for st in done)
+        lat = np.array([st["done_s"] - st["submit_s"] for st in done]) if done else np.zeros(0)
+        by_run = sum(1 for t in fin if t <= args.stream_run)
+        by_window = sum(1 for t in fin if t <= window_end)
+        # every k-point solved by the end of the run, of completed and of cancelled requests
+        kdone = sum(results[i][1]["kpoints_done"] for i in range(len(reqs)))
+        k_end = max((results[i][1]["done_s"] - sub0 for i in range(len(reqs)) if results[i][1]["kpoints_done"]),
+                    default=1.0)
+        nat = [len(r.crystal.species) for r in reqs]
+        out = {"window_s": args.stream_window, "run_s": args.stream_run, "offered": len(reqs),
+               "offered_req_per_s": 20.0, "atoms_min": min(nat), "atoms_max": max(nat),
+               "kpoints_offered": int(sum(len(r.kpoints) for r in reqs)),
+               "completed_by_run_end": by_run, "sustained_req_per_s": by_run / args.stream_run,
+               "backlog_at_window_end": len(reqs) - by_window, "backlog_at_run_end": len(reqs) - by_run,
+               "cancelled_requests": cancelled, "kpoint_solves": int(kdone),
+               "kpoint_solves_per_s": kdone / max(k_end, 1e-9),
+               "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+               "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+               "eig_barrier_redos": int(sum(st.get("eig_barrier_redos", 0) for st in done)),
+               "workers": args.batch_workers, "tolerance_ry": 2e-6}
+        log(f"stream: {by_run}/{len(reqs)} requests done in {args.stream_run:g} s "
+            f"({out['sustained_req_per_s']:.2f} req/s of 20 offered), backlog {out['backlog_at_run_end']}")
+        return out
+    except Exception as e:  # reported, not hidden
+        log(f"stream serving failed: {e}")
+        return {"error": str(e)}
+
+
 def serving(args):
     """A Poisson stream (20 req/s, seed 4) of band-structure requests through the continuous-
     batching server.  The sample: the first --serve-requests requests of the stream with the cell
@@ -416,7 +504,10 @@ def main():
         print(json.dumps(out))
         return 0
     if args.mode == "serve":
-        print(json.dumps({"mode": "serve", **serving(args)}))
+        out = {"mode": "serve", **serving(args)}
+        if not args.no_stream:
+            out["stream"] = serving_stream(args)
+        print(json.dumps(out))
         return 0
 
     from q3_amd import FP64, MIXED, Context, PlanewaveDavidsonDiagonalizer, PlanewaveHamiltonianApplicator
@@ -566,6 +657,8 @@ def main():
         tols = {k: v["tolerance_ry"] for k, v in davidson["all_bands_1e-6_ha"].items() if isinstance(v, dict)}
     batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args, tols)}
     serve = None if args.no_serve else serving(args)
+    if serve is not None and not args.no_stream:
+        serve = {**serve, "stream": serving_stream(args)}
 
     cpu = None if args.no_cpu else cpu_baseline(basis, kp, V, beta, bs, db, psi0)
 

@@ -345,11 +345,18 @@ typedef struct {
     double solve_s;                    /* worker time in the Davidson solves */
     int eig_barrier_redos;             /* subspace eigensolves redone on one workgroup (see
                                           q3x_davidson_report), summed over the k-points */
+    int kpoints_done;                  /* k-points solved (fewer than requested after a cancel) */
 } q3x_request_stats;
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation */
 q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out);
 /* waits for every submitted request, then stops the workers */
 q3x_status q3x_server_destroy(q3x_server s);
+/* Drops every k-point not yet started: requests not yet admitted, tasks of admitted requests not
+ * yet handed out, prepared k-points not yet taken by a worker.  k-points already on the device
+ * finish.  Each affected request completes with Q3X_ERR_STATE ("cancelled ...") from
+ * q3x_server_wait once its running k-points are done; *cancelled (may be null) counts them.  No
+ * counterpart in the reference (it serves one calculation per process). */
+q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled);
 q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* req, double* bytes);
 /* copies the request's arrays and returns at once with a ticket */
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* req, long* ticket);

@@ -40,7 +40,7 @@ class BandRequest(ctypes.Structure):
 
 class RequestStats(ctypes.Structure):
     _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i),
-                ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i)]
+                ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i), ("kpoints_done", _i)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/q3x.h
@@ -94,6 +94,7 @@ SIGNATURES = {
                                        ctypes.POINTER(DavidsonReport)]),
     "q3x_server_create": (_i, [_i, _i, _d, _pp]),
     "q3x_server_destroy": (_i, [_p]),
+    "q3x_server_cancel_pending": (_i, [_p, _p]),
     "q3x_server_estimate": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_d)]),
     "q3x_server_submit": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_l)]),
     "q3x_server_wait": (_i, [_p, _l, _p, ctypes.POINTER(RequestStats)]),

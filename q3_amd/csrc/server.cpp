@@ -6,10 +6,10 @@
 // budget (the reference's PlanewaveMemoryEstimate::compute terms,
 // iterator/kohn_sham/planewave/internal/memory_estimate.hpp:75-215); an admitted request's
 // k-points join one shared task queue.  P preparation threads take k-point tasks from it in
-// order and do their host work (the k+G list, the KB projectors, the seeded starting block);
-// W worker threads -- each with its own context, stream and Davidson workspace -- take prepared
-// k-points, upload them and solve, so the host setup of the next k-points overlaps the device
-// work of the current ones.  A request that arrives
+// order and do their host work (the k+G list, the box maps and the stick layout); W worker
+// threads -- each with its own context, stream and Davidson workspace -- take prepared k-points,
+// form the KB projector set and the seeded starting block on the device and solve, so the host
+// setup of the next k-points overlaps the device work of the current ones.  A request that arrives
 // while others are running is admitted as soon as the budget allows and its k-points interleave
 // with theirs on the device.  What admission counts as resident: the wavefunctions of every
 // admitted request not yet collected, the blocks parked in the reuse pool, and the larger of the
@@ -59,6 +59,7 @@ struct Request {
     std::vector<size_t> psi_bytes;
     int iterations = 0;
     int eig_barrier_redos = 0;
+    int kpoints_done = 0;
     double t_submit = 0, t_admit = 0, t_done = 0;
     double setup_s = 0, solve_s = 0;
 };
@@ -70,8 +71,7 @@ struct Prepared {
     std::shared_ptr<Request> r;
     int k = -1;
     q3x_kpoint kp = nullptr;
-    int npw = 0, nproj = 0;
-    std::vector<double> beta, p0;  // projectors npw x nproj (column-major), starting block npw x nb
+    int npw = 0;
     q3x_status status = Q3X_OK;
     std::string err;
     double host_s = 0.0;
@@ -195,25 +195,10 @@ struct q3x_server_s {
             p->err = q3x_last_error();
             return std::move(p);
         };
+        // the host part of a k-point is its plane-wave list, box maps and stick layout; its
+        // projector set and starting block are formed on the device by the worker
         q3x_status s = q3x_kpoint_create(r->basis, &r->kpts[3 * k], &p->kp, &p->npw);
         if (s != Q3X_OK) return fail(s);
-        auto kb = [&](double* out) {
-            return q3x_kb_projectors(r->basis, p->kp, r->natoms, r->atom_pos.data(), r->atom_species.data(),
-                                     r->nspecies, r->nmesh.data(), r->r_all.data(), r->rab_all.data(),
-                                     r->nbeta.data(), r->l_all.data(), r->cutoff_all.data(), r->beta_r_all.data(),
-                                     out, &p->nproj);
-        };
-        if ((s = kb(nullptr)) != Q3X_OK) return fail(s);
-        // q3x_kb_projectors writes nproj x npw (one projector after another): the npw x nproj
-        // column-major layout the Hamiltonian takes
-        p->beta.resize(static_cast<size_t>(p->npw) * std::max(p->nproj, 1) * 2);
-        if ((s = kb(p->beta.data())) != Q3X_OK) return fail(s);
-        // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, so a
-        // request's result does not depend on what else runs (plane-wave unit vectors, the
-        // solver's own seed, can miss symmetry sectors of these undisplaced crystals)
-        p->p0.resize(static_cast<size_t>(p->npw) * r->nb * 2);
-        if ((s = q3x_kpoint_random_psi(p->kp, r->nb, static_cast<unsigned long long>(k), p->p0.data())) != Q3X_OK)
-            return fail(s);
         p->host_s = std::chrono::duration<double>(clk::now() - ts).count();
         return p;
     }
@@ -267,7 +252,7 @@ struct q3x_server_s {
     q3x_status solve_kpoint(int w, q3x_context ctx, Prepared& p, std::string& err) {
         const clk::time_point ts = clk::now();
         Request& r = *p.r;
-        const int k = p.k, npw = p.npw, nproj = p.nproj;
+        const int k = p.k, npw = p.npw;
         q3x_hamiltonian& h = hams[w];
         auto fail = [&](q3x_status st) {
             err = q3x_last_error();
@@ -283,9 +268,13 @@ struct q3x_server_s {
                 return fail(s);
             ham_ticket[w] = r.ticket;
         }
+        // projector set on the device (form factors x structure factors; the beta(q) tables of the
+        // cell come from the shared cache)
         if ((s = q3x_hamiltonian_prepare_kpoint(h, p.kp)) != Q3X_OK ||
-            (nproj > 0 && (s = q3x_hamiltonian_set_projectors(h, p.beta.data(), nproj, r.natoms, r.block_sizes.data(),
-                                                               r.d_blocks.data())) != Q3X_OK))
+            (s = q3x_hamiltonian_build_projectors(h, r.natoms, r.atom_pos.data(), r.atom_species.data(), r.nspecies,
+                                                  r.nmesh.data(), r.r_all.data(), r.rab_all.data(), r.nbeta.data(),
+                                                  r.l_all.data(), r.cutoff_all.data(), r.beta_r_all.data(),
+                                                  r.block_sizes.data(), r.d_blocks.data())) != Q3X_OK)
             return fail(s);
         const size_t psi_bytes = static_cast<size_t>(npw) * r.nb * 16;
         void* psi = take_psi(ctx, psi_bytes);
@@ -295,7 +284,10 @@ struct q3x_server_s {
             r.psi[k] = {ctx, psi};
             r.psi_bytes[k] = psi_bytes;
         }
-        if ((s = q3x_copy_to_device(ctx, psi, p.p0.data(), p.p0.size() * sizeof(double))) != Q3X_OK) return fail(s);
+        // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, formed on the
+        // device, so a request's result does not depend on what else runs (plane-wave unit vectors,
+        // the solver's own seed, can miss symmetry sectors of these undisplaced crystals)
+        if ((s = q3x_hamiltonian_random_psi(h, r.nb, static_cast<unsigned long long>(k), psi)) != Q3X_OK) return fail(s);
         q3x_davidson_options o = r.opt;
         o.seed_from_psi = 1;
         q3x_davidson_report rep{};
@@ -307,6 +299,7 @@ struct q3x_server_s {
             std::lock_guard<std::mutex> g(mu);
             r.iterations += rep.iterations;
             r.eig_barrier_redos += rep.eig_barrier_redos;
+            ++r.kpoints_done;
             r.setup_s += p.host_s + std::chrono::duration<double>(tsolve - ts).count();
             r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
         }
@@ -410,6 +403,60 @@ q3x_status q3x_server_destroy(q3x_server s) {
         return Q3X_OK;
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_server_destroy: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
+}
+
+q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled) {
+    try {
+        if (!s) {
+            set_error("q3x_server_cancel_pending: null server");
+            return Q3X_ERR_ARGUMENT;
+        }
+        std::lock_guard<std::mutex> g(s->mu);
+        int n = 0;
+        auto finish = [&](Request& r) {  // k-points still running finish on their own
+            if (r.status == Q3X_OK) {
+                r.status = Q3X_ERR_STATE;
+                r.error = "cancelled before all k-points were solved";
+                ++n;
+            }
+            if (r.remaining == 0 && !r.done) {
+                r.done = true;
+                r.t_done = now_s(s->t0);
+                if (r.admitted) {
+                    s->work_running -= r.bytes - r.bytes_psi;
+                    if (s->work_running < 1.0) s->work_running = 0.0;
+                }
+            }
+        };
+        // not admitted: nothing of theirs is on the device or in the budget
+        for (auto& r : s->waiting) {
+            r->remaining = 0;
+            finish(*r);
+        }
+        s->waiting.clear();
+        // admitted, k-points not yet handed to a preparation thread
+        for (auto& r : s->running) {
+            r->remaining -= r->nk - r->next_task;
+            r->next_task = r->nk;
+            finish(*r);
+        }
+        s->running.clear();
+        // prepared, not yet taken by a worker
+        for (auto& p : s->ready) {
+            if (p->kp) q3x_kpoint_destroy(p->kp);
+            --p->r->remaining;
+            finish(*p->r);
+        }
+        s->ready.clear();
+        if (cancelled) *cancelled = n;
+        s->admit_locked();
+        s->cv_done.notify_all();
+        s->cv_task.notify_all();
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_server_cancel_pending: ") + e.what());
         return Q3X_ERR_INTERNAL;
     }
 }
@@ -536,6 +583,7 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
             stats->setup_s = r->setup_s;
             stats->solve_s = r->solve_s;
             stats->eig_barrier_redos = r->eig_barrier_redos;
+            stats->kpoints_done = r->kpoints_done;
         }
         {
             // the request's wavefunction blocks go back to the pool, and its share of the
@@ -546,7 +594,7 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
                     s->psi_pool.emplace(r->psi_bytes[k], r->psi[k].second);
                     s->pool_bytes += static_cast<double>(r->psi_bytes[k]);
                 }
-            s->psi_held -= r->bytes_psi;
+            if (r->admitted) s->psi_held -= r->bytes_psi;  // a request cancelled before admission held none
             if (s->psi_held < 1.0) s->psi_held = 0.0;
             s->admit_locked();
         }

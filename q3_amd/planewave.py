@@ -500,20 +500,36 @@ class BandStructureServer:
         return t.value
 
     def wait(self, ticket: int):
-        """Returns (eigenvalues nk x nb [Ry], stats dict) and frees the request's device state."""
+        """Returns (eigenvalues nk x nb [Ry], stats dict) and frees the request's device state;
+        raises if the request failed or was cancelled."""
+        status, eig, stats = self.wait_status(ticket)
+        check(status)
+        return eig, stats
+
+    def wait_status(self, ticket: int):
+        """Like wait() but returns (status, eigenvalues, stats) instead of raising; the stats of
+        a cancelled request count the k-points it solved."""
         spec, arrays = self._live[ticket]
         eig = np.zeros((arrays["k"].shape[0], spec.nbands))
         st = _lib.RequestStats()
         try:
-            check(lib().q3x_server_wait(self.h, ticket, _ptr(eig), ctypes.byref(st)))
+            status = lib().q3x_server_wait(self.h, ticket, _ptr(eig), ctypes.byref(st))
         finally:
             del self._live[ticket]
-        return eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
-                     "footprint_bytes": st.footprint_bytes, "iterations": st.iterations,
-                     "setup_s": st.setup_s, "solve_s": st.solve_s, "eig_barrier_redos": st.eig_barrier_redos}
+        return status, eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
+                             "footprint_bytes": st.footprint_bytes, "iterations": st.iterations,
+                             "setup_s": st.setup_s, "solve_s": st.solve_s, "eig_barrier_redos": st.eig_barrier_redos,
+                             "kpoints_done": st.kpoints_done}
 
     def in_flight_bytes(self) -> float:
         return lib().q3x_server_in_flight_bytes(self.h)
+
+    def cancel_pending(self) -> int:
+        """Drops every k-point not yet started; affected requests then fail with StateError in
+        wait().  Returns the number of requests affected."""
+        n = ctypes.c_int(0)
+        check(lib().q3x_server_cancel_pending(self.h, ctypes.byref(n)))
+        return n.value
 
     def close(self):
         if getattr(self, "h", None):

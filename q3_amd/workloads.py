@@ -102,13 +102,22 @@ def local_potential(crystal: Crystal, basis, species_list, grid: str = "wave") -
     omega = basis.cell_volume
     vg = np.zeros(len(g2), np.complex128)
     nz = gabs2 > 1e-12
-    tau = np.asarray(crystal.positions) @ np.asarray(crystal.lattice)  # alat units
+    pos = np.asarray(crystal.positions, float)
+    # structure factors exp(-i 2 pi m . x) (crystal coordinates x, Miller m; = exp(-i G . tau)) of
+    # each species over the Miller box that holds the sphere, as products of per-axis phases
+    # summed over the atoms by one matrix product: O(atoms x box) multiply-adds, not exps
+    half = np.abs(m).max(axis=0)
+    axes = [np.arange(-h, h + 1) for h in half]
     for s_idx, sp in enumerate(species_list):
         form = np.zeros(len(g2))
         form[nz] = -4 * math.pi * sp["z_val"] * np.exp(-gabs2[nz] * sp["r_loc"] ** 2 / 2) / (omega * gabs2[nz])
-        sf = np.zeros(len(g2), np.complex128)
-        for a in np.nonzero(np.asarray(crystal.species) == s_idx)[0]:
-            sf += np.exp(-2j * math.pi * (g @ tau[a]))
+        x = pos[np.asarray(crystal.species) == s_idx]
+        if len(x) == 0:
+            continue
+        e = [np.exp(-2j * math.pi * np.outer(x[:, d], axes[d])) for d in range(3)]  # atoms x n_d
+        p12 = (e[0][:, :, None] * e[1][:, None, :]).reshape(len(x), -1)
+        sbox = (p12.T @ e[2]).reshape(len(axes[0]), len(axes[1]), len(axes[2]))
+        sf = sbox[m[:, 0] + half[0], m[:, 1] + half[1], m[:, 2] + half[2]]
         vg += form * sf
     dims = basis.wave_dims if grid == "wave" else basis.rho_dims
     box = np.zeros(dims, np.complex128)

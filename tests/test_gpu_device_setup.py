@@ -97,4 +97,4 @@ def test_device_random_block_is_the_host_block(gpu_context):
         gpu_context.synchronize()
         got = d.download((kp.npw, nb))
         want = kp.random_psi(nb, 77)
-        assert np.array_equal(got.view(np.uint64), want.view(np.uint64))
+        assert got.tobytes(order="F") == want.tobytes(order="F")
