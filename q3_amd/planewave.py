@@ -16,7 +16,9 @@ Arrays: complex band blocks are numpy ``complex128`` of shape (npw, nvec) in For
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -30,6 +32,28 @@ MIXED = 1
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# Device handles still open when the interpreter exits are released by an exit handler, which
+# runs before the HIP runtime tears itself down, in a fixed order: servers and batches first
+# (their worker threads use contexts of their own), then applicators and device arrays, then
+# contexts.  Left to garbage collection the order is arbitrary.
+_open = (weakref.WeakSet(), weakref.WeakSet(), weakref.WeakSet())
+
+
+def _track(obj, rank: int):
+    _open[rank].add(obj)
+    return obj
+
+
+@atexit.register
+def _release_open_handles():
+    for group in _open:
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:  # noqa: BLE001 - best effort at exit
+                pass
 
 
 @dataclass
@@ -55,6 +79,7 @@ class Context:
         h = ctypes.c_void_p()
         check(lib().q3x_context_create(device, ctypes.byref(h)))
         self.h = h
+        _track(self, 2)
 
     def synchronize(self):
         check(lib().q3x_context_synchronize(self.h))
@@ -77,6 +102,7 @@ class DeviceArray:
         p = ctypes.c_void_p()
         check(lib().q3x_device_alloc(context.h, self.nbytes, ctypes.byref(p)))
         self.ptr = p.value
+        _track(self, 1)
 
     @classmethod
     def from_host(cls, context: Context, a: np.ndarray):
@@ -97,9 +123,12 @@ class DeviceArray:
         return out
 
     def free(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and getattr(self.context, "h", None):
             lib().q3x_device_free(self.context.h, ctypes.c_void_p(self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+    def close(self):
+        self.free()
 
     def __del__(self):
         self.free()
@@ -231,6 +260,7 @@ class PlanewaveHamiltonianApplicator:
         self.h = h
         self.kpoint = None
         self.nproj = 0
+        _track(self, 1)
 
     def update_potential(self, v_r: np.ndarray):
         v = np.ascontiguousarray(v_r, dtype=np.float64).ravel()
@@ -292,10 +322,13 @@ class PlanewaveHamiltonianApplicator:
         check(lib().q3x_hamiltonian_apply_noalloc(self.h, ctypes.c_void_p(d_psi), ctypes.c_void_p(d_hpsi), nvec,
                                                   precision))
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "h", None):
             lib().q3x_hamiltonian_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class PlanewaveDavidsonDiagonalizer:
@@ -361,6 +394,7 @@ class KPointBatch:
         h = ctypes.c_void_p()
         check(lib().q3x_batch_create(device, workers, ctypes.byref(h)))
         self.h = h
+        _track(self, 0)
         self.workers = workers
         self.npw = []
         self._keep = []  # the batch's Hamiltonians refer to these bases and k-point lists
@@ -451,6 +485,7 @@ class BandStructureServer:
         h = ctypes.c_void_p()
         check(lib().q3x_server_create(device, workers, float(hbm_budget_bytes), ctypes.byref(h)))
         self.h = h
+        _track(self, 0)
         self._live = {}  # ticket -> (spec, arrays) kept alive until the result is collected
 
     def _request(self, spec: BandRequestSpec):
