@@ -246,6 +246,10 @@ typedef struct {
     int seed_from_psi;        /* 1: psi_inout seeds the basis (davidson.hpp:1000-1045);
                                  0: kinetic-sorted unit vectors (davidson.hpp:1047-1101) */
     int final_fp64_sweep;     /* MIXED only: finish with fp64 Rayleigh-Ritz + residual sweep */
+    int stored_eigenvalues;   /* 1: eig_out holds the k-point state's stored eigenvalues on entry;
+                                 when any is nonzero they are the previous eigenvalues of the
+                                 first convergence test, else the diagonal of the starting H_sub
+                                 is (davidson.hpp:473-493).  0: always the diagonal. */
 } q3x_davidson_options;
 
 typedef struct {
@@ -335,7 +339,23 @@ typedef struct {
     const double* d_blocks;
     int nbands;
     int noccupied;
-    q3x_davidson_options options;  /* tolerance, iterations, precision (seed_from_psi ignored) */
+    q3x_davidson_options options;  /* tolerance, iterations, precision (seed_from_psi and
+                                      stored_eigenvalues are set by the server) */
+    /* The k-loop of the reference's non-self-consistent run over a band path
+     * (iterator/kohn_sham/planewave/pseudopotential/init.hpp:1036-1106, 1599-1815).
+     * kpath_propagation 0: the k-points are independent, each starts from
+     *   q3x_hamiltonian_random_psi with its index as the seed, and they run concurrently on the
+     *   server's workers.
+     * kpath_propagation 1: the k-points are solved in order on one worker; k-point k starts
+     *   from k-1's eigenvectors (the leading min(npw) coefficients of each band, zeros after them)
+     *   and eigenvalues unless k-1 is a band-path segment boundary (a multiple of
+     *   band_path_points_per_segment below band_path_segments x points; none without a band
+     *   path), else from its own random start; then, when nk > 2, the backward fix re-solves
+     *   k-points edge+5 .. edge in descending order for edge 0 and the first k-point of every
+     *   later segment, each seeded by the k-point after it. */
+    int kpath_propagation;
+    int band_path_segments;            /* band_path_.size() - 1; 0: no band path */
+    int band_path_points_per_segment;
 } q3x_band_request;
 typedef struct {
     double submit_s, admit_s, done_s;  /* seconds since the server started */
@@ -346,6 +366,7 @@ typedef struct {
     int eig_barrier_redos;             /* subspace eigensolves redone on one workgroup (see
                                           q3x_davidson_report), summed over the k-points */
     int kpoints_done;                  /* k-points solved (fewer than requested after a cancel) */
+    int solves;                        /* Davidson solves run, the backward fix's re-solves included */
 } q3x_request_stats;
 /* hbm_budget_bytes <= 0: 80% of the device memory free at creation */
 q3x_status q3x_server_create(int device, int workers, double hbm_budget_bytes, q3x_server* out);

@@ -291,8 +291,10 @@ def zhegv(H, S):
 
 
 def davidson(dims, fftmap, ekin, V, beta, block_sizes, dblocks, nb, nocc, psi0=None, tol=1e-8,
-             max_iter=20, factor=2, minus_map=None):
-    """Returns (eigenvalues, psi (npw x nb), report dict)."""
+             max_iter=20, factor=2, minus_map=None, stored_eigenvalues=None):
+    """Returns (eigenvalues, psi (npw x nb), report dict).  stored_eigenvalues: the k-point
+    state's stored eigenvalues, the previous eigenvalues of the first convergence test when any
+    is nonzero (davidson.hpp:473-493)."""
     dims, fm, ek, Vr, b, nproj, bs, db = _ham_args(dims, fftmap, ekin, V, beta, block_sizes, dblocks)
     npw = len(ek)
     if psi0 is None:
@@ -302,6 +304,9 @@ def davidson(dims, fftmap, ekin, V, beta, block_sizes, dblocks, nb, nocc, psi0=N
         psi = np.asfortranarray(psi0, dtype=np.complex128).copy(order="F")
         seed = 1
     eig = np.zeros(nb)
+    if stored_eigenvalues is not None:
+        eig[:] = np.asarray(stored_eigenvalues, dtype=np.float64).reshape(nb)
+        seed |= 2
     rep = np.zeros(3, np.int32)
     mm = None if minus_map is None else _c(minus_map, np.int32)
     lib().orc_davidson2(_ptr(dims), npw, _ptr(fm), None if mm is None else _ptr(mm), _ptr(ek), _ptr(Vr), nproj,
@@ -326,3 +331,81 @@ def set_threads(n: int) -> None:
 
 def get_threads() -> int:
     return int(lib().orc_get_threads())
+
+
+# ---- the non-self-consistent k-loop over a band path
+# (iterator/kohn_sham/planewave/pseudopotential/init.hpp:1036-1106, 1599-1815)
+
+BACKWARD_SWEEP_DEPTH = 5  # pseudopotential/init.hpp:1767
+
+
+def at_band_path_segment_boundary(k: int, segments: int, points_per_segment: int) -> bool:
+    """is_at_band_path_segment_boundary (pseudopotential/init.hpp:1697-1715): k is a multiple of
+    the points per segment below the last segment's end; never without a band path."""
+    if segments <= 0 or points_per_segment <= 0:
+        return False
+    if k >= segments * points_per_segment:
+        return False
+    return k % points_per_segment == 0
+
+
+def nscf_schedule(nk: int, segments: int = 0, points_per_segment: int = 0):
+    """The solves of the k-loop in order, as (target k-point, seed k-point or None).  None: the
+    k-point starts from its own initial state; a seed s: s's wavefunctions (truncated or
+    zero-padded to the target's plane-wave count) and eigenvalues are copied in first.
+
+    Forward pass (init.hpp:1056-1096): k = 0 .. nk-1, each seeded by k-1 unless k-1 is a segment
+    boundary.  Backward fix (init.hpp:1735-1800), when nk > 2: for each edge (k-point 0, and the
+    first k-point of every segment after the first), the k-points edge+5 .. edge (clipped to the
+    path) are re-solved in descending order, the first from the k-point after them (or the last
+    one), each next one from the one just solved."""
+    steps = []
+    for k in range(nk):
+        seeded = k > 0 and not at_band_path_segment_boundary(k - 1, segments, points_per_segment)
+        steps.append((k, k - 1 if seeded else None))
+    if nk <= 2:
+        return steps
+    edges = [0]
+    if segments > 0 and points_per_segment > 0:
+        for s in range(1, segments):
+            if s * points_per_segment < nk:
+                edges.append(s * points_per_segment)
+    for e in edges:
+        end = min(e + BACKWARD_SWEEP_DEPTH, nk - 1)
+        seed = min(end + 1, nk - 1)
+        for t in range(end, e - 1, -1):
+            steps.append((t, seed))
+            seed = t
+    return steps
+
+
+def propagate(psi_src: np.ndarray, npw_target: int) -> np.ndarray:
+    """propagate_wavefunctions_to_next_kpoint / _backward (init.hpp:1599-1688): the leading
+    min(npw) coefficients of every band, zeros after them."""
+    nb = psi_src.shape[1]
+    out = np.zeros((npw_target, nb), np.complex128, order="F")
+    n = min(psi_src.shape[0], npw_target)
+    out[:n] = psi_src[:n]
+    return out
+
+
+def nscf_band_path(solve, initial, npw, nb, segments=0, points_per_segment=0):
+    """Runs the k-loop over len(npw) k-points.  solve(k, psi0, stored_eigenvalues) -> (eig, psi,
+    report); initial(k) -> the k-point's starting block (npw[k] x nb).  Stored eigenvalues start
+    at zero.  Returns (eigenvalues nk x nb, reports in solve order, the schedule)."""
+    nk = len(npw)
+    psi = [None] * nk
+    eig = np.zeros((nk, nb))
+    reports = []
+    steps = nscf_schedule(nk, segments, points_per_segment)
+    for t, s in steps:
+        if s is None:
+            start = psi[t] if psi[t] is not None else initial(t)
+        else:
+            start = propagate(psi[s], npw[t])
+            eig[t] = eig[s]
+        e, p, rep = solve(t, start, eig[t].copy())
+        eig[t] = e
+        psi[t] = p
+        reports.append(rep)
+    return eig, reports, steps

@@ -1312,8 +1312,9 @@ int orc_zhegv(int n, const double* H, const double* S, double* w, double* X) {
 // (subspace_manager/serial.hpp), norm-conserving (S = identity, SV aliases V),
 // full-sphere (not gamma-only).
 //
-// psi_inout: nb x npw complex; if seed_from_psi != 0 the bands seed the basis
-// (davidson.hpp:1000-1045), else kinetic-sorted unit vectors (davidson.hpp:1047-1101).
+// psi_inout: nb x npw complex; if bit 0 of seed_from_psi is set the bands seed the basis
+// (davidson.hpp:1000-1045), else kinetic-sorted unit vectors (davidson.hpp:1047-1101); bit 1:
+// eig_out holds stored eigenvalues on entry (davidson.hpp:473-487).
 // report[0]=iterations run, report[1]=unconverged bands, report[2]=H applications.
 // ---------------------------------------------------------------------------
 namespace {
@@ -1498,7 +1499,7 @@ int orc_davidson2(const int* dims, int npw, const int* fftmap, const int* minusm
     };
 
     int basis = 0;
-    if (seed_from_psi) {  // davidson.hpp:1000-1045
+    if (seed_from_psi & 1) {  // davidson.hpp:1000-1045
         std::memcpy(d.V.data(), psi, sizeof(cplx) * static_cast<size_t>(nb) * npw);
         int acc = d.orthogonalize(0, nb);
         basis = acc;
@@ -1524,6 +1525,14 @@ int orc_davidson2(const int* dims, int npw, const int* fftmap, const int* minusm
     lap(0);
     std::vector<double> prev(nb, 0.0);
     for (int b = 0; b < nb; b++) prev[b] = d.Hs[static_cast<size_t>(b) * mmax + b].real();  // davidson.hpp:488-493
+    // bit 1 of seed_from_psi: eig_out holds the k-point state's stored eigenvalues; any nonzero
+    // one makes them the previous eigenvalues of the first test (davidson.hpp:473-487)
+    if (seed_from_psi & 2) {
+        bool stored = false;
+        for (int b = 0; b < nb; b++) stored = stored || eig_out[b] != 0.0;
+        if (stored)
+            for (int b = 0; b < nb; b++) prev[b] = eig_out[b];
+    }
     std::vector<double> w(mmax, 0.0);
     std::vector<cplx> X;
     int unconv = nb, it = 0;

@@ -21,7 +21,8 @@ _pp = ctypes.POINTER(ctypes.c_void_p)
 
 class DavidsonOptions(ctypes.Structure):
     _fields_ = [("tolerance_ry", _d), ("max_iterations", _i), ("basis_factor", _i), ("n_occupied", _i),
-                ("precision", _i), ("seed_from_psi", _i), ("final_fp64_sweep", _i)]
+                ("precision", _i), ("seed_from_psi", _i), ("final_fp64_sweep", _i),
+                ("stored_eigenvalues", _i)]
 
 
 class DavidsonReport(ctypes.Structure):
@@ -35,12 +36,14 @@ class BandRequest(ctypes.Structure):
                 ("atom_pos", _p), ("atom_species", _p), ("nspecies", _i), ("nmesh", _p), ("r_all", _p),
                 ("rab_all", _p), ("nbeta", _p), ("l_all", _p), ("cutoff_all", _p), ("beta_r_all", _p),
                 ("block_sizes", _p), ("d_blocks", _p), ("nbands", _i), ("noccupied", _i),
-                ("options", DavidsonOptions)]
+                ("options", DavidsonOptions), ("kpath_propagation", _i), ("band_path_segments", _i),
+                ("band_path_points_per_segment", _i)]
 
 
 class RequestStats(ctypes.Structure):
     _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i),
-                ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i), ("kpoints_done", _i)]
+                ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i), ("kpoints_done", _i),
+                ("solves", _i)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/q3x.h

@@ -271,6 +271,7 @@ extern "C" void q3x_davidson_default_options(q3x_davidson_options* o) {
     o->precision = Q3X_FP64;
     o->seed_from_psi = 0;
     o->final_fp64_sweep = 1;
+    o->stored_eigenvalues = 0;
 }
 
 namespace {
@@ -364,6 +365,10 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
         s.pin().rewind();
         for (int b = 0; b < nb && b < basis; ++b) prev[b] = diag[b].real();
     }
+    // stored eigenvalues of the k-point state (e.g. carried over from the previous k-point of a
+    // band path) replace the diagonal when any of them is nonzero (davidson.hpp:473-493)
+    if (opt->stored_eigenvalues && std::any_of(eig_out, eig_out + nb, [](double e) { return e != 0.0; }))
+        for (int b = 0; b < nb; ++b) prev[b] = eig_out[b];
     s.stage[0] += s.now_ms() - t0;
 
     const double tol = opt->tolerance_ry;

@@ -17,6 +17,12 @@
 // Hamiltonian, which persist between k-points).  A request's per-k-point share is released when
 // its last k-point finishes, its wavefunction share when the client collects the result; when
 // the head of the queue does not fit, the pool is emptied first.
+//
+// A k-path request (kpath_propagation 1) follows the reference's non-self-consistent k-loop
+// (iterator/kohn_sham/planewave/pseudopotential/init.hpp:1036-1106): each k-point starts from its
+// predecessor's converged wavefunctions and eigenvalues, and the edge k-points are re-solved
+// backwards after the forward pass.  Its solves depend on each other, so the request is one task:
+// one worker runs its k-loop in order while the other workers serve other requests.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -60,11 +66,50 @@ struct Request {
     int iterations = 0;
     int eig_barrier_redos = 0;
     int kpoints_done = 0;
+    int solves = 0;
+    // k-path requests (kpath_propagation 1): the k-loop's solves in order, (target, seed or -1
+    // for the k-point's own random start); one worker runs them, so the request is one task
+    bool kpath = false;
+    std::vector<std::pair<int, int>> steps;
+    bool cancel = false;  // set by cancel_pending: the k-path stops before its next solve
     double t_submit = 0, t_admit = 0, t_done = 0;
     double setup_s = 0, solve_s = 0;
 };
 
 double now_s(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
+
+// is_at_band_path_segment_boundary (iterator/kohn_sham/planewave/pseudopotential/init.hpp:1697-1715)
+bool segment_boundary(int k, int segments, int pps) {
+    if (segments <= 0 || pps <= 0 || k >= segments * pps) return false;
+    return k % pps == 0;
+}
+
+// The solves of the non-self-consistent k-loop in order (pseudopotential/init.hpp:1036-1106):
+// the forward pass, k seeded by k-1 unless k-1 is a segment boundary
+// (propagate_wavefunctions_to_next_kpoint, :1599-1645), then for nk > 2 the backward fix
+// (fix_edge_kpoints_with_backward_propagation, :1735-1800) over edge 0 and the first k-point of
+// every later segment: edge+5 .. edge in descending order, the first seeded by the k-point after
+// the sweep (or the last k-point), each next one by the one just solved.
+std::vector<std::pair<int, int>> nscf_schedule(int nk, int segments, int pps) {
+    std::vector<std::pair<int, int>> steps;
+    for (int k = 0; k < nk; ++k)
+        steps.emplace_back(k, k > 0 && !segment_boundary(k - 1, segments, pps) ? k - 1 : -1);
+    if (nk <= 2) return steps;
+    std::vector<int> edges{0};
+    if (segments > 0 && pps > 0)
+        for (int sg = 1; sg < segments; ++sg)
+            if (sg * pps < nk) edges.push_back(sg * pps);
+    constexpr int kSweepDepth = 5;  // backward_sweep_depth, :1767
+    for (int e : edges) {
+        const int end = std::min(e + kSweepDepth, nk - 1);
+        int seed = std::min(end + 1, nk - 1);
+        for (int t = end; t >= e; --t) {
+            steps.emplace_back(t, seed);
+            seed = t;
+        }
+    }
+    return steps;
+}
 
 // The host part of a k-point task, done ahead of the device part by a preparation thread.
 struct Prepared {
@@ -115,18 +160,21 @@ struct q3x_server_s {
     // a k-point takes a block from here instead of allocating (hipFree synchronises the device)
     std::multimap<size_t, void*> psi_pool;
 
-    void* take_psi(q3x_context ctx, size_t bytes) {
+    // a block of at least `bytes`; *held = its size
+    void* take_psi(q3x_context ctx, size_t bytes, size_t* held) {
         {
             std::lock_guard<std::mutex> g(mu);
             auto it = psi_pool.lower_bound(bytes);
             if (it != psi_pool.end()) {
                 void* p = it->second;
+                *held = it->first;
                 pool_bytes -= static_cast<double>(it->first);
                 psi_pool.erase(it);
                 return p;
             }
         }
         void* p = nullptr;
+        *held = bytes;
         return q3x_device_alloc(ctx, static_cast<unsigned long>(bytes), &p) == Q3X_OK ? p : nullptr;
     }
 
@@ -164,7 +212,11 @@ struct q3x_server_s {
                 });
                 if (stop) return;
                 r = running.front();
-                k = r->next_task++;
+                if (r->kpath) {  // one task; the worker prepares its k-points itself
+                    r->next_task = r->nk;
+                } else {
+                    k = r->next_task++;
+                }
                 if (r->next_task >= r->nk) running.pop_front();
                 ++preparing;
             }
@@ -190,6 +242,7 @@ struct q3x_server_s {
         auto p = std::make_unique<Prepared>();
         p->r = r;
         p->k = k;
+        if (k < 0) return p;  // a k-path request
         auto fail = [&](q3x_status st) {
             p->status = st;
             p->err = q3x_last_error();
@@ -227,7 +280,7 @@ struct q3x_server_s {
                 s = Q3X_ERR_DEVICE;
                 err = "the worker could not select device " + std::to_string(device);
             } else {
-                s = solve_kpoint(w, ctx, *p, err);
+                s = p->k < 0 ? solve_path(w, ctx, r, err) : solve_kpoint(w, ctx, *p, err);
             }
             if (p->kp) q3x_kpoint_destroy(p->kp);
             const double held = device_ok ? q3x::device_bytes_held(ctx, hams[w]) : 0.0;
@@ -235,7 +288,7 @@ struct q3x_server_s {
             worker_bytes[w] = held;
             if (s != Q3X_OK && r.status == Q3X_OK) {
                 r.status = s;
-                r.error = "k-point " + std::to_string(p->k) + ": " + err;
+                r.error = (p->k >= 0 ? "k-point " + std::to_string(p->k) + ": " : std::string()) + err;
             }
             if (--r.remaining == 0) {
                 r.done = true;
@@ -248,62 +301,144 @@ struct q3x_server_s {
         }
     }
 
-    // device part of a k-point task: bind the worker's Hamiltonian, upload, solve
+    // bind worker w's Hamiltonian to r's cell (kept while the worker's k-points come from the same
+    // request) and to kp, with kp's projector set formed on the device (form factors x structure
+    // factors; the beta(q) tables of the cell come from the shared cache)
+    q3x_status bind(int w, q3x_context ctx, Request& r, q3x_kpoint kp) {
+        q3x_hamiltonian& h = hams[w];
+        q3x_status s = Q3X_OK;
+        if (!(h && ham_ticket[w] == r.ticket)) {
+            ham_ticket[w] = 0;
+            if ((s = h ? q3x::hamiltonian_rebind(h, r.basis) : q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
+                (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK)
+                return s;
+            ham_ticket[w] = r.ticket;
+        }
+        if ((s = q3x_hamiltonian_prepare_kpoint(h, kp)) != Q3X_OK) return s;
+        return q3x_hamiltonian_build_projectors(h, r.natoms, r.atom_pos.data(), r.atom_species.data(), r.nspecies,
+                                                r.nmesh.data(), r.r_all.data(), r.rab_all.data(), r.nbeta.data(),
+                                                r.l_all.data(), r.cutoff_all.data(), r.beta_r_all.data(),
+                                                r.block_sizes.data(), r.d_blocks.data());
+    }
+
+    // k-point k's device wavefunction block, taken on its first solve
+    void* psi_block(q3x_context ctx, Request& r, int k, int npw) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (r.psi[k].second) return r.psi[k].second;
+        }
+        size_t held = 0;
+        void* psi = take_psi(ctx, static_cast<size_t>(npw) * r.nb * 16, &held);
+        if (!psi) return nullptr;
+        std::lock_guard<std::mutex> g(mu);
+        r.psi[k] = {ctx, psi};
+        r.psi_bytes[k] = held;
+        return psi;
+    }
+
+    // Davidson on k-point k in place on its block (seeded by the block); eigenvalues into row k
+    q3x_status davidson(int w, Request& r, int k, void* psi, bool stored, clk::time_point ts, double host_s,
+                        bool first_solve) {
+        q3x_davidson_options o = r.opt;
+        o.seed_from_psi = 1;
+        o.stored_eigenvalues = stored ? 1 : 0;
+        q3x_davidson_report rep{};
+        const clk::time_point tsolve = clk::now();
+        const q3x_status s = q3x_davidson_solve_device(hams[w], r.nb, &o, psi, &r.eig[static_cast<size_t>(k) * r.nb], &rep);
+        if (s != Q3X_OK) return s;
+        const clk::time_point tend = clk::now();
+        std::lock_guard<std::mutex> g(mu);
+        r.iterations += rep.iterations;
+        r.eig_barrier_redos += rep.eig_barrier_redos;
+        r.kpoints_done += first_solve ? 1 : 0;
+        ++r.solves;
+        r.setup_s += host_s + std::chrono::duration<double>(tsolve - ts).count();
+        r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
+        return Q3X_OK;
+    }
+
+    // device part of an independent k-point task: bind, random start, solve
     q3x_status solve_kpoint(int w, q3x_context ctx, Prepared& p, std::string& err) {
         const clk::time_point ts = clk::now();
         Request& r = *p.r;
-        const int k = p.k, npw = p.npw;
-        q3x_hamiltonian& h = hams[w];
         auto fail = [&](q3x_status st) {
             err = q3x_last_error();
             return st;
         };
-        q3x_status s = Q3X_OK;
-        // a worker's next k-point usually belongs to the same request: cell, tables and V stay
-        const bool same_cell = h && ham_ticket[w] == r.ticket;
-        if (!same_cell) {
-            ham_ticket[w] = 0;
-            if ((s = h ? q3x::hamiltonian_rebind(h, r.basis) : q3x_hamiltonian_create(ctx, r.basis, &h)) != Q3X_OK ||
-                (s = q3x_hamiltonian_set_potential(h, r.v.data(), static_cast<long>(r.v.size()))) != Q3X_OK)
-                return fail(s);
-            ham_ticket[w] = r.ticket;
-        }
-        // projector set on the device (form factors x structure factors; the beta(q) tables of the
-        // cell come from the shared cache)
-        if ((s = q3x_hamiltonian_prepare_kpoint(h, p.kp)) != Q3X_OK ||
-            (s = q3x_hamiltonian_build_projectors(h, r.natoms, r.atom_pos.data(), r.atom_species.data(), r.nspecies,
-                                                  r.nmesh.data(), r.r_all.data(), r.rab_all.data(), r.nbeta.data(),
-                                                  r.l_all.data(), r.cutoff_all.data(), r.beta_r_all.data(),
-                                                  r.block_sizes.data(), r.d_blocks.data())) != Q3X_OK)
-            return fail(s);
-        const size_t psi_bytes = static_cast<size_t>(npw) * r.nb * 16;
-        void* psi = take_psi(ctx, psi_bytes);
+        q3x_status s = bind(w, ctx, r, p.kp);
+        if (s != Q3X_OK) return fail(s);
+        void* psi = psi_block(ctx, r, p.k, p.npw);
         if (!psi) return fail(Q3X_ERR_DEVICE);
-        {
-            std::lock_guard<std::mutex> g(mu);
-            r.psi[k] = {ctx, psi};
-            r.psi_bytes[k] = psi_bytes;
-        }
         // starting block: q3x_kpoint_random_psi with the k-point's index as the seed, formed on the
         // device, so a request's result does not depend on what else runs (plane-wave unit vectors,
         // the solver's own seed, can miss symmetry sectors of these undisplaced crystals)
-        if ((s = q3x_hamiltonian_random_psi(h, r.nb, static_cast<unsigned long long>(k), psi)) != Q3X_OK) return fail(s);
-        q3x_davidson_options o = r.opt;
-        o.seed_from_psi = 1;
-        q3x_davidson_report rep{};
-        const clk::time_point tsolve = clk::now();
-        s = q3x_davidson_solve_device(h, r.nb, &o, psi, &r.eig[static_cast<size_t>(k) * r.nb], &rep);
-        if (s != Q3X_OK) return fail(s);
-        const clk::time_point tend = clk::now();
+        if ((s = q3x_hamiltonian_random_psi(hams[w], r.nb, static_cast<unsigned long long>(p.k), psi)) != Q3X_OK)
+            return fail(s);
+        if ((s = davidson(w, r, p.k, psi, false, ts, p.host_s, true)) != Q3X_OK) return fail(s);
+        return Q3X_OK;
+    }
+
+    // a k-path request: its k-loop's solves in order on this worker (nscf_schedule).  A seeded
+    // solve starts from the seed's eigenvectors, the leading min(npw) coefficients of each band
+    // with zeros after them, and the seed's eigenvalues as the stored ones
+    // (pseudopotential/init.hpp:1599-1688); an unseeded one from the k-point's random start.
+    q3x_status solve_path(int w, q3x_context ctx, Request& r, std::string& err) {
+        std::vector<q3x_kpoint> kps(r.nk, nullptr);
+        std::vector<int> npw(r.nk, 0);
+        std::vector<char> solved(r.nk, 0);
+        q3x_status s = Q3X_OK;
+        for (const auto& step : r.steps) {
+            {
+                std::lock_guard<std::mutex> g(mu);
+                if (r.cancel) break;
+            }
+            const int t = step.first, seed = step.second;
+            const clk::time_point ts = clk::now();
+            double host_s = 0.0;
+            if (!kps[t]) {
+                if ((s = q3x_kpoint_create(r.basis, &r.kpts[3 * static_cast<size_t>(t)], &kps[t], &npw[t])) != Q3X_OK)
+                    break;
+                host_s = std::chrono::duration<double>(clk::now() - ts).count();
+            }
+            if ((s = bind(w, ctx, r, kps[t])) != Q3X_OK) break;
+            void* psi = psi_block(ctx, r, t, npw[t]);
+            if (!psi) {
+                s = Q3X_ERR_DEVICE;
+                break;
+            }
+            double* eig_t = &r.eig[static_cast<size_t>(t) * r.nb];
+            if (seed < 0) {
+                if ((s = q3x_hamiltonian_random_psi(hams[w], r.nb, static_cast<unsigned long long>(t), psi)) != Q3X_OK)
+                    break;
+            } else if (seed != t) {
+                const size_t ns = static_cast<size_t>(npw[seed]), nt = static_cast<size_t>(npw[t]);
+                const size_t nc = std::min(ns, nt);
+                hipStream_t st = ctx->stream;
+                if (!q3x::hip_ok(hipMemcpy2DAsync(psi, nt * 16, r.psi[seed].second, ns * 16, nc * 16, r.nb,
+                                             hipMemcpyDeviceToDevice, st),
+                            "propagating wavefunctions") ||
+                    (nt > nc && !q3x::hip_ok(hipMemset2DAsync(static_cast<char*>(psi) + nc * 16, nt * 16, 0, (nt - nc) * 16,
+                                                         r.nb, st),
+                                        "zero-padding wavefunctions"))) {
+                    s = Q3X_ERR_DEVICE;
+                    break;
+                }
+                std::copy(&r.eig[static_cast<size_t>(seed) * r.nb], &r.eig[static_cast<size_t>(seed + 1) * r.nb], eig_t);
+            }
+            if ((s = davidson(w, r, t, psi, true, ts, host_s, !solved[t])) != Q3X_OK) break;
+            solved[t] = 1;
+        }
+        if (s != Q3X_OK) err = q3x_last_error();
+        for (q3x_kpoint kp : kps)
+            if (kp) q3x_kpoint_destroy(kp);
         {
             std::lock_guard<std::mutex> g(mu);
-            r.iterations += rep.iterations;
-            r.eig_barrier_redos += rep.eig_barrier_redos;
-            ++r.kpoints_done;
-            r.setup_s += p.host_s + std::chrono::duration<double>(tsolve - ts).count();
-            r.solve_s += std::chrono::duration<double>(tend - tsolve).count();
+            if (s == Q3X_OK && r.cancel) {
+                s = Q3X_ERR_STATE;
+                err = "cancelled before all k-points were solved";
+            }
         }
-        return Q3X_OK;
+        return s;
     }
 };
 
@@ -438,7 +573,7 @@ q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled) {
         s->waiting.clear();
         // admitted, k-points not yet handed to a preparation thread
         for (auto& r : s->running) {
-            r->remaining -= r->nk - r->next_task;
+            r->remaining -= r->kpath ? (r->next_task < r->nk ? 1 : 0) : r->nk - r->next_task;
             r->next_task = r->nk;
             finish(*r);
         }
@@ -450,6 +585,14 @@ q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled) {
             finish(*p->r);
         }
         s->ready.clear();
+        // k-path requests on a worker (or on their way to one) stop before their next solve
+        for (auto& kv : s->all) {
+            Request& r = *kv.second;
+            if (r.kpath && r.admitted && !r.done) {
+                r.cancel = true;
+                finish(r);
+            }
+        }
         if (cancelled) *cancelled = n;
         s->admit_locked();
         s->cv_done.notify_all();
@@ -478,9 +621,10 @@ q3x_status estimate(q3x_server s, const q3x_band_request* q, double* bytes, doub
         q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
         int nproj = 0;
         for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
+        // a k-path request runs on one worker at a time
+        const int concurrent = q->kpath_propagation ? 1 : static_cast<int>(s->workers.size());
         *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2],
-                           static_cast<long>(wave[0]) * wave[1] * wave[2],
-                           nproj, static_cast<int>(s->workers.size()), psi_share);
+                           static_cast<long>(wave[0]) * wave[1] * wave[2], nproj, concurrent, psi_share);
         return Q3X_OK;
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_server_estimate: ") + e.what());
@@ -495,7 +639,9 @@ q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* 
 
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* ticket) {
     try {
-        if (!s || !q || !q->basis || !q->v_r || !q->kpoints || q->nk < 1 || q->nbands < 1 || !ticket) {
+        if (!s || !q || !q->basis || !q->v_r || !q->kpoints || q->nk < 1 || q->nbands < 1 || !ticket ||
+            q->kpath_propagation < 0 || q->kpath_propagation > 1 || q->band_path_segments < 0 ||
+            q->band_path_points_per_segment < 0) {
             set_error("q3x_server_submit: invalid argument");
             return Q3X_ERR_ARGUMENT;
         }
@@ -531,7 +677,9 @@ q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* tick
         r->nocc = q->noccupied;
         r->opt = q->options;
         r->opt.n_occupied = q->noccupied;
-        r->remaining = q->nk;
+        r->kpath = q->kpath_propagation == 1;
+        if (r->kpath) r->steps = nscf_schedule(q->nk, q->band_path_segments, q->band_path_points_per_segment);
+        r->remaining = r->kpath ? 1 : q->nk;
         r->eig.assign(static_cast<size_t>(q->nk) * q->nbands, 0.0);
         r->psi.assign(q->nk, {nullptr, nullptr});
         r->psi_bytes.assign(q->nk, 0);
@@ -584,6 +732,7 @@ q3x_status q3x_server_wait(q3x_server s, long ticket, double* eig_out, q3x_reque
             stats->solve_s = r->solve_s;
             stats->eig_barrier_redos = r->eig_barrier_redos;
             stats->kpoints_done = r->kpoints_done;
+            stats->solves = r->solves;
         }
         {
             // the request's wavefunction blocks go back to the pool, and its share of the

@@ -353,8 +353,10 @@ class PlanewaveDavidsonDiagonalizer:
         return self.report.unconverged
 
     def solve(self, number_of_bands: int, number_of_occupied: int, psi0: np.ndarray | None = None,
-              precision: int = FP64, final_fp64_sweep: bool = True):
-        """Returns (eigenvalues [Ry], psi (npw x nb))."""
+              precision: int = FP64, final_fp64_sweep: bool = True, stored_eigenvalues=None):
+        """Returns (eigenvalues [Ry], psi (npw x nb)).  stored_eigenvalues: the k-point state's
+        stored eigenvalues (nb, Ry), the previous eigenvalues of the first convergence test when
+        any is nonzero (davidson.hpp:473-493)."""
         kp = self.applicator.kpoint
         if kp is None:
             raise _lib.StateError(3, "prepare_kpoint() must be called before solve")
@@ -369,6 +371,9 @@ class PlanewaveDavidsonDiagonalizer:
             psi = np.asfortranarray(psi0, dtype=np.complex128).copy(order="F")
             self.options.seed_from_psi = 1
         eig = np.zeros(nb)
+        self.options.stored_eigenvalues = int(stored_eigenvalues is not None)
+        if stored_eigenvalues is not None:
+            eig[:] = np.asarray(stored_eigenvalues, dtype=np.float64).reshape(nb)
         self.report = _lib.DavidsonReport()
         check(lib().q3x_davidson_solve(self.applicator.h, nb, ctypes.byref(self.options), _ptr(psi), _ptr(eig),
                                        ctypes.byref(self.report)))
@@ -462,7 +467,10 @@ def _species_arrays(species):
 
 @dataclass
 class BandRequestSpec:
-    """One band-structure request: a cell (basis), its V(r), the k-points and the band count."""
+    """One band-structure request: a cell (basis), its V(r), the k-points and the band count.
+    kpath_propagation: 0 independent k-points; 1 the reference's non-self-consistent k-loop
+    (forward propagation along the path, segment boundaries, the backward fix), with
+    band_path_segments segments of band_path_points_per_segment points (0: no band path)."""
     basis: PlanewaveBasis
     v_r: np.ndarray
     kpoints: np.ndarray
@@ -474,6 +482,9 @@ class BandRequestSpec:
     tolerance_ry: float = 2e-6
     max_iterations: int = 60
     precision: int = FP64
+    kpath_propagation: int = 0
+    band_path_segments: int = 0
+    band_path_points_per_segment: int = 0
 
 
 class BandStructureServer:
@@ -519,6 +530,9 @@ class BandStructureServer:
         r.options.tolerance_ry = spec.tolerance_ry
         r.options.max_iterations = spec.max_iterations
         r.options.precision = spec.precision
+        r.kpath_propagation = spec.kpath_propagation
+        r.band_path_segments = spec.band_path_segments
+        r.band_path_points_per_segment = spec.band_path_points_per_segment
         return r, arrays
 
     def estimate(self, spec: BandRequestSpec) -> float:
@@ -554,7 +568,7 @@ class BandStructureServer:
         return status, eig, {"submit_s": st.submit_s, "admit_s": st.admit_s, "done_s": st.done_s,
                              "footprint_bytes": st.footprint_bytes, "iterations": st.iterations,
                              "setup_s": st.setup_s, "solve_s": st.solve_s, "eig_barrier_redos": st.eig_barrier_redos,
-                             "kpoints_done": st.kpoints_done}
+                             "kpoints_done": st.kpoints_done, "solves": st.solves}
 
     def in_flight_bytes(self) -> float:
         return lib().q3x_server_in_flight_bytes(self.h)

@@ -6,6 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from oracle import oracle as O
 from q3_amd import workloads as W
 from q3_amd.planewave import KPoint, PlanewaveBasis
 
@@ -85,3 +86,44 @@ def sine_h_closed_form(miller, fft_map, dims, m0, ekin, psi):
     out[lo >= 0] += -0.5j * psi[lo[lo >= 0]]
     out[hi >= 0] += 0.5j * psi[hi[hi >= 0]]
     return out
+
+
+# ---- a small band path for the k-loop tests (every part from the oracle)
+
+class OracleBasis:
+    def __init__(self, ob, crystal):
+        self.ob = ob
+        self.wave_dims = ob.wave_dims
+        self.gamma_only = False
+        _r, self.cell_volume, _t = O.crystal(crystal.lattice, crystal.alat)
+
+    def g_vectors(self):
+        return self.ob.gvectors()
+
+
+def small_path_problem(nk=9, ecut_ry=16.0):
+    """The first 2-atom request of the serving stream at a low cutoff, on nk points of its line,
+    every part from the oracle."""
+    r = next(q for q in W.cfg5_requests(80, seed=4) if len(q.crystal.species) == 2)
+    c = r.crystal
+    sp = W.serving_species()
+    ob = O.Basis(c.lattice, c.alat, ecut_ry, 0.0, True, (1, 1, 1))
+    V = W.local_potential(c, OracleBasis(ob, c), sp)
+    bs, db = O.d_blocks(c.species, sp)
+    ks = r.kpoints[0][None, :] + np.linspace(0.0, 1.0, nk)[:, None] * (r.kpoints[-1] - r.kpoints[0])[None, :]
+    per_k = []
+    for k in ks:
+        gi, ek, fm = ob.kpoint(k)
+        per_k.append((ek, fm, ob.kb_projectors(k, gi, c.positions, c.species, sp)))
+    return r, ob, V, bs, db, ks, per_k
+
+
+def oracle_solver(ob, V, bs, db, per_k, nb, tol):
+    def solve(k, psi0, stored):
+        ek, fm, beta = per_k[k]
+        return O.davidson(ob.wave_dims, fm, ek, V, beta, bs, db, nb, nb, psi0=psi0, tol=tol, max_iter=400,
+                          stored_eigenvalues=stored)
+
+    def initial(k):
+        return W.random_psi(per_k[k][0], nb, seed=k)
+    return solve, initial
