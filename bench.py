@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--stream-run", type=float, default=40.0,
                    help="full-distribution serving: seconds after the first arrival at which unstarted work is "
                         "cancelled")
+    p.add_argument("--stream-kpath", type=int, choices=[0, 1], default=1,
+                   help="1: each stream request is a band path (one straight segment) solved with the "
+                        "non-self-consistent k-loop (propagation + backward fix); 0: independent k-points")
     p.add_argument("--davidson-iters", type=int, default=60)
     p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "copy"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
@@ -341,7 +344,7 @@ def batched_gamma(args, tolerances=None):
 
 def serving_stream(args):
     """The cfg5 stream as stated (Poisson 20 req/s, seed 4, Si/C cells of 2 to 216 atoms, 15-25 Ha,
-    8-40 k-points, bands 2 N + 8), over a bounded arrival window: the requests arriving in the
+    8-40 k-points on a straight path, bands 2 N + 8), over a bounded arrival window: the requests arriving in the
     first --stream-window seconds are submitted at their arrival times; --stream-run seconds after
     the first arrival the work not yet started is cancelled.  Reported on the server's clock:
     requests completed by then (sustained req/s against the 20 offered), their p50/p99 latency,
@@ -362,8 +365,12 @@ def serving_stream(args):
             basis = PlanewaveBasis(r.crystal, r.ecutwfc_ry, 0.0, True, (1, 1, 1))
             V = W.local_potential(r.crystal, basis, sp)
             bs, db = W.d_blocks_for(r.crystal, sp)
+            # a straight path is one band-path segment of nk - 1 steps
             specs.append(BandRequestSpec(basis, V, r.kpoints, sp, bs, db, r.nbands, r.noccupied,
-                                         tolerance_ry=2e-6, max_iterations=args.davidson_iters, precision=FP64))
+                                         tolerance_ry=2e-6, max_iterations=args.davidson_iters, precision=FP64,
+                                         kpath_propagation=args.stream_kpath,
+                                         band_path_segments=1 if args.stream_kpath else 0,
+                                         band_path_points_per_segment=len(r.kpoints) - 1 if args.stream_kpath else 0))
         log(f"stream setup {time.time() - t0:.1f}s: {len(specs)} requests in the first {args.stream_window:g} s")
         srv = BandStructureServer(workers=args.batch_workers)
         results = {}
@@ -399,6 +406,7 @@ def serving_stream(args):
         by_window = sum(1 for t in fin if t <= window_end)
         # every k-point solved by the end of the run, of completed and of cancelled requests
         kdone = sum(results[i][1]["kpoints_done"] for i in range(len(reqs)))
+        solves = sum(results[i][1]["solves"] for i in range(len(reqs)))
         k_end = max((results[i][1]["done_s"] - sub0 for i in range(len(reqs)) if results[i][1]["kpoints_done"]),
                     default=1.0)
         nat = [len(r.crystal.species) for r in reqs]
@@ -407,7 +415,8 @@ def serving_stream(args):
                "kpoints_offered": int(sum(len(r.kpoints) for r in reqs)),
                "completed_by_run_end": by_run, "sustained_req_per_s": by_run / args.stream_run,
                "backlog_at_window_end": len(reqs) - by_window, "backlog_at_run_end": len(reqs) - by_run,
-               "cancelled_requests": cancelled, "kpoint_solves": int(kdone),
+               "cancelled_requests": cancelled, "kpoint_solves": int(kdone), "davidson_solves": int(solves),
+               "kpath_propagation": args.stream_kpath,
                "kpoint_solves_per_s": kdone / max(k_end, 1e-9),
                "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
                "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,

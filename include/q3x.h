@@ -308,6 +308,44 @@ int q3x_batch_size(q3x_batch b);
 q3x_status q3x_batch_solve(q3x_batch b, int nb, const q3x_davidson_options* opt,
                            double* const* psi_inout, double* eig_out, q3x_davidson_report* reports);
 
+/* ---- footprint estimate (PlanewaveMemoryEstimate::compute,
+ *      src/layer_4_computation/iterator/kohn_sham/planewave/internal/memory_estimate.hpp:37-215):
+ *      the reference's inputs, terms (label, formula, bytes), integer arithmetic and totals.
+ *      Host-only. */
+typedef struct {
+    int number_of_bands;
+    int max_plane_waves_per_kpoint;
+    int number_of_g_vectors;
+    int number_of_projectors;
+    int number_of_atomic_orbitals;
+    int number_of_atoms;
+    int number_of_spin_components;
+    int local_number_of_kpoint_states;
+    int mixing_history_depth;
+    long long fft_rho_grid_points;
+    long long fft_wave_grid_points;
+    int davidson_basis_expansion_factor;
+    int overlap_is_identity;       /* false for ultrasoft and PAW (S stored) */
+    int diagonalizer_is_davidson;  /* false for the direct solver */
+} q3x_memory_estimate_inputs;
+typedef struct {
+    const char* label;    /* static strings */
+    const char* formula;
+    unsigned long long bytes;
+} q3x_memory_term;
+typedef struct {
+    q3x_memory_term steady_state[16];
+    int n_steady_state;
+    q3x_memory_term transient[4];
+    int n_transient;
+    unsigned long long steady_state_bytes, solver_bytes, setup_peak_bytes, peak_bytes;
+    int maximum_basis_size;
+} q3x_memory_estimate;
+/* the reference's defaults (PlanewaveMemoryEstimateInputs{}): zeros, one spin, one k-point
+ * state, expansion factor 2, S not the identity, the Davidson solver */
+void q3x_memory_estimate_default_inputs(q3x_memory_estimate_inputs* in);
+q3x_status q3x_memory_estimate_compute(const q3x_memory_estimate_inputs* in, q3x_memory_estimate* out);
+
 /* ---- band-structure request server (continuous batching).  No counterpart in the reference,
  *      which runs one calculation per process; it serves non-self-consistent band solves of the
  *      reference's kind (flavors/non_self_consistent_field.hpp:78-112) for many cells at once.
@@ -379,6 +417,21 @@ q3x_status q3x_server_destroy(q3x_server s);
  * counterpart in the reference (it serves one calculation per process). */
 q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled);
 q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* req, double* bytes);
+/* The admission footprint of a request, itemised; host-only (needs no server or device).
+ * reference_out: PlanewaveMemoryEstimate::compute of the request's dimensions (bands, the first
+ * k-point's plane waves plus 5% and 16, G vectors, projectors, atoms, nk wavefunction sets, the
+ * boxes, the expansion factor; no atomic orbitals, no mixing history, S = identity).  The
+ * footprint counts the reference terms this path holds in HBM -- "wavefunctions" once, and
+ * "Davidson basis V, HV", "rotation scratch", "subspace matrices", "nonlocal projectors" and
+ * "block-diagonal D matrix" per concurrently running k-point -- plus device_terms, the buffers
+ * the reference's estimate has no line for, also per concurrently running k-point; the
+ * reference's host-side lines (G-vector table, FFT buffers, density and potential, Broyden
+ * history) and its atomic-orbital start are not on the device.
+ * concurrent_kpoints: 1 for a k-path request, else the server's worker count (the estimate
+ * q3x_server_estimate uses).  device_terms holds at most 8 entries. */
+q3x_status q3x_band_request_footprint(const q3x_band_request* req, int concurrent_kpoints,
+                                      q3x_memory_estimate* reference_out, q3x_memory_term* device_terms,
+                                      int* n_device_terms, double* bytes);
 /* copies the request's arrays and returns at once with a ticket */
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* req, long* ticket);
 /* blocks until the request is done: eig_out nk x nb (Ry, row k = k-point k); frees its state */

@@ -23,6 +23,8 @@ _p = ctypes.c_void_p
 
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    # the reference's own footprint estimate, compiled from its tree when that is present
+    subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
     return _LIB_PATH
 
 

@@ -40,6 +40,26 @@ class BandRequest(ctypes.Structure):
                 ("band_path_points_per_segment", _i)]
 
 
+class MemoryEstimateInputs(ctypes.Structure):
+    _fields_ = [("number_of_bands", _i), ("max_plane_waves_per_kpoint", _i), ("number_of_g_vectors", _i),
+                ("number_of_projectors", _i), ("number_of_atomic_orbitals", _i), ("number_of_atoms", _i),
+                ("number_of_spin_components", _i), ("local_number_of_kpoint_states", _i),
+                ("mixing_history_depth", _i), ("fft_rho_grid_points", ctypes.c_longlong),
+                ("fft_wave_grid_points", ctypes.c_longlong), ("davidson_basis_expansion_factor", _i),
+                ("overlap_is_identity", _i), ("diagonalizer_is_davidson", _i)]
+
+
+class MemoryTerm(ctypes.Structure):
+    _fields_ = [("label", ctypes.c_char_p), ("formula", ctypes.c_char_p), ("bytes", ctypes.c_ulonglong)]
+
+
+class MemoryEstimate(ctypes.Structure):
+    _fields_ = [("steady_state", MemoryTerm * 16), ("n_steady_state", _i), ("transient", MemoryTerm * 4),
+                ("n_transient", _i), ("steady_state_bytes", ctypes.c_ulonglong),
+                ("solver_bytes", ctypes.c_ulonglong), ("setup_peak_bytes", ctypes.c_ulonglong),
+                ("peak_bytes", ctypes.c_ulonglong), ("maximum_basis_size", _i)]
+
+
 class RequestStats(ctypes.Structure):
     _fields_ = [("submit_s", _d), ("admit_s", _d), ("done_s", _d), ("footprint_bytes", _d), ("iterations", _i),
                 ("setup_s", _d), ("solve_s", _d), ("eig_barrier_redos", _i), ("kpoints_done", _i),
@@ -99,6 +119,10 @@ SIGNATURES = {
     "q3x_server_destroy": (_i, [_p]),
     "q3x_server_cancel_pending": (_i, [_p, _p]),
     "q3x_server_estimate": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_d)]),
+    "q3x_memory_estimate_default_inputs": (None, [ctypes.POINTER(MemoryEstimateInputs)]),
+    "q3x_memory_estimate_compute": (_i, [ctypes.POINTER(MemoryEstimateInputs), ctypes.POINTER(MemoryEstimate)]),
+    "q3x_band_request_footprint": (_i, [ctypes.POINTER(BandRequest), _i, ctypes.POINTER(MemoryEstimate),
+                                        ctypes.POINTER(MemoryTerm), _p, ctypes.POINTER(_d)]),
     "q3x_server_submit": (_i, [_p, ctypes.POINTER(BandRequest), ctypes.POINTER(_l)]),
     "q3x_server_wait": (_i, [_p, _l, _p, ctypes.POINTER(RequestStats)]),
     "q3x_server_in_flight_bytes": (_d, [_p]),

@@ -30,7 +30,8 @@ EXTRA = {
     "davidson.cpp": ["-fopenmp"],
 }
 SOURCES = ["setup.cpp", "kb_projectors.cpp", "api.cpp", "hlocal.hip", "plane_tl.hip", "zgemm.hip", "aux_kernels.hip",
-           "subspace_eig.hip", "potential.hip", "noalloc.hip", "gamma_kernels.hip", "kb_device.hip", "dense.cpp", "davidson.cpp", "batch.cpp", "server.cpp"]
+           "subspace_eig.hip", "potential.hip", "noalloc.hip", "gamma_kernels.hip", "kb_device.hip", "dense.cpp", "davidson.cpp", "batch.cpp", "server.cpp",
+           "memory_estimate.cpp"]
 
 
 def _hipcc():

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -451,21 +452,71 @@ std::vector<T> copy_of(const T* p, size_t n) {
     return p ? std::vector<T>(p, p + n) : std::vector<T>(n, T{});
 }
 
-// Footprint of one request in bytes: its wavefunctions (kept for all k-points) plus, for each
-// k-point that may run at once, the Davidson basis V and HV twice (the rotation ping-pongs
-// between two copies), the orthogonalisation buffer, the subspace
-// matrices and eigensolver workspace, the projectors (complex128 + complex64), the z-stick
-// buffer and the potential -- the terms of PlanewaveMemoryEstimate::compute
-// (memory_estimate.hpp:75-215) for this path.
-double footprint(const q3x_band_request* q, int npw, int n3, long npoints, int nproj, int workers,
-                 double* psi_share = nullptr) {
-    const double c = 16.0, nb = q->nbands, m = std::max(std::min(2.0 * nb, static_cast<double>(npw)), nb);
-    const double psi = static_cast<double>(q->nk) * nb * npw * c;
-    if (psi_share) *psi_share = psi;
-    const double sticks = std::max(1.0, static_cast<double>(npw) / std::max(1, n3)) * 1.6;  // z-sticks
-    const double per_k = 4 * m * npw * c + nb * npw * c + 30.0 * m * m * c + 1.5 * nproj * npw * c +
-                         nb * n3 * sticks * c + 12.0 * npoints;
-    return psi + per_k * std::min(q->nk, workers);
+// Admission footprint of a request (q3x_band_request_footprint): PlanewaveMemoryEstimate::compute
+// (memory_estimate.hpp:75-215) of the request's dimensions, of which the lines this path holds in
+// HBM count -- the wavefunctions of every k-point once, the solver's lines, the projectors and the
+// D blocks per concurrently running k-point -- plus the device buffers that estimate has no line
+// for, also per running k-point.
+struct Footprint {
+    q3x_memory_estimate ref{};
+    q3x_memory_term dev[8]{};
+    int ndev = 0;
+    double psi = 0.0, per_kpoint = 0.0, total = 0.0;
+};
+
+q3x_status request_footprint(const q3x_band_request* q, int concurrent, Footprint& f) {
+    if (!q || !q->basis || !q->kpoints || q->nk < 1 || q->nbands < 1 || concurrent < 1 || q->natoms < 0 ||
+        (q->natoms > 0 && !q->block_sizes)) {
+        set_error("q3x_band_request_footprint: invalid argument");
+        return Q3X_ERR_ARGUMENT;
+    }
+    q3x_kpoint kp = nullptr;
+    int npw = 0;
+    const q3x_status st = q3x_kpoint_create(q->basis, q->kpoints, &kp, &npw);
+    if (st != Q3X_OK) return st;
+    q3x_kpoint_destroy(kp);
+    int rho[3], wave[3], ng = 0;
+    double vol = 0, tpa2 = 0;
+    q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
+    int nproj = 0;
+    for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
+    // the first k-point's plane waves stand for the path's; the others may hold a few more
+    const int npw_est = static_cast<int>(npw * 1.05) + 16;
+    q3x_memory_estimate_inputs in;
+    q3x_memory_estimate_default_inputs(&in);
+    in.number_of_bands = q->nbands;
+    in.max_plane_waves_per_kpoint = npw_est;
+    in.number_of_g_vectors = ng;
+    in.number_of_projectors = nproj;
+    in.number_of_atoms = q->natoms;
+    in.local_number_of_kpoint_states = q->nk;
+    in.fft_rho_grid_points = static_cast<long long>(rho[0]) * rho[1] * rho[2];
+    in.fft_wave_grid_points = static_cast<long long>(wave[0]) * wave[1] * wave[2];
+    in.davidson_basis_expansion_factor = q->options.basis_factor >= 1 ? q->options.basis_factor : 2;
+    in.overlap_is_identity = 1;
+    q3x_memory_estimate_compute(&in, &f.ref);
+    static const char* const held[] = {"Davidson basis V, HV", "rotation scratch", "subspace matrices",
+                                       "nonlocal projectors", "block-diagonal D matrix"};
+    for (int i = 0; i < f.ref.n_steady_state; ++i) {
+        const q3x_memory_term& t = f.ref.steady_state[i];
+        if (std::strcmp(t.label, "wavefunctions") == 0) f.psi += static_cast<double>(t.bytes);
+        for (const char* h : held)
+            if (std::strcmp(t.label, h) == 0) f.per_kpoint += static_cast<double>(t.bytes);
+    }
+    const double m = f.ref.maximum_basis_size, n = npw_est, nb = q->nbands, n3 = wave[2];
+    const double sticks = std::max(1.0, n / std::max(1.0, n3)) * 1.6;  // occupied z-sticks, with margin
+    auto dev = [&](const char* label, const char* formula, double bytes) {
+        const unsigned long long b = static_cast<unsigned long long>(std::ceil(bytes));
+        f.dev[f.ndev++] = {label, formula, b};
+        f.per_kpoint += static_cast<double>(b);
+    };
+    dev("second Davidson basis V, HV", "2 x M x npw x 16", 2.0 * m * n * 16.0);
+    dev("subspace eigensolver and Gram workspace", "26 x M^2 x 16", 26.0 * m * m * 16.0);
+    dev("nonlocal projectors, complex64 copy", "nkb x npw x 8", static_cast<double>(nproj) * n * 8.0);
+    dev("z-stick buffer", "nb x N3 x 1.6 max(1, npw / N3) x 16", nb * n3 * sticks * 16.0);
+    dev("potential, fp64 and fp32", "Nwave x 12", 12.0 * static_cast<double>(in.fft_wave_grid_points));
+    f.total = f.psi + f.per_kpoint * std::min(q->nk, concurrent);
+    return Q3X_OK;
 }
 
 }  // namespace
@@ -607,24 +658,17 @@ q3x_status q3x_server_cancel_pending(q3x_server s, int* cancelled) {
 namespace {
 q3x_status estimate(q3x_server s, const q3x_band_request* q, double* bytes, double* psi_share) {
     try {
-        if (!s || !q || !q->basis || !bytes || q->nk < 1) {
+        if (!s || !q || !bytes) {
             set_error("q3x_server_estimate: invalid argument");
             return Q3X_ERR_ARGUMENT;
         }
-        q3x_kpoint kp = nullptr;
-        int npw = 0;
-        const q3x_status st = q3x_kpoint_create(q->basis, q->kpoints, &kp, &npw);
-        if (st != Q3X_OK) return st;
-        q3x_kpoint_destroy(kp);
-        int rho[3], wave[3], ng = 0;
-        double vol = 0, tpa2 = 0;
-        q3x_basis_info(q->basis, rho, wave, &ng, &vol, &tpa2);
-        int nproj = 0;
-        for (int a = 0; a < q->natoms; ++a) nproj += q->block_sizes[a];
         // a k-path request runs on one worker at a time
         const int concurrent = q->kpath_propagation ? 1 : static_cast<int>(s->workers.size());
-        *bytes = footprint(q, static_cast<int>(npw * 1.05) + 16, wave[2],
-                           static_cast<long>(wave[0]) * wave[1] * wave[2], nproj, concurrent, psi_share);
+        Footprint f;
+        const q3x_status st = request_footprint(q, concurrent, f);
+        if (st != Q3X_OK) return st;
+        *bytes = f.total;
+        if (psi_share) *psi_share = f.psi;
         return Q3X_OK;
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_server_estimate: ") + e.what());
@@ -635,6 +679,25 @@ q3x_status estimate(q3x_server s, const q3x_band_request* q, double* bytes, doub
 
 q3x_status q3x_server_estimate(q3x_server s, const q3x_band_request* q, double* bytes) {
     return estimate(s, q, bytes, nullptr);
+}
+
+q3x_status q3x_band_request_footprint(const q3x_band_request* q, int concurrent_kpoints,
+                                      q3x_memory_estimate* reference_out, q3x_memory_term* device_terms,
+                                      int* n_device_terms, double* bytes) {
+    try {
+        Footprint f;
+        const q3x_status st = request_footprint(q, concurrent_kpoints, f);
+        if (st != Q3X_OK) return st;
+        if (reference_out) *reference_out = f.ref;
+        if (device_terms)
+            for (int i = 0; i < f.ndev; ++i) device_terms[i] = f.dev[i];
+        if (n_device_terms) *n_device_terms = f.ndev;
+        if (bytes) *bytes = f.total;
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_band_request_footprint: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
 }
 
 q3x_status q3x_server_submit(q3x_server s, const q3x_band_request* q, long* ticket) {

@@ -487,6 +487,44 @@ class BandRequestSpec:
     band_path_points_per_segment: int = 0
 
 
+def _band_request(spec: BandRequestSpec):
+    """The q3x_band_request of a spec and the arrays it points into (keep them alive)."""
+    c = spec.basis.crystal
+    arrays = {
+        "v": np.ascontiguousarray(spec.v_r, dtype=np.float64).ravel(),
+        "k": np.ascontiguousarray(spec.kpoints, dtype=np.float64).reshape(-1, 3),
+        "pos": np.ascontiguousarray(c.positions, dtype=np.float64),
+        "sp": np.ascontiguousarray(c.species, dtype=np.int32),
+        "bs": np.ascontiguousarray(spec.block_sizes, dtype=np.int32),
+        "db": np.ascontiguousarray(spec.d_blocks, dtype=np.float64),
+        **_species_arrays(spec.species),
+    }
+    r = _lib.BandRequest()
+    r.basis = spec.basis.h
+    r.v_r = arrays["v"].ctypes.data
+    r.nv = arrays["v"].size
+    r.nk = arrays["k"].shape[0]
+    r.kpoints = arrays["k"].ctypes.data
+    r.natoms = len(arrays["sp"])
+    r.atom_pos = arrays["pos"].ctypes.data
+    r.atom_species = arrays["sp"].ctypes.data
+    r.nspecies = len(spec.species)
+    for name in ("nmesh", "r_all", "rab_all", "nbeta", "l_all", "cutoff_all", "beta_r_all"):
+        setattr(r, name, arrays[name].ctypes.data)
+    r.block_sizes = arrays["bs"].ctypes.data
+    r.d_blocks = arrays["db"].ctypes.data
+    r.nbands = spec.nbands
+    r.noccupied = spec.noccupied
+    lib().q3x_davidson_default_options(ctypes.byref(r.options))
+    r.options.tolerance_ry = spec.tolerance_ry
+    r.options.max_iterations = spec.max_iterations
+    r.options.precision = spec.precision
+    r.kpath_propagation = spec.kpath_propagation
+    r.band_path_segments = spec.band_path_segments
+    r.band_path_points_per_segment = spec.band_path_points_per_segment
+    return r, arrays
+
+
 class BandStructureServer:
     """Continuous-batching server of band-structure requests (q3x_server_*): requests are
     admitted in arrival order while their estimated HBM footprint fits the budget, and the
@@ -500,40 +538,7 @@ class BandStructureServer:
         self._live = {}  # ticket -> (spec, arrays) kept alive until the result is collected
 
     def _request(self, spec: BandRequestSpec):
-        c = spec.basis.crystal
-        arrays = {
-            "v": np.ascontiguousarray(spec.v_r, dtype=np.float64).ravel(),
-            "k": np.ascontiguousarray(spec.kpoints, dtype=np.float64).reshape(-1, 3),
-            "pos": np.ascontiguousarray(c.positions, dtype=np.float64),
-            "sp": np.ascontiguousarray(c.species, dtype=np.int32),
-            "bs": np.ascontiguousarray(spec.block_sizes, dtype=np.int32),
-            "db": np.ascontiguousarray(spec.d_blocks, dtype=np.float64),
-            **_species_arrays(spec.species),
-        }
-        r = _lib.BandRequest()
-        r.basis = spec.basis.h
-        r.v_r = arrays["v"].ctypes.data
-        r.nv = arrays["v"].size
-        r.nk = arrays["k"].shape[0]
-        r.kpoints = arrays["k"].ctypes.data
-        r.natoms = len(arrays["sp"])
-        r.atom_pos = arrays["pos"].ctypes.data
-        r.atom_species = arrays["sp"].ctypes.data
-        r.nspecies = len(spec.species)
-        for name in ("nmesh", "r_all", "rab_all", "nbeta", "l_all", "cutoff_all", "beta_r_all"):
-            setattr(r, name, arrays[name].ctypes.data)
-        r.block_sizes = arrays["bs"].ctypes.data
-        r.d_blocks = arrays["db"].ctypes.data
-        r.nbands = spec.nbands
-        r.noccupied = spec.noccupied
-        lib().q3x_davidson_default_options(ctypes.byref(r.options))
-        r.options.tolerance_ry = spec.tolerance_ry
-        r.options.max_iterations = spec.max_iterations
-        r.options.precision = spec.precision
-        r.kpath_propagation = spec.kpath_propagation
-        r.band_path_segments = spec.band_path_segments
-        r.band_path_points_per_segment = spec.band_path_points_per_segment
-        return r, arrays
+        return _band_request(spec)
 
     def estimate(self, spec: BandRequestSpec) -> float:
         r, _arrays = self._request(spec)
@@ -588,3 +593,42 @@ class BandStructureServer:
 
     def __del__(self):
         self.close()
+
+
+def memory_estimate(**inputs) -> dict:
+    """PlanewaveMemoryEstimate::compute (iterator/kohn_sham/planewave/internal/memory_estimate.hpp:
+    75-215) in the library: keyword arguments are the reference's input fields (defaults as
+    PlanewaveMemoryEstimateInputs{}); returns the terms and totals."""
+    x = _lib.MemoryEstimateInputs()
+    lib().q3x_memory_estimate_default_inputs(ctypes.byref(x))
+    for k, v in inputs.items():
+        if not hasattr(x, k):
+            raise TypeError(f"unknown input {k!r}")
+        setattr(x, k, int(v))
+    e = _lib.MemoryEstimate()
+    check(lib().q3x_memory_estimate_compute(ctypes.byref(x), ctypes.byref(e)))
+    return _estimate_dict(e)
+
+
+def _terms(arr, n):
+    return [(t.label.decode(), t.formula.decode(), int(t.bytes)) for t in arr[:n]]
+
+
+def _estimate_dict(e) -> dict:
+    return {"steady_state": _terms(e.steady_state, e.n_steady_state), "transient": _terms(e.transient, e.n_transient),
+            "steady_state_bytes": int(e.steady_state_bytes), "solver_bytes": int(e.solver_bytes),
+            "setup_peak_bytes": int(e.setup_peak_bytes), "peak_bytes": int(e.peak_bytes),
+            "maximum_basis_size": int(e.maximum_basis_size)}
+
+
+def band_request_footprint(spec: BandRequestSpec, concurrent_kpoints: int) -> dict:
+    """The server's admission footprint of a request, itemised (q3x_band_request_footprint):
+    {"bytes", "reference": the reference estimate of its dimensions, "device_terms"}."""
+    r, _arrays = _band_request(spec)
+    e = _lib.MemoryEstimate()
+    terms = (_lib.MemoryTerm * 8)()
+    n = ctypes.c_int(0)
+    b = ctypes.c_double()
+    check(lib().q3x_band_request_footprint(ctypes.byref(r), int(concurrent_kpoints), ctypes.byref(e), terms,
+                                           ctypes.byref(n), ctypes.byref(b)))
+    return {"bytes": b.value, "reference": _estimate_dict(e), "device_terms": _terms(terms, n.value)}

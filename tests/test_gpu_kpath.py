@@ -121,3 +121,13 @@ def test_cancel_stops_running_kpaths():
     eig, st = srv.wait(srv.submit(_spec(r, ks[:3], 1e-8, segments=1, pps=2)))
     srv.close()
     assert st["solves"] == len(O.nscf_schedule(3, 1, 2)) and np.all(np.isfinite(eig))
+
+
+def test_server_estimate_is_the_itemised_footprint(path_problem):
+    from q3_amd.planewave import band_request_footprint
+    r, _ob, _V, _bs, _db, ks, _per_k = path_problem
+    srv = BandStructureServer(workers=3)
+    for kpath in (0, 1):
+        spec = _spec(r, ks, 1e-8, kpath=kpath)
+        assert srv.estimate(spec) == band_request_footprint(spec, 1 if kpath else 3)["bytes"]
+    srv.close()
