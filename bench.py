@@ -131,6 +131,13 @@ def _solve(ctx, h, nb, nocc, psi0, prec, tol, iters):
     return eig, dt, rep
 
 
+def _band_energy_err(eig, fix, nocc, natoms):
+    """|E_band - E_band(oracle)| per atom in Ha, E_band = 2 sum of the occupied eigenvalues (two
+    electrons per band): the energy a fixed-potential solve determines, the part of the total
+    energy this path computes."""
+    return float(abs(2.0 * (np.sum(eig[:nocc]) - np.sum(fix[:nocc]))) * HA_PER_RY / natoms)
+
+
 def davidson_section(args, ctx, h, w, psi0):
     """cfg2 Davidson solves from the fixed starting block (seed 2), measured against the FP64
     oracle's eigenvalues (tests/golden/cfg2_eigs.npz: the oracle's block Davidson held to 1e-10 Ry
@@ -144,6 +151,7 @@ def davidson_section(args, ctx, h, w, psi0):
     stop a slowly converging top band short of it); every attempt is reported."""
     from q3_amd import FP64, MIXED
     nb, nocc = w.nbands, w.noccupied
+    natoms = len(w.crystal.species)
     fix = np.load(FIXTURE_CFG2)["eigenvalues_ry"]
     for prec in (FP64, MIXED):  # untimed warm-up: first launches load kernels, workspaces grow
         _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, 2)
@@ -153,9 +161,11 @@ def davidson_section(args, ctx, h, w, psi0):
         eig, dt, rep = _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, args.davidson_iters)
         err = np.abs(eig - fix) * HA_PER_RY
         ref[label] = {"s_per_solve": dt, "tolerance_ry": 2e-6, "n_occupied": nocc, **rep,
-                      "max_abs_err_ha_occupied": float(err[:nocc].max()), "max_abs_err_ha_all": float(err.max())}
+                      "max_abs_err_ha_occupied": float(err[:nocc].max()), "max_abs_err_ha_all": float(err.max()),
+                      "band_energy_err_ha_per_atom": _band_energy_err(eig, fix, nocc, natoms)}
         log(f"davidson {label} (reference settings): {dt:.3f}s {rep['iterations']} it, occupied err "
-            f"{err[:nocc].max():.2e} Ha, all {err.max():.2e} Ha")
+            f"{err[:nocc].max():.2e} Ha, all {err.max():.2e} Ha, band energy "
+            f"{ref[label]['band_energy_err_ha_per_atom']:.1e} Ha/atom")
     ref["speedup_mixed_vs_fp64"] = ref["fp64"]["s_per_solve"] / ref["mixed"]["s_per_solve"]
     out["reference_settings"] = ref
     allb = {}
@@ -165,7 +175,8 @@ def davidson_section(args, ctx, h, w, psi0):
             eig, dt, rep = _solve(ctx, h, nb, nb, psi0, prec, tol, 4 * args.davidson_iters)
             err = float(np.abs(eig - fix).max() * HA_PER_RY)
             attempts.append({"tolerance_ry": tol, "s_per_solve": dt, "iterations": rep["iterations"],
-                             "max_abs_err_ha_all": err})
+                             "max_abs_err_ha_all": err,
+                             "band_energy_err_ha_per_atom": _band_energy_err(eig, fix, nocc, natoms)})
             log(f"davidson {label} all bands at {tol:g} Ry: {dt:.3f}s {rep['iterations']} it, max err {err:.2e} Ha")
             if err <= 1e-6:
                 break

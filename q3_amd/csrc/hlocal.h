@@ -34,7 +34,8 @@ size_t hlocal_lds_bytes(const HLocalArgs& a, bool fp32, int which);
 struct PlaneTlConfig {
     bool ok = false;
     int lc = 0, lr = 0;  // LDS strides (complex elements) of a column and of a row in flight
-    int rb = 0;          // rows in flight per chunk
+    int rb = 0;          // rows in flight per wave
+    int row_waves = 0;   // waves with a row buffer (the first row_waves of threads / 64)
     int threads = 0;
     size_t lds = 0;
     bool plane_global = false;  // plane too large for LDS: kept in a global slot per workgroup

@@ -47,12 +47,18 @@ __device__ __forceinline__ void wave_order() {
 // lanes of a group are contiguous in the wave (lane = g * R + index, R lanes per transform in a
 // stage).  Waves run their groups independently, so one wave's LDS and ALU latency overlaps
 // another's; block barriers separate only the scatter, the column pass, the row pass (a row
-// needs every column), the forward column pass and the gather.  Each wave has its own buffer of
-// G rows in flight.
+// needs every column), the forward column pass and the gather.  The first nrw waves have a
+// buffer of G rows in flight each and share the row pass; every wave takes part in the column
+// passes, so a workgroup can have more waves than row buffers fit beside the plane (the FP64
+// plane at one workgroup per CU: the column passes then finish in one round of groups).
+// Launch bound: 12 waves, except the FP64 shapes whose radix-15/16 stages need more than the
+// 168 registers that allows (8 waves, 256 registers).
+constexpr int plane_tl_max_threads(bool fp64, int n) { return fp64 && n >= 128 ? 512 : 768; }
+
 template <typename T, int A, int B, bool PG>
-__global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
+__global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B)) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
                                                  cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg, int lc,
-                                                 int lr, int rb, cx<T>* __restrict__ pglob) {
+                                                 int lr, int nrw, cx<T>* __restrict__ pglob) {
     constexpr int N = A * B;
     constexpr int S = B + 1;  // stored length of a block of B slots
     constexpr int G = 64 / (A > B ? A : B);
@@ -65,8 +71,7 @@ __global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restric
                   : lds_next;                // [ncol][lc]
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-    cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * G * lr;  // this wave's [G][lr]
-    (void)rb;
+    cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * G * lr;  // this wave's [G][lr] (waves < nrw)
     const int z = blockIdx.x, band = blockIdx.y;
     const cx<T> zero = {T(0), T(0)};
 
@@ -117,7 +122,15 @@ __global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restric
     __syncthreads();
 
     // rows; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
-    for (int r0 = wave * G; r0 < N; r0 += nw * G) {
+    for (int r0 = wave * G; wave < nrw && r0 < N; r0 += nrw * G) {
+        // this lane's V(r) values for its stage-2 role, loaded ahead of stage 1 so that their
+        // latency overlaps it
+        T vv[B];
+        if (gA < G && r0 + gA < N) {
+            const T* vrow = vplane + (static_cast<long>(z) * N + r0 + gA) * N + iA;
+#pragma unroll
+            for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
+        }
         if (gB < G && r0 + gB < N) {  // backward stage 1, lane n2
             const int n2 = iB, i2 = r0 + gB;
             const int pp = (i2 % A) * S + i2 / A;
@@ -137,12 +150,8 @@ __global__ void __launch_bounds__(512) plane_tl(HLocalArgs a, const T* __restric
         }
         wave_order();
         if (gA < G && r0 + gA < N) {  // backward stage 2, x V(r)/N, forward stage A; lane k1
-            const int k1 = iA, i2 = r0 + gA;
+            const int k1 = iA;
             cx<T>* blk = Rw + gA * lr + k1 * S;
-            const T* vrow = vplane + (static_cast<long>(z) * N + i2) * N + k1;
-            T vv[B];
-#pragma unroll
-            for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
             cx<T> u[B];
 #pragma unroll
             for (int j = 0; j < B; ++j) u[j] = blk[j];
@@ -386,10 +395,16 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
     const int want = std::min(8, std::max(4, (std::max(n, ncol) + G - 1) / G));
     int nw = static_cast<int>(std::min<size_t>(want, (max_lds - fixed) / per_wave));
     const int nw_half = fixed < max_lds / 2 ? static_cast<int>((max_lds / 2 - fixed) / per_wave) : 0;
-    if (!cfg.plane_global && nw_half >= 4) nw = std::min(nw, nw_half);
+    const bool two_per_cu = !cfg.plane_global && nw_half >= 4;
+    if (two_per_cu) nw = std::min(nw, nw_half);
     if (nw < 1) return cfg;
     cfg.rb = G;
-    cfg.threads = 64 * nw;
+    cfg.row_waves = nw;
+    // one workgroup per CU: more waves (no row buffers of their own) until the column passes
+    // take one round of groups, up to the kernel's launch bound
+    const int max_waves = plane_tl_max_threads(!fp32, n) / 64;
+    const int col_waves = two_per_cu ? nw : std::min(max_waves, std::max(nw, (ncol + G - 1) / G));
+    cfg.threads = 64 * col_waves;
     cfg.lds = fixed + per_wave * nw;
     cfg.plane_bytes = plane;
     cfg.ok = true;
@@ -491,7 +506,7 @@ static hipError_t launch_one(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, 
         raised = true;
     }
     hipLaunchKernelGGL((plane_tl<T, A, B, PG>), dim3(a.p3.n, nvec), dim3(c.threads), c.lds, st, a, vplane, zbuf,
-                       static_cast<const cx<T>*>(a.twp), c.lc, c.lr, c.rb, static_cast<cx<T>*>(pglob));
+                       static_cast<const cx<T>*>(a.twp), c.lc, c.lr, c.row_waves, static_cast<cx<T>*>(pglob));
     return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@ the oracle's block Davidson from the same starting block, every band held to 1e-
 Bars (1 Ha = 2 Ry): FP64 every band within 1e-8 Ha when every band is held to 1e-10 Ry; FP64
 occupied bands within 1e-8 Ha under the reference's own settings (empty bands relaxed,
 davidson.hpp:607-629); mixed precision every band within 1e-6 Ha, alone and in a batch of 8
-concurrent solves on 8 worker streams.
+concurrent solves on 8 worker streams; the mixed band energy (2 x the sum of the occupied
+eigenvalues) under the reference's settings within 1e-6 Ha per atom.
 """
 import os
 
@@ -71,6 +72,19 @@ def test_mixed_every_band_within_1e6_ha(cfg2):
     eig, rep = _solve(h, 160, psi0, MIXED, 2e-7)
     assert rep["unconverged"] == 0 and rep["fp64_refinements"] == 1
     assert np.abs(eig - fix).max() <= 1e-6 * HA
+
+
+def test_mixed_band_energy_per_atom_under_reference_settings(cfg2):
+    """The precision form's energy statement: the band energy E = 2 sum of the 128 occupied
+    eigenvalues (two electrons per band) from a mixed solve under the reference's settings
+    (2e-6 Ry, empty bands relaxed) within 1e-6 Ha per atom of the oracle's (64 atoms)."""
+    _, h, psi0, fix = cfg2
+    print("cfg2 mixed, reference settings", flush=True)
+    eig, rep = _solve(h, 128, psi0, MIXED, 2e-6, iters=60)
+    assert rep["unconverged"] == 0
+    err = abs(2.0 * (eig[:128].sum() - fix[:128].sum())) / HA / 64
+    print(f"band energy error {err:.2e} Ha/atom", flush=True)
+    assert err <= 1e-6
 
 
 def test_mixed_batch_of_eight_within_1e6_ha(cfg2):
