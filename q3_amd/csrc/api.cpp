@@ -866,8 +866,8 @@ q3x_status q3x_hamiltonian_prepare_kpoint(q3x_hamiltonian h, q3x_kpoint k) {
         // Q3X_FFT_KERNELS=generic keeps the Stockham kernels (used to compare the two)
         const char* pk = std::getenv("Q3X_FFT_KERNELS");
         const bool use_tl = !(pk && std::strcmp(pk, "generic") == 0) && plane_tl_supported(n1, n2);
-        h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds) : PlaneTlConfig{};
-        h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds) : PlaneTlConfig{};
+        h->tl64 = use_tl ? plane_tl_config(n1, S.ncol, false, h->ctx->max_lds, h->ctx->num_cus) : PlaneTlConfig{};
+        h->tl32 = use_tl ? plane_tl_config(n1, S.ncol, true, h->ctx->max_lds, h->ctx->num_cus) : PlaneTlConfig{};
         const bool use_tz = !(pk && std::strcmp(pk, "generic") == 0) && stick_tl_supported(n3);
         h->tz64 = use_tz ? stick_tl_config(n3, false) : StickTlConfig{};
         h->tz32 = use_tz ? stick_tl_config(n3, true) : StickTlConfig{};

@@ -37,12 +37,13 @@ struct PlaneTlConfig {
     int rb = 0;          // rows in flight per wave
     int row_waves = 0;   // waves with a row buffer (the first row_waves of threads / 64)
     int threads = 0;
+    int workgroups = 0;  // persistent workgroups of a launch (each loops over planes)
     size_t lds = 0;
     bool plane_global = false;  // plane too large for LDS: kept in a global slot per workgroup
     size_t plane_bytes = 0;     // bytes of one plane slot
 };
 bool plane_tl_supported(int n1, int n2);
-PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds);
+PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds, int num_cus);
 // Two-level z-stick kernels (plane_tl.hip) for the n3 the plane sizes list holds.
 struct StickTlConfig {
     bool ok = false;

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <limits>
 
 #include "fft_lds.h"
 #include "hlocal.h"
@@ -51,14 +52,18 @@ __device__ __forceinline__ void wave_order() {
 // buffer of G rows in flight each and share the row pass; every wave takes part in the column
 // passes, so a workgroup can have more waves than row buffers fit beside the plane (the FP64
 // plane at one workgroup per CU: the column passes then finish in one round of groups).
-// Launch bound: 12 waves, except the FP64 shapes whose radix-15/16 stages need more than the
-// 168 registers that allows (8 waves, 256 registers).
-constexpr int plane_tl_max_threads(bool fp64, int n) { return fp64 && n >= 128 ? 512 : 768; }
+// Launch bound: 12 waves in complex64; 8 in FP64, whose stages with the prefetched stick values
+// need more than the 168 registers that 12 waves allow.
+constexpr int plane_tl_max_threads(bool fp64, int n) { return fp64 || n > 144 ? 512 : 768; }
+// waves per SIMD the register allocation must allow: two complex64 workgroups of eight waves
+// per CU need four (128 registers); the radix-15/16 complex64 shapes need more and get two
+constexpr int plane_tl_min_waves(bool fp64, int a) { return fp64 || a > 12 ? 2 : 4; }
 
 template <typename T, int A, int B, bool PG>
-__global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B)) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
+__global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
+                                  plane_tl_min_waves(sizeof(T) == 8, A)) plane_tl(HLocalArgs a, const T* __restrict__ vplane,
                                                  cx<T>* __restrict__ zbuf, const cx<T>* __restrict__ twg, int lc,
-                                                 int lr, int nrw, cx<T>* __restrict__ pglob) {
+                                                 int lr, int nrw, int nplanes, cx<T>* __restrict__ pglob) {
     constexpr int N = A * B;
     constexpr int S = B + 1;  // stored length of a block of B slots
     constexpr int G = 64 / (A > B ? A : B);
@@ -67,152 +72,214 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B)) p
     int* colmap = reinterpret_cast<int*>(tw + N);    // i1 -> occupied column or -1
     cx<T>* lds_next = reinterpret_cast<cx<T>*>(colmap + ((N + 3) & ~3));
     const int ncol = a.ncol;
-    cx<T>* P = PG ? pglob + (static_cast<long>(blockIdx.y) * gridDim.x + blockIdx.x) * ncol * lc
-                  : lds_next;                // [ncol][lc]
+    cx<T>* P = PG ? pglob + static_cast<long>(blockIdx.x) * ncol * lc : lds_next;  // [ncol][lc]
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * G * lr;  // this wave's [G][lr] (waves < nrw)
-    const int z = blockIdx.x, band = blockIdx.y;
     const cx<T> zero = {T(0), T(0)};
 
     for (int i = tid; i < N; i += nt) {
         tw[i] = twg[i];
         colmap[i] = -1;
     }
-    for (int i = tid; i < ncol * lc; i += nt) P[i] = zero;
     __syncthreads();
     for (int c = tid; c < ncol; c += nt) colmap[a.col_i1[c]] = c;
-    cx<T>* zp = zbuf + (static_cast<long>(band) * a.p3.n + z) * a.nst;
-    for (int s = tid; s < a.nst; s += nt) {
-        const int q = a.stick_i2[s];
-        P[a.stick_col[s] * lc + q + q / B] = zp[s];
-    }
     __syncthreads();
-
-    // lanes of stage-1 / stage-B (B lanes per transform) and stage-2 / stage-A (A lanes)
-    const int gB = lane / B, iB = lane - gB * B;
-    const int gA = lane / A, iA = lane - gA * A;
-
-    // columns, backward: stage 1 (lane n2 = iB), stage 2 (lane k1 = iA)
-    for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
-        if (gB < G && c0 + gB < ncol) {
-            const int n2 = iB;
-            cx<T>* col = P + (c0 + gB) * lc + n2;
-            cx<T> v[A];
+    // FP64 (persistent): LDS offset of the occupied column i1 = B j + (lane mod B) (or -1), which
+    // row stage 1 reads and stage B writes for every row of every plane, held in registers
+    int cofs[A];
+    {
+        const int ib = (tid & 63) % B;
 #pragma unroll
-            for (int j = 0; j < A; ++j) v[j] = col[j * S];
-            Dft<T, A>::run(v, T(1));
-#pragma unroll
-            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
-#pragma unroll
-            for (int k = 0; k < A; ++k) col[k * S] = v[k];
+        for (int j = 0; j < A; ++j) {
+            const int c = colmap[B * j + ib];
+            cofs[j] = c >= 0 ? c * lc : -1;
         }
-        wave_order();
-        if (gA < G && c0 + gA < ncol) {
-            cx<T>* blk = P + (c0 + gA) * lc + iA * S;
-            cx<T> u[B];
-#pragma unroll
-            for (int j = 0; j < B; ++j) u[j] = blk[j];
-            Dft<T, B>::run(u, T(1));
-#pragma unroll
-            for (int j = 0; j < B; ++j) blk[j] = u[j];
-        }
-        wave_order();
     }
-    __syncthreads();
-
-    // rows; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
-    for (int r0 = wave * G; wave < nrw && r0 < N; r0 += nrw * G) {
-        // this lane's V(r) values for its stage-2 role, loaded ahead of stage 1 so that their
-        // latency overlaps it
-        T vv[B];
-        if (gA < G && r0 + gA < N) {
-            const T* vrow = vplane + (static_cast<long>(z) * N + r0 + gA) * N + iA;
-#pragma unroll
-            for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
+    auto col_offset = [&](int j, int ib) {
+        if constexpr (sizeof(T) == 8) {
+            return cofs[j];
+        } else {
+            const int c = colmap[B * j + ib];
+            return c >= 0 ? c * lc : -1;
         }
-        if (gB < G && r0 + gB < N) {  // backward stage 1, lane n2
-            const int n2 = iB, i2 = r0 + gB;
-            const int pp = (i2 % A) * S + i2 / A;
-            cx<T> v[A];
+    };
+
+    // FP64 (one workgroup per CU): persistent over planes p = blockIdx.x + k gridDim.x (plane
+    // p = band * n3 + z); the stick values of the next plane are loaded into registers while the
+    // current one is transformed, so their latency is hidden behind the passes (a block barrier
+    // waits for LDS only), and the sticks' LDS slots stay in registers.  Complex64 (two
+    // workgroups per CU, which overlap each other's loads): one plane per workgroup.
+    constexpr bool persist = sizeof(T) == 8;
+    constexpr int PF = persist ? 4 : 1;  // stick values per thread carried over; the rest load at once
+    cx<T> pre[PF];
+    int soff[PF];  // LDS slot of stick tid + i nt, the same in every plane
 #pragma unroll
-            for (int j = 0; j < A; ++j) {
-                const int c = colmap[B * j + n2];
-                v[j] = zero;
-                if (c >= 0) v[j] = P[c * lc + pp];
+    for (int i = 0; i < PF; ++i) {
+        const int s = tid + i * nt;
+        soff[i] = 0;
+        if (persist && s < a.nst) {
+            const int q = a.stick_i2[s];
+            soff[i] = a.stick_col[s] * lc + q + q / B;
+        }
+    }
+    auto prefetch = [&](int plane) {
+        const cx<T>* src = zbuf + static_cast<long>(plane) * a.nst;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int s = tid + i * nt;
+            if (s < a.nst) pre[i] = src[s];
+        }
+    };
+    if (persist && blockIdx.x < nplanes) prefetch(blockIdx.x);
+    for (int plane = blockIdx.x; plane < nplanes; plane += persist ? gridDim.x : nplanes) {
+        const int z = plane % a.p3.n;
+        for (int i = tid; i < ncol * lc; i += nt) P[i] = zero;
+        __syncthreads();
+        cx<T>* zp = zbuf + static_cast<long>(plane) * a.nst;
+        if (persist) {
+#pragma unroll
+            for (int i = 0; i < PF; ++i)
+                if (tid + i * nt < a.nst) P[soff[i]] = pre[i];
+        }
+        for (int s = tid + (persist ? PF * nt : 0); s < a.nst; s += nt) {
+            const int q = a.stick_i2[s];
+            P[a.stick_col[s] * lc + q + q / B] = zp[s];
+        }
+        __syncthreads();
+        if (persist && plane + static_cast<int>(gridDim.x) < nplanes) prefetch(plane + gridDim.x);
+
+        // lanes of stage-1 / stage-B (B lanes per transform) and stage-2 / stage-A (A lanes)
+        const int gB = lane / B, iB = lane - gB * B;
+        const int gA = lane / A, iA = lane - gA * A;
+
+        // columns, backward: stage 1 (lane n2 = iB), stage 2 (lane k1 = iA)
+        for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
+            if (gB < G && c0 + gB < ncol) {
+                const int n2 = iB;
+                cx<T>* col = P + (c0 + gB) * lc + n2;
+                cx<T> v[A];
+#pragma unroll
+                for (int j = 0; j < A; ++j) v[j] = col[j * S];
+                Dft<T, A>::run(v, T(1));
+#pragma unroll
+                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+#pragma unroll
+                for (int k = 0; k < A; ++k) col[k * S] = v[k];
             }
-            Dft<T, A>::run(v, T(1));
+            wave_order();
+            if (gA < G && c0 + gA < ncol) {
+                cx<T>* blk = P + (c0 + gA) * lc + iA * S;
+                cx<T> u[B];
 #pragma unroll
-            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
-            cx<T>* row = Rw + gB * lr + n2;
+                for (int j = 0; j < B; ++j) u[j] = blk[j];
+                Dft<T, B>::run(u, T(1));
 #pragma unroll
-            for (int k = 0; k < A; ++k) row[k * S] = v[k];
-        }
-        wave_order();
-        if (gA < G && r0 + gA < N) {  // backward stage 2, x V(r)/N, forward stage A; lane k1
-            const int k1 = iA;
-            cx<T>* blk = Rw + gA * lr + k1 * S;
-            cx<T> u[B];
-#pragma unroll
-            for (int j = 0; j < B; ++j) u[j] = blk[j];
-            Dft<T, B>::run(u, T(1));
-#pragma unroll
-            for (int j = 0; j < B; ++j) u[j] = cscale(u[j], vv[j]);
-            Dft<T, B>::run(u, T(-1));
-#pragma unroll
-            for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[k1 * m]);
-#pragma unroll
-            for (int j = 0; j < B; ++j) blk[j] = u[j];
-        }
-        wave_order();
-        if (gB < G && r0 + gB < N) {  // forward stage B, lane m2; outputs i1 = m2 + B m1, occupied only
-            const int m2 = iB, i2 = r0 + gB;
-            const int pp = (i2 % A) * S + i2 / A;
-            const cx<T>* row = Rw + gB * lr + m2;
-            cx<T> v[A];
-#pragma unroll
-            for (int j = 0; j < A; ++j) v[j] = row[j * S];
-            Dft<T, A>::run(v, T(-1));
-#pragma unroll
-            for (int m = 0; m < A; ++m) {
-                const int c = colmap[m2 + B * m];
-                if (c >= 0) P[c * lc + pp] = v[m];
+                for (int j = 0; j < B; ++j) blk[j] = u[j];
             }
+            wave_order();
         }
-        wave_order();
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // columns, forward: stage A (lane n1 = iA, block n1 * S holds elements n1 + A n2), stage B
-    for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
-        if (gA < G && c0 + gA < ncol) {
-            const int n1 = iA;
-            cx<T>* blk = P + (c0 + gA) * lc + n1 * S;
-            cx<T> u[B];
+        // rows; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
+        for (int r0 = wave * G; wave < nrw && r0 < N; r0 += nrw * G) {
+            // this lane's V(r) values for its stage-2 role, loaded ahead of stage 1 so that their
+            // latency overlaps it
+            T vv[B];
+            if (gA < G && r0 + gA < N) {
+                const T* vrow = vplane + (static_cast<long>(z) * N + r0 + gA) * N + iA;
 #pragma unroll
-            for (int j = 0; j < B; ++j) u[j] = blk[j];
-            Dft<T, B>::run(u, T(-1));
+                for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
+            }
+            if (gB < G && r0 + gB < N) {  // backward stage 1, lane n2
+                const int n2 = iB, i2 = r0 + gB;
+                const int pp = (i2 % A) * S + i2 / A;
+                cx<T> v[A];
 #pragma unroll
-            for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+                for (int j = 0; j < A; ++j) {
+                    const int o = col_offset(j, n2);
+                    v[j] = zero;
+                    if (o >= 0) v[j] = P[o + pp];
+                }
+                Dft<T, A>::run(v, T(1));
 #pragma unroll
-            for (int j = 0; j < B; ++j) blk[j] = u[j];
+                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+                cx<T>* row = Rw + gB * lr + n2;
+#pragma unroll
+                for (int k = 0; k < A; ++k) row[k * S] = v[k];
+            }
+            wave_order();
+            if (gA < G && r0 + gA < N) {  // backward stage 2, x V(r)/N, forward stage A; lane k1
+                const int k1 = iA;
+                cx<T>* blk = Rw + gA * lr + k1 * S;
+                cx<T> u[B];
+#pragma unroll
+                for (int j = 0; j < B; ++j) u[j] = blk[j];
+                Dft<T, B>::run(u, T(1));
+#pragma unroll
+                for (int j = 0; j < B; ++j) u[j] = cscale(u[j], vv[j]);
+                Dft<T, B>::run(u, T(-1));
+#pragma unroll
+                for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[k1 * m]);
+#pragma unroll
+                for (int j = 0; j < B; ++j) blk[j] = u[j];
+            }
+            wave_order();
+            if (gB < G && r0 + gB < N) {  // forward stage B, lane m2; outputs i1 = m2 + B m1, occupied only
+                const int m2 = iB, i2 = r0 + gB;
+                const int pp = (i2 % A) * S + i2 / A;
+                const cx<T>* row = Rw + gB * lr + m2;
+                cx<T> v[A];
+#pragma unroll
+                for (int j = 0; j < A; ++j) v[j] = row[j * S];
+                Dft<T, A>::run(v, T(-1));
+#pragma unroll
+                for (int m = 0; m < A; ++m) {
+                    const int o = col_offset(m, m2);
+                    if (o >= 0) P[o + pp] = v[m];
+                }
+            }
+            wave_order();
         }
-        wave_order();
-        if (gB < G && c0 + gB < ncol) {
-            cx<T>* col = P + (c0 + gB) * lc + iB;
-            cx<T> v[A];
+        __syncthreads();
+
+        // columns, forward: stage A (lane n1 = iA, block n1 * S holds elements n1 + A n2), stage B
+        for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
+            if (gA < G && c0 + gA < ncol) {
+                const int n1 = iA;
+                cx<T>* blk = P + (c0 + gA) * lc + n1 * S;
+                cx<T> u[B];
 #pragma unroll
-            for (int j = 0; j < A; ++j) v[j] = col[j * S];
-            Dft<T, A>::run(v, T(-1));
+                for (int j = 0; j < B; ++j) u[j] = blk[j];
+                Dft<T, B>::run(u, T(-1));
 #pragma unroll
-            for (int m = 0; m < A; ++m) col[m * S] = v[m];
+                for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+#pragma unroll
+                for (int j = 0; j < B; ++j) blk[j] = u[j];
+            }
+            wave_order();
+            if (gB < G && c0 + gB < ncol) {
+                cx<T>* col = P + (c0 + gB) * lc + iB;
+                cx<T> v[A];
+#pragma unroll
+                for (int j = 0; j < A; ++j) v[j] = col[j * S];
+                Dft<T, A>::run(v, T(-1));
+#pragma unroll
+                for (int m = 0; m < A; ++m) col[m * S] = v[m];
+            }
+            wave_order();
         }
-        wave_order();
-    }
-    __syncthreads();
-    for (int s = tid; s < a.nst; s += nt) {
-        const int q = a.stick_i2[s];
-        zp[s] = P[a.stick_col[s] * lc + q + q / B];
+        __syncthreads();
+        if (persist) {
+#pragma unroll
+            for (int i = 0; i < PF; ++i)
+                if (tid + i * nt < a.nst) zp[tid + i * nt] = P[soff[i]];
+        }
+        for (int s = tid + (persist ? PF * nt : 0); s < a.nst; s += nt) {
+            const int q = a.stick_i2[s];
+            zp[s] = P[a.stick_col[s] * lc + q + q / B];
+        }
+        __syncthreads();  // P is cleared for the next plane
     }
 }
 
@@ -371,7 +438,7 @@ bool plane_tl_supported(int n1, int n2) {
     return false;
 }
 
-PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
+PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds, int num_cus) {
     PlaneTlConfig cfg{};
     int A = 0, B = 0;
     for (const TlShape& s : kTlShapes)
@@ -406,6 +473,8 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds) {
     const int col_waves = two_per_cu ? nw : std::min(max_waves, std::max(nw, (ncol + G - 1) / G));
     cfg.threads = 64 * col_waves;
     cfg.lds = fixed + per_wave * nw;
+    // FP64: persistent workgroups, one per CU; complex64: one per plane
+    cfg.workgroups = fp32 ? std::numeric_limits<int>::max() : std::max(1, num_cus);
     cfg.plane_bytes = plane;
     cfg.ok = true;
     return cfg;
@@ -505,8 +574,10 @@ static hipError_t launch_one(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, 
         if (e != hipSuccess) return e;
         raised = true;
     }
-    hipLaunchKernelGGL((plane_tl<T, A, B, PG>), dim3(a.p3.n, nvec), dim3(c.threads), c.lds, st, a, vplane, zbuf,
-                       static_cast<const cx<T>*>(a.twp), c.lc, c.lr, c.row_waves, static_cast<cx<T>*>(pglob));
+    const int nplanes = a.p3.n * nvec;
+    const int grid = std::min(nplanes, std::max(1, c.workgroups));
+    hipLaunchKernelGGL((plane_tl<T, A, B, PG>), dim3(grid), dim3(c.threads), c.lds, st, a, vplane, zbuf,
+                       static_cast<const cx<T>*>(a.twp), c.lc, c.lr, c.row_waves, nplanes, static_cast<cx<T>*>(pglob));
     return hipGetLastError();
 }
 
