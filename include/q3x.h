@@ -169,6 +169,17 @@ q3x_status q3x_hamiltonian_apply(q3x_hamiltonian h, const double* psi, double* h
  * context stream, returns after queueing (call q3x_context_synchronize to wait). */
 q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, void* d_hpsi,
                                         int nvec, int precision);
+/* Charge density of the prepared k-point's bands (PlanewaveChargeDensityAccumulator::
+ * accumulate_kpoint_state, src/layer_4_computation/charge_density/planewave/internal/
+ * accumulator.hpp:159-280, with the per-band kernel of accumulator_gpu.hpp:267):
+ * rho(r) += sum_b weights[b] |psi_b(r)|^2 on the wavefunction box, psi_b(r) = sum_G c_b(G)
+ * e^{i(k+G).r} (the unnormalised backward transform).  d_psi: npw x nb complex128 in HBM (with the
+ * Gamma trick, the half-sphere blocks: pairs of real bands share one transform); weights: nb
+ * host doubles (w_k f_nk / Omega for the reference's rho); d_rho: n1 n2 n3 doubles in HBM,
+ * C order (i1, i2, i3), added to (the caller clears it).  Stream-ordered.  FP64.  Needs a box the
+ * two-level kernels cover with every plane wave inside it (Q3X_ERR_ARGUMENT otherwise). */
+q3x_status q3x_hamiltonian_accumulate_density(q3x_hamiltonian h, const void* d_psi, int nb, const double* weights,
+                                              void* d_rho);
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code under
  * stream capture, serving loops that forbid hipMalloc): the product of
  * q3x_hamiltonian_apply_device with every scratch buffer (stick buffer, plane slots, projector

@@ -92,6 +92,7 @@ SIGNATURES = {
     "q3x_hamiltonian_random_psi": (_i, [_p, _i, ctypes.c_ulonglong, _p]),
     "q3x_hamiltonian_apply": (_i, [_p, _p, _p, _i, _i]),
     "q3x_hamiltonian_apply_device": (_i, [_p, _p, _p, _i, _i]),
+    "q3x_hamiltonian_accumulate_density": (_i, [_p, _p, _i, _p, _p]),
     "q3x_hamiltonian_apply_noalloc": (_i, [_p, _p, _p, _i, _i]),
     "q3x_noalloc_scratch_bytes": (ctypes.c_ulong, []),
     "q3x_basis_create_gamma": (_i, [_p, _d, _d, _d, _i, _p, _i, _pp]),

@@ -469,7 +469,7 @@ double device_bytes_held(const q3x_context_s* c, const q3x_hamiltonian_s* h) {
                                 &h->ent_pos, &h->ekin, &h->outside, &h->beta64, &h->beta32, &h->dblocks, &h->pstart,
                                 &h->psize, &h->poff, &h->zbuf, &h->P, &h->W, &h->gemm_work, &h->io_psi, &h->io_hpsi,
                                 &h->plane_slots, &h->gpack, &h->gout, &h->kb_g, &h->kb_tables, &h->kb_form,
-                                &h->kb_ints, &h->kb_dbl})
+                                &h->kb_ints, &h->kb_dbl, &h->dens_w})
             b += static_cast<double>(d->size());
         for (const FftTables& t : h->fft)
             b += static_cast<double>(t.tw64.size() + t.tw32.size() + t.plain64.size() + t.plain32.size());
@@ -990,6 +990,69 @@ q3x_status q3x_hamiltonian_apply_device(q3x_hamiltonian h, const void* d_psi, vo
                                         nvec, precision);
     } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
         set_error(std::string("q3x_hamiltonian_apply_device: ") + e.what());
+        return Q3X_ERR_INTERNAL;
+    }
+}
+
+q3x_status q3x_hamiltonian_accumulate_density(q3x_hamiltonian h, const void* d_psi, int nb, const double* weights,
+                                              void* d_rho) {
+    try {
+        if (!h || !d_psi || !weights || !d_rho || nb < 0) {
+            set_error("q3x_hamiltonian_accumulate_density: invalid argument");
+            return Q3X_ERR_ARGUMENT;
+        }
+        if (!h->kp) {
+            set_error("q3x_hamiltonian_accumulate_density: prepare_kpoint() must come first");
+            return Q3X_ERR_STATE;
+        }
+        if (!h->tl64.ok || h->tl64.plane_global || !h->tz64.ok || h->n_outside > 0) {
+            set_error("q3x_hamiltonian_accumulate_density: the wavefunction box must be one the two-level kernels "
+                      "cover (square planes of a listed size, the plane in LDS) and hold every plane wave");
+            return Q3X_ERR_ARGUMENT;
+        }
+        if (nb == 0) return Q3X_OK;
+        hipStream_t st = h->ctx->stream;
+        const HLocalArgs args = make_args(h, false);
+        const int ncols = h->gamma ? (nb + 1) / 2 : nb;
+        // per column (w_re, w_im): a band's weight twice, or the weights of a packed Gamma pair
+        std::vector<double> w2(2 * static_cast<size_t>(ncols));
+        for (int c = 0; c < ncols; ++c) {
+            if (h->gamma) {
+                w2[2 * c] = weights[2 * c];
+                w2[2 * c + 1] = 2 * c + 1 < nb ? weights[2 * c + 1] : 0.0;
+            } else {
+                w2[2 * c] = w2[2 * c + 1] = weights[c];
+            }
+        }
+        if (!h->dens_w.upload(w2.data(), w2.size() * sizeof(double), st, &h->ctx->pin)) return Q3X_ERR_DEVICE;
+        const int chunk = band_chunk(h, h->tl64, ncols);
+        if (!h->zbuf.ensure(static_cast<size_t>(chunk) * h->dims[2] * h->nst * 16)) return Q3X_ERR_DEVICE;
+        if (h->gamma && !h->gpack.ensure(static_cast<size_t>(h->npw_ext) * chunk * 16)) return Q3X_ERR_DEVICE;
+        const double* psi = static_cast<const double*>(d_psi);
+        for (int c0 = 0; c0 < ncols; c0 += chunk) {
+            const int nc = std::min(chunk, ncols - c0);
+            const double* in = psi + 2L * c0 * h->npw;
+            long ldin = h->npw;
+            if (h->gamma) {  // bands 2 c0 .. packed in pairs on the extended list (accumulator.hpp:52-56)
+                const int nbands = std::min(2 * nc, nb - 2 * c0);
+                if (!hip_ok(launch_gamma_pack(h->npw, nbands, psi + 4L * c0 * h->npw, h->npw, h->gpack.as<double>(),
+                                              h->npw_ext, st),
+                            "Gamma pack"))
+                    return Q3X_ERR_DEVICE;
+                in = h->gpack.as<double>();
+                ldin = h->npw_ext;
+            }
+            if (!hip_ok(launch_sticks_tl<double>(args, +1, in, ldin, h->zbuf.as<cx<double>>(), nullptr, 0, h->tz64, nc,
+                                                 st),
+                        "density z-sticks") ||
+                !hip_ok(launch_plane_density(args, h->zbuf.as<cx<double>>(), h->tl64, nc, h->dens_w.as<double>() + 2L * c0,
+                                             static_cast<double*>(d_rho), st),
+                        "density planes"))
+                return Q3X_ERR_DEVICE;
+        }
+        return Q3X_OK;
+    } catch (const std::exception& e) {  // nothing crosses the C ABI as an exception
+        set_error(std::string("q3x_hamiltonian_accumulate_density: ") + e.what());
         return Q3X_ERR_INTERNAL;
     }
 }

@@ -62,6 +62,11 @@ hipError_t launch_sticks_tl(const HLocalArgs& a, int dir, const double* psi, lon
 template <typename T>
 hipError_t launch_plane_tl(const HLocalArgs& a, const T* vplane, cx<T>* zbuf, const PlaneTlConfig& c, int nvec,
                            void* pglob, hipStream_t st);
+// Charge density of nvec columns of the stick buffer (after the inverse z-stick pass):
+// rho(i1, i2, i3) += wts[2c] Re(f_c(r))^2 + wts[2c+1] Im(f_c(r))^2, rho C-order (i1, i2, i3), FP64.
+// Needs the two-level plane configuration with the plane in LDS.
+hipError_t launch_plane_density(const HLocalArgs& a, const cx<double>* zbuf, const PlaneTlConfig& c, int nvec,
+                                const double* wts, double* rho, hipStream_t st);
 // psi / hpsi: complex128, column per band with leading dimension ld (in complex elements).
 // marks (optional): 4 events recorded before/between/after the three kernels.
 // vplane: V(r)/N in plane-major layout [i3][i2][i1], double or float to match fp32.

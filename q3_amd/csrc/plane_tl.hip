@@ -629,4 +629,157 @@ template hipError_t launch_plane_tl<double>(const HLocalArgs&, const double*, cx
 template hipError_t launch_plane_tl<float>(const HLocalArgs&, const float*, cx<float>*, const PlaneTlConfig&, int,
                                            void*, hipStream_t);
 
+// ---- charge density (PlanewaveChargeDensityAccumulator::accumulate, accumulator.hpp:77-99, and the
+// per-band |psi(r)|^2 kernel of accumulator_gpu.hpp:267 / gpu_kernels_cuda.cu:156-172), FP64.
+// Per (column, i3) plane of the stick buffer: the backward column and row transforms of
+// plane_tl, then rho(i1, i2, i3) += w_re Re(f)^2 + w_im Im(f)^2 for the real-space values f of the
+// column.  A column is one band (w_re = w_im = its weight) or, with the Gamma trick, a packed pair
+// f = f1 + i f2 of real bands (w_re, w_im = their weights).  rho is the C-order (i1, i2, i3) box;
+// the planes of different columns add into it atomically.
+template <int A, int B>
+__global__ void __launch_bounds__(512) plane_density(HLocalArgs a, const cx<double>* __restrict__ zbuf,
+                                                     const cx<double>* __restrict__ twg, int lc, int lr, int nrw,
+                                                     const double2* __restrict__ wts, double* __restrict__ rho) {
+    using T = double;
+    constexpr int N = A * B;
+    constexpr int S = B + 1;
+    constexpr int G = 64 / (A > B ? A : B);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    int* colmap = reinterpret_cast<int*>(tw + N);
+    cx<T>* P = reinterpret_cast<cx<T>*>(colmap + ((N + 3) & ~3));  // [ncol][lc]
+    const int ncol = a.ncol, n3 = a.p3.n;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    cx<T>* Rw = P + ncol * lc + wave * G * lr;  // this wave's [G][lr] (waves < nrw)
+    const int z = blockIdx.x, band = blockIdx.y;
+    const cx<T> zero = {T(0), T(0)};
+    for (int i = tid; i < N; i += nt) {
+        tw[i] = twg[i];
+        colmap[i] = -1;
+    }
+    for (int i = tid; i < ncol * lc; i += nt) P[i] = zero;
+    __syncthreads();
+    for (int c = tid; c < ncol; c += nt) colmap[a.col_i1[c]] = c;
+    const cx<T>* zp = zbuf + (static_cast<long>(band) * n3 + z) * a.nst;
+    for (int s = tid; s < a.nst; s += nt) {
+        const int q = a.stick_i2[s];
+        P[a.stick_col[s] * lc + q + q / B] = zp[s];
+    }
+    __syncthreads();
+    const int gB = lane / B, iB = lane - gB * B;
+    const int gA = lane / A, iA = lane - gA * A;
+    // columns, backward (as plane_tl)
+    for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
+        if (gB < G && c0 + gB < ncol) {
+            cx<T>* col = P + (c0 + gB) * lc + iB;
+            cx<T> v[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) v[j] = col[j * S];
+            Dft<T, A>::run(v, T(1));
+#pragma unroll
+            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[iB * k]);
+#pragma unroll
+            for (int k = 0; k < A; ++k) col[k * S] = v[k];
+        }
+        wave_order();
+        if (gA < G && c0 + gA < ncol) {
+            cx<T>* blk = P + (c0 + gA) * lc + iA * S;
+            cx<T> u[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) u[j] = blk[j];
+            Dft<T, B>::run(u, T(1));
+#pragma unroll
+            for (int j = 0; j < B; ++j) blk[j] = u[j];
+        }
+        wave_order();
+    }
+    __syncthreads();
+    const double wr = wts[band].x, wi = wts[band].y;
+    // rows, backward, then the weighted squares; row i2 sits at slot (i2 % A) * S + i2 / A
+    for (int r0 = wave * G; wave < nrw && r0 < N; r0 += nrw * G) {
+        if (gB < G && r0 + gB < N) {
+            const int i2 = r0 + gB;
+            const int pp = (i2 % A) * S + i2 / A;
+            cx<T> v[A];
+#pragma unroll
+            for (int j = 0; j < A; ++j) {
+                const int c = colmap[B * j + iB];
+                v[j] = zero;
+                if (c >= 0) v[j] = P[c * lc + pp];
+            }
+            Dft<T, A>::run(v, T(1));
+#pragma unroll
+            for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[iB * k]);
+            cx<T>* row = Rw + gB * lr + iB;
+#pragma unroll
+            for (int k = 0; k < A; ++k) row[k * S] = v[k];
+        }
+        wave_order();
+        if (gA < G && r0 + gA < N) {  // lane k1: elements i1 = k1 + A j of row i2
+            const int i2 = r0 + gA;
+            const cx<T>* blk = Rw + gA * lr + iA * S;
+            cx<T> u[B];
+#pragma unroll
+            for (int j = 0; j < B; ++j) u[j] = blk[j];
+            Dft<T, B>::run(u, T(1));
+#pragma unroll
+            for (int j = 0; j < B; ++j) {
+                const long i1 = iA + A * j;
+                atomicAdd(rho + (i1 * N + i2) * n3 + z, wr * u[j].x * u[j].x + wi * u[j].y * u[j].y);
+            }
+        }
+        wave_order();
+    }
+}
+
+template <int A, int B>
+static hipError_t launch_density_shape(const HLocalArgs& a, const cx<double>* zbuf, const PlaneTlConfig& c, int nvec,
+                                       const double* wts, double* rho, hipStream_t st) {
+    static std::atomic<bool> raised{false};
+    if (!raised) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&plane_density<A, B>),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        raised = true;
+    }
+    hipLaunchKernelGGL((plane_density<A, B>), dim3(a.p3.n, nvec), dim3(std::min(c.threads, 512)), c.lds, st, a, zbuf,
+                       static_cast<const cx<double>*>(a.twp), c.lc, c.lr, c.row_waves,
+                       reinterpret_cast<const double2*>(wts), rho);
+    return hipGetLastError();
+}
+
+hipError_t launch_plane_density(const HLocalArgs& a, const cx<double>* zbuf, const PlaneTlConfig& c, int nvec,
+                                const double* wts, double* rho, hipStream_t st) {
+    if (!c.ok || c.plane_global) return hipErrorInvalidValue;
+    switch (a.p1.n) {
+#define Q3X_TL(NN, AA, BB) \
+    case NN: return launch_density_shape<AA, BB>(a, zbuf, c, nvec, wts, rho, st);
+        Q3X_TL(24, 6, 4)
+        Q3X_TL(30, 6, 5)
+        Q3X_TL(32, 8, 4)
+        Q3X_TL(36, 6, 6)
+        Q3X_TL(40, 8, 5)
+        Q3X_TL(45, 9, 5)
+        Q3X_TL(48, 8, 6)
+        Q3X_TL(54, 9, 6)
+        Q3X_TL(60, 10, 6)
+        Q3X_TL(64, 8, 8)
+        Q3X_TL(72, 9, 8)
+        Q3X_TL(75, 15, 5)
+        Q3X_TL(80, 10, 8)
+        Q3X_TL(81, 9, 9)
+        Q3X_TL(90, 10, 9)
+        Q3X_TL(96, 12, 8)
+        Q3X_TL(100, 10, 10)
+        Q3X_TL(108, 12, 9)
+        Q3X_TL(120, 12, 10)
+        Q3X_TL(128, 16, 8)
+        Q3X_TL(135, 15, 9)
+        Q3X_TL(144, 12, 12)
+#undef Q3X_TL
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace q3x

@@ -140,6 +140,7 @@ struct q3x_hamiltonian_s {
     bool gamma = false;
     int npw_ext = 0;
     q3x::DevBuf gpack, gout;
+    q3x::DevBuf dens_w;  // per-column weights of q3x_hamiltonian_accumulate_density
     // scratch
     q3x::DevBuf zbuf, P, W, gemm_work, io_psi, io_hpsi;
     q3x::DevBuf plane_slots;  // per-workgroup planes of the two-level kernel when a plane exceeds LDS

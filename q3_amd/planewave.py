@@ -316,6 +316,33 @@ class PlanewaveHamiltonianApplicator:
         check(lib().q3x_hamiltonian_apply_device(self.h, ctypes.c_void_p(d_psi), ctypes.c_void_p(d_hpsi), nvec,
                                                  precision))
 
+    def accumulate_density_device(self, d_psi: int, nbands: int, weights, d_rho: int):
+        """rho(r) += sum_b weights[b] |psi_b(r)|^2 on the wavefunction box (device pointers: psi
+        npw x nbands complex128, rho n1 n2 n3 doubles, C order), stream-ordered
+        (accumulator.hpp:159-280)."""
+        w = np.ascontiguousarray(weights, dtype=np.float64).ravel()
+        if w.size != nbands:
+            raise ValueError("one weight per band")
+        check(lib().q3x_hamiltonian_accumulate_density(self.h, ctypes.c_void_p(d_psi), nbands, _ptr(w),
+                                                       ctypes.c_void_p(d_rho)))
+
+    def density(self, psi: np.ndarray, weights) -> np.ndarray:
+        """sum_b weights[b] |psi_b(r)|^2 of host bands (npw x nb) on the wavefunction box, as an
+        (n1, n2, n3) array."""
+        psi = np.asfortranarray(psi, dtype=np.complex128)
+        if psi.ndim == 1:
+            psi = psi[:, None]
+        dims = tuple(self.basis.wave_dims)
+        d_psi = DeviceArray.from_host(self.context, psi)
+        d_rho = DeviceArray.from_host(self.context, np.zeros(int(np.prod(dims))))
+        try:
+            self.accumulate_density_device(d_psi.ptr, psi.shape[1], weights, d_rho.ptr)
+            self.context.synchronize()
+            return d_rho.download((int(np.prod(dims)),), np.float64).reshape(dims)
+        finally:
+            d_psi.free()
+            d_rho.free()
+
     def apply_noalloc(self, d_psi: int, d_hpsi: int, nvec: int, precision: int = FP64):
         """apply_device with every scratch buffer in the library's fixed per-GPU array: no device
         allocation (for callers that cannot allocate).  Calls overlapping on one GPU race."""
