@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-davidson", action="store_true")
     p.add_argument("--no-gamma-hpsi", action="store_true", help="skip the Gamma-trick H|psi> step")
+    p.add_argument("--no-scf", action="store_true", help="skip the cfg1 SCF loop")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-workers", type=int, default=8)
@@ -58,7 +59,7 @@ def parse():
                    help="1: each stream request is a band path (one straight segment) solved with the "
                         "non-self-consistent k-loop (propagation + backward fix); 0: independent k-points")
     p.add_argument("--davidson-iters", type=int, default=60)
-    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "copy"], default="full",
+    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "scf", "copy"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
                         "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; serve: the request server only; copy: a plain 1 GiB device copy, the reference for HBM byte counters")
     p.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
@@ -254,6 +255,42 @@ def davidson_section(args, ctx, h, w, psi0):
     if allb["fp64"]["met_1e-6_ha"] and allb["mixed"]["met_1e-6_ha"]:
         allb["speedup_mixed_vs_fp64"] = allb["fp64"]["s_per_solve"] / allb["mixed"]["s_per_solve"]
     out["all_bands_1e-6_ha"] = allb
+    return out
+
+
+def scf_section(args):
+    """cfg1 end to end: the SCF loop (q3_amd.scf: Davidson and the density on the GPU, potential and
+    Broyden mixing on the host) from a uniform density to dr2 < 1e-12, FP64 and with the mixed
+    Davidson, each against the oracle's self-consistent solution (tests/golden/cfg1_scf.npz)."""
+    from q3_amd import FP64, MIXED, Context
+    from q3_amd import workloads as W
+    from q3_amd.scf import PlanewaveSCF
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "cfg1_scf.npz"))
+    w = W.cfg1()
+    out = {"workload": "cfg1: Si2 FCC a=10.26 bohr, 15 Ha, 30^3, 4x4x4 shifted MP (64 k), 8 bands, Fermi-Dirac "
+                       "0.01 Ry, PZ LDA", "conv_thr": float(fx["conv_thr"]),
+           "oracle": "tests/golden/cfg1_scf.npz (CPU oracle SCF)"}
+    ctx = Context(0)
+    try:
+        for label, prec in (("fp64", FP64), ("mixed", MIXED)):
+            scf = PlanewaveSCF(ctx, w.crystal, w.species, w.ecutwfc_ry, w.kpoints, np.full(64, 1 / 64), w.nbands, 8,
+                               fft_factors=w.fft_factors, sigma=float(fx["sigma"]), conv_thr=float(fx["conv_thr"]),
+                               precision=prec)
+            try:
+                t = time.perf_counter()
+                res = scf.run()
+                dt = time.perf_counter() - t
+            finally:
+                scf.close()
+            out[label] = {"s_total": dt, "converged": bool(res.converged), "iterations": res.iterations,
+                          "davidson_solves": scf.solves, "davidson_iterations": scf.davidson_iterations,
+                          "etot_ry": res.etot, "etot_err_ha": abs(res.etot - float(fx["etot_ry"])) * HA_PER_RY,
+                          "etot_err_ha_per_atom": abs(res.etot - float(fx["etot_ry"])) * HA_PER_RY / 2,
+                          "max_eig_err_ha": float(np.abs(res.eigenvalues - fx["eigenvalues_ry"]).max()) * HA_PER_RY}
+            log(f"scf {label}: {dt:.2f} s, {res.iterations} iterations, {scf.solves} solves, E err "
+                f"{out[label]['etot_err_ha']:.1e} Ha")
+    finally:
+        ctx.close()
     return out
 
 
@@ -591,6 +628,9 @@ def main():
             out["gamma"] = batched_gamma(args)
         print(json.dumps(out))
         return 0
+    if args.mode == "scf":
+        print(json.dumps({"mode": "scf", **scf_section(args)}))
+        return 0
     if args.mode == "serve":
         out = {"mode": "serve", **serving(args)}
         if not args.no_stream:
@@ -732,6 +772,13 @@ def main():
     tols = {}
     if isinstance(davidson, dict) and "all_bands_1e-6_ha" in davidson:
         tols = {k: v["tolerance_ry"] for k, v in davidson["all_bands_1e-6_ha"].items() if isinstance(v, dict)}
+    scf = None
+    if not args.no_scf:
+        try:
+            scf = scf_section(args)
+        except Exception as e:  # reported, not hidden
+            scf = {"error": str(e)}
+            log(f"scf failed: {e}")
     batched = None if args.no_batch else {"kpoints": batched_kpoints(args), "gamma": batched_gamma(args, tols)}
     serve = None if args.no_serve else serving(args)
     if serve is not None and not args.no_stream:
@@ -761,6 +808,7 @@ def main():
         "roofline": roof,
         "gamma_trick": gamma,
         "davidson": davidson,
+        "scf": scf,
         "batched": batched,
         "serving": serve,
         "cpu_baseline": cpu,
