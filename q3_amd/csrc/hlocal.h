@@ -47,10 +47,11 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds, int nu
 // Two-level z-stick kernels (plane_tl.hip) for the n3 the plane sizes list holds.
 struct StickTlConfig {
     bool ok = false;
-    int spb = 0;  // sticks per workgroup
-    int lz = 0;   // LDS stride of a stick (complex elements)
+    int spb = 0;    // sticks per workgroup of the inverse pass
+    int spb_f = 0;  // and of the forward pass (the stick buffer's layout does not depend on it)
+    int lz = 0;     // LDS stride of a stick (complex elements)
     int threads = 0;
-    size_t lds = 0;
+    size_t lds = 0, lds_f = 0;
 };
 bool stick_tl_supported(int n3);
 StickTlConfig stick_tl_config(int n3, bool fp32);
