@@ -123,6 +123,10 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
             soff[i] = a.stick_col[s] * lc + q + q / B;
         }
     }
+    // FP64: work item q runs plane (q mod nvec) n3 + q / nvec, i3-major, so that the workgroups
+    // running at one time share a few i3 and their V(r) planes stay in L2; complex64: q = plane
+    const int nvec = nplanes / a.p3.n;
+    auto plane_of = [&](int q) { return persist ? (q % nvec) * a.p3.n + q / nvec : q; };
     auto prefetch = [&](int plane) {
         const cx<T>* src = zbuf + static_cast<long>(plane) * a.nst;
 #pragma unroll
@@ -131,8 +135,9 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
             if (s < a.nst) pre[i] = src[s];
         }
     };
-    if (persist && blockIdx.x < nplanes) prefetch(blockIdx.x);
-    for (int plane = blockIdx.x; plane < nplanes; plane += persist ? gridDim.x : nplanes) {
+    if (persist && blockIdx.x < nplanes) prefetch(plane_of(blockIdx.x));
+    for (int q = blockIdx.x; q < nplanes; q += persist ? gridDim.x : nplanes) {
+        const int plane = plane_of(q);
         const int z = plane % a.p3.n;
         for (int i = tid; i < ncol * lc; i += nt) P[i] = zero;
         __syncthreads();
@@ -147,7 +152,7 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
             P[a.stick_col[s] * lc + q + q / B] = zp[s];
         }
         __syncthreads();
-        if (persist && plane + static_cast<int>(gridDim.x) < nplanes) prefetch(plane + gridDim.x);
+        if (persist && q + static_cast<int>(gridDim.x) < nplanes) prefetch(plane_of(q + gridDim.x));
 
         // lanes of stage-1 / stage-B (B lanes per transform) and stage-2 / stage-A (A lanes)
         const int gB = lane / B, iB = lane - gB * B;
