@@ -112,7 +112,9 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
     // workgroups per CU, which overlap each other's loads): one plane per workgroup.
     constexpr bool persist = sizeof(T) == 8;
     constexpr int PF = persist ? 4 : 1;  // stick values per thread carried over; the rest load at once
-    cx<T> pre[PF];
+    cx<T> pre[PF];  // (initialised: a partly written array would be kept in scratch memory)
+#pragma unroll
+    for (int i = 0; i < PF; ++i) pre[i] = zero;
     int soff[PF];  // LDS slot of stick tid + i nt, the same in every plane
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
