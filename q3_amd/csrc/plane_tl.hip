@@ -105,8 +105,8 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
         }
     };
 
-    // FP64 (one workgroup per CU): persistent over planes p = blockIdx.x + k gridDim.x (plane
-    // p = band * n3 + z); the stick values of the next plane are loaded into registers while the
+    // FP64 (one workgroup per CU): persistent over work items q = blockIdx.x + k gridDim.x (the
+    // plane of each given by plane_of below); the stick values of the next plane are loaded into registers while the
     // current one is transformed, so their latency is hidden behind the passes (a block barrier
     // waits for LDS only), and the sticks' LDS slots stay in registers.  Complex64 (two
     // workgroups per CU, which overlap each other's loads): one plane per workgroup.
@@ -125,10 +125,22 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
             soff[i] = a.stick_col[s] * lc + q + q / B;
         }
     }
-    // FP64: work item q runs plane (q mod nvec) n3 + q / nvec, i3-major, so that the workgroups
-    // running at one time share a few i3 and their V(r) planes stay in L2; complex64: q = plane
+    // FP64: work items are spread over the XCDs by q mod 8 (workgroups are dispatched to the
+    // eight XCDs round-robin and the grid is a multiple of 8), so when 8 divides n3, item q runs
+    // band j mod nvec at i3 = 8 (j / nvec) + (q mod 8), j = q / 8: each XCD transforms only the
+    // planes i3 = x mod 8, i3-major, its concurrent workgroups share one i3 and every V(r) plane
+    // is read into one XCD's L2 once.  Otherwise band q mod nvec at i3 = q / nvec.  Both are
+    // bijections of [0, nplanes).  Complex64: q = plane.
     const int nvec = nplanes / a.p3.n;
-    auto plane_of = [&](int q) { return persist ? (q % nvec) * a.p3.n + q / nvec : q; };
+    const bool by_xcd = (a.p3.n & 7) == 0;
+    auto plane_of = [&](int q) {
+        if (!persist) return q;
+        if (by_xcd) {
+            const int j = q >> 3;
+            return (j % nvec) * a.p3.n + (j / nvec) * 8 + (q & 7);
+        }
+        return (q % nvec) * a.p3.n + q / nvec;
+    };
     auto prefetch = [&](int plane) {
         const cx<T>* src = zbuf + static_cast<long>(plane) * a.nst;
 #pragma unroll
