@@ -55,6 +55,25 @@ __device__ __forceinline__ void wave_order() {
 // Launch bound: 12 waves in complex64; 8 in FP64, whose stages with the prefetched stick values
 // need more than the 168 registers that 12 waves allow.
 constexpr int plane_tl_max_threads(bool fp64, int n) { return fp64 || n > 144 ? 512 : 768; }
+// transforms per wave group: 64 / max(A, B), so that one group's stage-2 / stage-A lanes (A per
+// transform) fit in a wave.  The FP64 column passes of planes of 72 and more take 64 / B
+// columns instead, so that stage 1 / B (B lanes per transform) fills the wave, and run stage 2 /
+// A in ceil(GC A / 64) sub-passes (cfg2, 96 = 12 x 8: 8 columns, two sub-passes; 0.98 vs 1.01 ms)
+__host__ __device__ constexpr int plane_group_base(int a, int b) { return 64 / (a > b ? a : b); }
+__host__ __device__ constexpr int plane_group_cols(bool fp64, int a, int b) {
+    return fp64 && a * b >= 72 ? 64 / b : plane_group_base(a, b);
+}
+__host__ __device__ constexpr int plane_group_rows(int a, int b) { return plane_group_base(a, b); }
+// Twiddles in plane_tl's LDS.  FP64: the stage 1 / B table [A][B] and the stage 2 / A table
+// [B][SA] (SA = A rounded up to a multiple of 8), laid out so that the 8 lanes the LDS serves
+// together read 8 different banks (read as w^(n2 k) from one table of N, the lanes n2 = 0..7 at
+// even k hit the same banks: about a quarter of the kernel's LDS cycles were such conflicts;
+// 0.98 -> 0.95 ms at cfg2).  Complex64: one table of N (the split tables measured slower there).
+__host__ __device__ constexpr bool plane_tw_split(bool fp64) { return fp64; }
+__host__ __device__ constexpr int plane_tw2_stride(int a) { return (a + 7) & ~7; }
+__host__ __device__ constexpr int plane_tw_entries(bool fp64, int a, int b) {
+    return plane_tw_split(fp64) ? a * b + b * plane_tw2_stride(a) : a * b;
+}
 // waves per SIMD the register allocation must allow: two complex64 workgroups of eight waves
 // per CU need four (128 registers); the radix-15/16 complex64 shapes need more and get two
 constexpr int plane_tl_min_waves(bool fp64, int a) { return fp64 || a > 12 ? 2 : 4; }
@@ -66,21 +85,41 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
                                                  int lr, int nrw, int nplanes, cx<T>* __restrict__ pglob) {
     constexpr int N = A * B;
     constexpr int S = B + 1;  // stored length of a block of B slots
-    constexpr int G = 64 / (A > B ? A : B);
+    constexpr int GC = plane_group_cols(sizeof(T) == 8, A, B), GR = plane_group_rows(A, B);
+    constexpr int NSC = (GC * A + 63) / 64, NSR = (GR * A + 63) / 64;  // stage-2 / stage-A sub-passes
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);  // exp(-2 pi i j / N), j < N
-    int* colmap = reinterpret_cast<int*>(tw + N);    // i1 -> occupied column or -1
+    // twiddles w_N^j = exp(-2 pi i j / N).  Split (FP64): stage 1 / B (lane n2) reads w^(n2 k)
+    // at tw1[k B + n2], stage 2 / A (task k1) reads w^(k1 m) at tw2[m SA + k1]; otherwise tw[j]
+    constexpr bool split = plane_tw_split(sizeof(T) == 8);
+    constexpr int SA = plane_tw2_stride(A);
+    cx<T>* tw = reinterpret_cast<cx<T>*>(smem_raw);
+    cx<T>* tw1 = tw;
+    cx<T>* tw2 = tw + N;
+    int* colmap = reinterpret_cast<int*>(tw + plane_tw_entries(sizeof(T) == 8, A, B));  // i1 -> column or -1
+    auto w1 = [&](int n2, int k) -> cx<T> { return split ? tw1[k * B + n2] : tw[n2 * k]; };
+    auto w2 = [&](int k1, int m) -> cx<T> { return split ? tw2[m * SA + k1] : tw[k1 * m]; };
     cx<T>* lds_next = reinterpret_cast<cx<T>*>(colmap + ((N + 3) & ~3));
     const int ncol = a.ncol;
     cx<T>* P = PG ? pglob + static_cast<long>(blockIdx.x) * ncol * lc : lds_next;  // [ncol][lc]
     const int tid = threadIdx.x, nt = blockDim.x;
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
-    cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * G * lr;  // this wave's [G][lr] (waves < nrw)
+    cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * GR * lr;  // this wave's [GR][lr] (waves < nrw)
     const cx<T> zero = {T(0), T(0)};
 
     for (int i = tid; i < N; i += nt) {
-        tw[i] = twg[i];
+        if (split) {
+            const int k = i / B, n2 = i - k * B;
+            tw1[i] = twg[n2 * k];
+        } else {
+            tw[i] = twg[i];
+        }
         colmap[i] = -1;
+    }
+    if (split) {
+        for (int i = tid; i < B * SA; i += nt) {
+            const int m = i / SA, k1 = i - m * SA;
+            if (k1 < A) tw2[i] = twg[k1 * m];
+        }
     }
     __syncthreads();
     for (int c = tid; c < ncol; c += nt) colmap[a.col_i1[c]] = c;
@@ -168,13 +207,13 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
         __syncthreads();
         if (persist && q + static_cast<int>(gridDim.x) < nplanes) prefetch(plane_of(q + gridDim.x));
 
-        // lanes of stage-1 / stage-B (B lanes per transform) and stage-2 / stage-A (A lanes)
+        // stage 1 / stage B: lane gB * B + n2 (B lanes per transform); stage 2 / stage A: task
+        // t = lane + 64 s of sub-pass s is (transform t / A, index t mod A)
         const int gB = lane / B, iB = lane - gB * B;
-        const int gA = lane / A, iA = lane - gA * A;
 
-        // columns, backward: stage 1 (lane n2 = iB), stage 2 (lane k1 = iA)
-        for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
-            if (gB < G && c0 + gB < ncol) {
+        // columns, backward: stage 1 (lane n2 = iB), stage 2 (task k1)
+        for (int c0 = wave * GC; c0 < ncol; c0 += nw * GC) {
+            if (gB < GC && c0 + gB < ncol) {
                 const int n2 = iB;
                 cx<T>* col = P + (c0 + gB) * lc + n2;
                 cx<T> v[A];
@@ -182,35 +221,43 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
                 for (int j = 0; j < A; ++j) v[j] = col[j * S];
                 Dft<T, A>::run(v, T(1));
 #pragma unroll
-                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], w1(n2, k));
 #pragma unroll
                 for (int k = 0; k < A; ++k) col[k * S] = v[k];
             }
             wave_order();
-            if (gA < G && c0 + gA < ncol) {
-                cx<T>* blk = P + (c0 + gA) * lc + iA * S;
-                cx<T> u[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) u[j] = blk[j];
-                Dft<T, B>::run(u, T(1));
+            for (int sp = 0; sp < NSC; ++sp) {
+                const int t = lane + 64 * sp, g = t / A;
+                if (g < GC && c0 + g < ncol) {
+                    cx<T>* blk = P + (c0 + g) * lc + (t - g * A) * S;
+                    cx<T> u[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) blk[j] = u[j];
+                    for (int j = 0; j < B; ++j) u[j] = blk[j];
+                    Dft<T, B>::run(u, T(1));
+#pragma unroll
+                    for (int j = 0; j < B; ++j) blk[j] = u[j];
+                }
             }
             wave_order();
         }
         __syncthreads();
 
         // rows; row i2 of the plane sits at slot (i2 % A) * S + i2 / A of each column
-        for (int r0 = wave * G; wave < nrw && r0 < N; r0 += nrw * G) {
-            // this lane's V(r) values for its stage-2 role, loaded ahead of stage 1 so that their
+        for (int r0 = wave * GR; wave < nrw && r0 < N; r0 += nrw * GR) {
+            // V(r) values of this lane's stage-2 tasks, loaded ahead of stage 1 so that their
             // latency overlaps it
-            T vv[B];
-            if (gA < G && r0 + gA < N) {
-                const T* vrow = vplane + (static_cast<long>(z) * N + r0 + gA) * N + iA;
+            T vv[NSR][B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) vv[j] = vrow[A * j];
+            for (int sp = 0; sp < NSR; ++sp) {
+                const int t = lane + 64 * sp, g = t / A, k1 = t - g * A;
+                if (g < GR && r0 + g < N) {
+                    const T* vrow = vplane + (static_cast<long>(z) * N + r0 + g) * N + k1;
+#pragma unroll
+                    for (int j = 0; j < B; ++j) vv[sp][j] = vrow[A * j];
+                }
             }
-            if (gB < G && r0 + gB < N) {  // backward stage 1, lane n2
+            if (gB < GR && r0 + gB < N) {  // backward stage 1, lane n2
                 const int n2 = iB, i2 = r0 + gB;
                 const int pp = (i2 % A) * S + i2 / A;
                 cx<T> v[A];
@@ -222,29 +269,32 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
                 }
                 Dft<T, A>::run(v, T(1));
 #pragma unroll
-                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], tw[n2 * k]);
+                for (int k = 1; k < A; ++k) v[k] = cmulc(v[k], w1(n2, k));
                 cx<T>* row = Rw + gB * lr + n2;
 #pragma unroll
                 for (int k = 0; k < A; ++k) row[k * S] = v[k];
             }
             wave_order();
-            if (gA < G && r0 + gA < N) {  // backward stage 2, x V(r)/N, forward stage A; lane k1
-                const int k1 = iA;
-                cx<T>* blk = Rw + gA * lr + k1 * S;
-                cx<T> u[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) u[j] = blk[j];
-                Dft<T, B>::run(u, T(1));
+            for (int sp = 0; sp < NSR; ++sp) {  // backward stage 2, x V(r)/N, forward stage A; task (row, k1)
+                const int t = lane + 64 * sp, g = t / A, k1 = t - g * A;
+                if (g < GR && r0 + g < N) {
+                    cx<T>* blk = Rw + g * lr + k1 * S;
+                    cx<T> u[B];
 #pragma unroll
-                for (int j = 0; j < B; ++j) u[j] = cscale(u[j], vv[j]);
-                Dft<T, B>::run(u, T(-1));
+                    for (int j = 0; j < B; ++j) u[j] = blk[j];
+                    Dft<T, B>::run(u, T(1));
 #pragma unroll
-                for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[k1 * m]);
+                    for (int j = 0; j < B; ++j) u[j] = cscale(u[j], vv[sp][j]);
+                    Dft<T, B>::run(u, T(-1));
 #pragma unroll
-                for (int j = 0; j < B; ++j) blk[j] = u[j];
+                    for (int m = 1; m < B; ++m) u[m] = cmul(u[m], w2(k1, m));
+#pragma unroll
+                    for (int j = 0; j < B; ++j) blk[j] = u[j];
+                }
             }
             wave_order();
-            if (gB < G && r0 + gB < N) {  // forward stage B, lane m2; outputs i1 = m2 + B m1, occupied only
+            if (gB < GR && r0 + gB < N) {  // forward stage B, lane m2; outputs i1 = m2 + B m1, occupied only
                 const int m2 = iB, i2 = r0 + gB;
                 const int pp = (i2 % A) * S + i2 / A;
                 const cx<T>* row = Rw + gB * lr + m2;
@@ -262,22 +312,26 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
         }
         __syncthreads();
 
-        // columns, forward: stage A (lane n1 = iA, block n1 * S holds elements n1 + A n2), stage B
-        for (int c0 = wave * G; c0 < ncol; c0 += nw * G) {
-            if (gA < G && c0 + gA < ncol) {
-                const int n1 = iA;
-                cx<T>* blk = P + (c0 + gA) * lc + n1 * S;
-                cx<T> u[B];
+        // columns, forward: stage A (task n1, block n1 * S holds elements n1 + A n2), stage B
+        for (int c0 = wave * GC; c0 < ncol; c0 += nw * GC) {
 #pragma unroll
-                for (int j = 0; j < B; ++j) u[j] = blk[j];
-                Dft<T, B>::run(u, T(-1));
+            for (int sp = 0; sp < NSC; ++sp) {
+                const int t = lane + 64 * sp, g = t / A;
+                if (g < GC && c0 + g < ncol) {
+                    const int n1 = t - g * A;
+                    cx<T>* blk = P + (c0 + g) * lc + n1 * S;
+                    cx<T> u[B];
 #pragma unroll
-                for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
+                    for (int j = 0; j < B; ++j) u[j] = blk[j];
+                    Dft<T, B>::run(u, T(-1));
 #pragma unroll
-                for (int j = 0; j < B; ++j) blk[j] = u[j];
+                    for (int m = 1; m < B; ++m) u[m] = cmul(u[m], w2(n1, m));
+#pragma unroll
+                    for (int j = 0; j < B; ++j) blk[j] = u[j];
+                }
             }
             wave_order();
-            if (gB < G && c0 + gB < ncol) {
+            if (gB < GC && c0 + gB < ncol) {
                 cx<T>* col = P + (c0 + gB) * lc + iB;
                 cx<T> v[A];
 #pragma unroll
@@ -463,11 +517,11 @@ PlaneTlConfig plane_tl_config(int n, int ncol, bool fp32, size_t max_lds, int nu
     for (const TlShape& s : kTlShapes)
         if (s.n == n) A = s.a, B = s.b;
     if (A == 0) return cfg;
-    const int G = 64 / std::max(A, B);  // transforms per wave group (plane_tl)
+    const int G = plane_group_rows(A, B);  // rows per wave group (plane_tl)
     const size_t cs = fp32 ? 8 : 16;
     cfg.lc = odd_at_least(n + A);
     cfg.lr = cfg.lc;
-    const size_t base = cs * n + 4 * static_cast<size_t>((n + 3) & ~3);
+    const size_t base = cs * plane_tw_entries(!fp32, A, B) + 4 * static_cast<size_t>((n + 3) & ~3);
     const size_t plane = cs * static_cast<size_t>(ncol) * cfg.lc;
     const size_t per_wave = cs * static_cast<size_t>(G) * cfg.lr;  // a wave's rows in flight
     // the plane goes to LDS when at least four waves' rows fit beside it
