@@ -65,10 +65,11 @@ __host__ __device__ constexpr int plane_group_cols(bool fp64, int a, int b) {
 }
 __host__ __device__ constexpr int plane_group_rows(int a, int b) { return plane_group_base(a, b); }
 // Twiddles in plane_tl's LDS.  FP64: the stage 1 / B table [A][B] and the stage 2 / A table
-// [B][SA] (SA = A rounded up to a multiple of 8), laid out so that the 8 lanes the LDS serves
-// together read 8 different banks (read as w^(n2 k) from one table of N, the lanes n2 = 0..7 at
-// even k hit the same banks: about a quarter of the kernel's LDS cycles were such conflicts;
-// 0.98 -> 0.95 ms at cfg2).  Complex64: one table of N (the split tables measured slower there).
+// [B][SA] (SA = A rounded up to a multiple of 8), laid out as the lanes read them, so that a
+// lane's reads of one table are one base address plus immediate offsets (w^(n2 k) read from one
+// table of N needs an address per k): 0.98 -> 0.95 ms at cfg2, 190 instead of 218 VGPRs, the
+// bank-conflict cycles about unchanged.  Complex64: one table of N (the split tables measured
+// no faster there).
 __host__ __device__ constexpr bool plane_tw_split(bool fp64) { return fp64; }
 __host__ __device__ constexpr int plane_tw2_stride(int a) { return (a + 7) & ~7; }
 __host__ __device__ constexpr int plane_tw_entries(bool fp64, int a, int b) {
@@ -96,8 +97,6 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
     cx<T>* tw1 = tw;
     cx<T>* tw2 = tw + N;
     int* colmap = reinterpret_cast<int*>(tw + plane_tw_entries(sizeof(T) == 8, A, B));  // i1 -> column or -1
-    auto w1 = [&](int n2, int k) -> cx<T> { return split ? tw1[k * B + n2] : tw[n2 * k]; };
-    auto w2 = [&](int k1, int m) -> cx<T> { return split ? tw2[m * SA + k1] : tw[k1 * m]; };
     cx<T>* lds_next = reinterpret_cast<cx<T>*>(colmap + ((N + 3) & ~3));
     const int ncol = a.ncol;
     cx<T>* P = PG ? pglob + static_cast<long>(blockIdx.x) * ncol * lc : lds_next;  // [ncol][lc]
@@ -105,6 +104,18 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
     const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
     cx<T>* Rw = (PG ? lds_next : P + ncol * lc) + wave * GR * lr;  // this wave's [GR][lr] (waves < nrw)
     const cx<T> zero = {T(0), T(0)};
+    // FP64 with A <= 12: stage 1 of every group of every plane runs on lane n2 = lane mod B, so
+    // its twiddles w^(n2 k) are held in registers (44 VGPRs at A = 12, within the 2-wave budget;
+    // A = 15, 16 would spill); 0.95 -> 0.93 ms at cfg2
+    constexpr bool w1_regs = sizeof(T) == 8 && A <= 12;
+    cx<T> w1r[A];
+#pragma unroll
+    for (int k = 0; k < A; ++k) w1r[k] = w1_regs ? twg[(lane % B) * k] : zero;
+    auto w1 = [&](int n2, int k) -> cx<T> {
+        if constexpr (w1_regs) return w1r[k];
+        return split ? tw1[k * B + n2] : tw[n2 * k];
+    };
+    auto w2 = [&](int k1, int m) -> cx<T> { return split ? tw2[m * SA + k1] : tw[k1 * m]; };
 
     for (int i = tid; i < N; i += nt) {
         if (split) {
