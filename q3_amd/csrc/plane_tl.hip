@@ -367,6 +367,14 @@ __global__ void __launch_bounds__(plane_tl_max_threads(sizeof(T) == 8, A * B),
     }
 }
 
+// streaming (non-temporal) load of the stick buffer Z, which a pass reads once
+template <typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T>* p) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return cx<T>{v.x, v.y};
+}
+
 // Workgroup -> (stick block, band) for the z-stick kernels.  Workgroups are dispatched to the
 // 8 XCDs round-robin by linear id, so id mod 8 names the XCD; giving every stick block of band b
 // to XCD b mod 8 makes each band's psi / Hpsi column live in one L2, where the scattered 16-byte
@@ -406,11 +414,29 @@ __global__ void __launch_bounds__(256) zsticks_inverse_tl(HLocalArgs a, const do
     const double2* src = reinterpret_cast<const double2*>(psi + band * ldpsi * 2);
     const int j0 = a.stick_off[s0], j1 = a.stick_off[s0 + nsb];
     const int base = s0 * N;
-    for (int j = j0 + tid; j < j1; j += nt) {
-        const int p = a.ent_pw[j];
-        const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
-        const double2 x = src[p];
-        buf[sl * lz + z + z / B] = cx<T>{T(x.x), T(x.y)};
+    // two entries per thread per round: their index loads, then their scattered psi loads, are
+    // issued together (cfg2 FP64 0.152 -> 0.145 ms; four or eight per round: 0.146, 0.148 ms;
+    // the same prefetch of psi ahead of the forward kernel's transforms made it slower)
+    constexpr int E = 2;
+    for (int jb = j0 + tid; jb < j1; jb += E * nt) {
+        int ep[E], eo[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int j = jb + e * nt;
+            ep[e] = -1;
+            eo[e] = 0;
+            if (j < j1) {
+                ep[e] = a.ent_pw[j];
+                const int pos = a.ent_pos[j] - base, sl = pos / N, z = pos - sl * N;
+                eo[e] = sl * lz + z + z / B;
+            }
+        }
+        double2 ex[E];
+#pragma unroll
+        for (int e = 0; e < E; ++e) ex[e] = ep[e] >= 0 ? src[ep[e]] : double2{0.0, 0.0};
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (ep[e] >= 0) buf[eo[e]] = cx<T>{T(ex[e].x), T(ex[e].y)};
     }
     __syncthreads();
     for (int t = tid; t < nsb * B; t += nt) {
@@ -465,7 +491,13 @@ __global__ void __launch_bounds__(256) zsticks_forward_tl(HLocalArgs a, const cx
         const int n1 = t / nsb, sl = t - n1 * nsb;
         cx<T> u[B];
 #pragma unroll
-        for (int j = 0; j < B; ++j) u[j] = in[static_cast<long>(n1 + A * j) * a.nst + sl];
+        for (int j = 0; j < B; ++j) {
+            // FP64: Z read as a stream, so that it does not push this band's psi and Hpsi lines
+            // (gathered and scattered below) out of L2 (cfg2 0.245 -> 0.226 ms; complex64 ran
+            // 9% slower with it)
+            const cx<T>* q = in + static_cast<long>(n1 + A * j) * a.nst + sl;
+            u[j] = sizeof(T) == 8 ? ld_stream(q) : *q;
+        }
         Dft<T, B>::run(u, T(-1));
 #pragma unroll
         for (int m = 1; m < B; ++m) u[m] = cmul(u[m], tw[n1 * m]);
