@@ -14,8 +14,10 @@
 //   collapse to nb Ritz vectors when the basis is full, rebuild H_sub/S_sub   serial.hpp:825-945
 //   zhegv failure -> forced collapse and retry, else previous eigenvalues   davidson.hpp:520-590
 //
-// Mixed precision: the H-apply runs in complex64 (FFT, V multiply, projector GEMMs); the
-// basis, Gram products, Cholesky and Rayleigh-Ritz stay fp64.  With final_fp64_sweep the
+// Mixed precision: the H-apply runs in complex64 (FFT, V multiply, projector GEMMs), and so do
+// the products that form basis vectors (rotations, corrections) when the tolerance is at least
+// 1e-6 Ry; the basis storage, Gram products, Cholesky and Rayleigh-Ritz stay fp64.  With
+// final_fp64_sweep the
 // converged mixed basis is re-applied with the fp64 H and the iteration continues in fp64
 // until the fp64 test passes, so the returned eigenpairs satisfy the fp64 criterion.
 #include <hip/hip_runtime.h>
@@ -40,6 +42,15 @@ namespace {
 constexpr double kNormThreshold = 1e-14;  // iterative_solver_norm_threshold (tolerances.hpp:39)
 constexpr double kEmptyFactor = 5.0;      // tolerances.hpp:65
 constexpr double kEmptyFloor = 1e-5;      // tolerances.hpp:66
+// mixed phase with complex64 basis products: the basis is orthonormal to their rounding, so
+// S = V^H V deviates from I by about 1e-6 (n max|S - I| up to a few 1e-4 at M = 320); the
+// first-order S^(-1/2) (error (3/8) (n max|S - I|)^2 <= 1e-7) is used up to 5e-4 instead of a
+// Newton-Schulz iteration and a second solve
+constexpr double kLowdinBoundMixed = 5e-4;
+// complex64 basis products only for tolerances above their rounding floor: below it the mixed
+// phase cannot meet the |de| test and runs on until its cap (cfg1 SCF, whose Davidson tolerance
+// falls to 1e-13: 3,949 -> 4,423 iterations with them at every tolerance)
+constexpr double kMixedProductsMinTol = 1e-6;
 
 using clk = std::chrono::steady_clock;
 
@@ -87,11 +98,17 @@ struct Solver {
     HostStage& pin() { return h->ctx->pin; }
     double now_ms() { return std::chrono::duration<double, std::milli>(clk::now().time_since_epoch()).count(); }
 
-    // C (m x n) = alpha * op(A) * B + beta * C, fp64 everywhere
+    // mixed phase: the products that form basis vectors (A B: projections out of the corrections,
+    // rotations, collapse) run on complex64 MFMA from the complex128 arrays; the Gram products
+    // (A^H B: S, H_sub, projections onto the basis) stay FP64, so Rayleigh-Ritz stays exact on
+    // whatever subspace the basis spans.  cfg2: 0.198 -> 0.194 s per solve, same errors; the
+    // Gram products in complex64 as well: 0.235 s (the complex64 A^H B tiles are no faster here)
+    bool fp32_products = false;
+    // C (m x n) = alpha * op(A) * B + beta * C over complex128 arrays
     bool gemm(int opa, int m, int n, int k, double alpha, const void* A, long lda, const void* B, long ldb,
               double beta, void* Cm, long ldc) {
-        return ok(zgemm(GemmTypes::D_D_D_D, opa, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, work.get(),
-                        work_bytes, st),
+        const GemmTypes t = fp32_products && opa == 0 ? GemmTypes::D_D_F_D : GemmTypes::D_D_D_D;
+        return ok(zgemm(t, opa, m, n, k, alpha, A, lda, B, ldb, beta, Cm, ldc, work.get(), work_bytes, st),
                   "Davidson GEMM");
     }
 
@@ -142,7 +159,8 @@ struct Solver {
         hipError_t e = hipSuccess;
         const int info = subspace_eig(n, Hs_d.as<double>(), mmax, Ss_d.as<double>(), mmax, nvec, w.data(),
                                       X_d.as<double>(), n, eigw.get(), eigw.size(), work.get(), work_bytes,
-                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, &eig_stats, &h->ctx->prof, &pin());
+                                      h->ctx->max_lds, h->ctx->num_cus, st, &e, &eig_stats, &h->ctx->prof, &pin(),
+                                      fp32_products ? kLowdinBoundMixed : 1e-7);
         if (info < 0) {
             ok(e == hipSuccess ? hipErrorUnknown : e, "subspace eigensolver");
             return -1;
@@ -316,6 +334,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
     s.valt = s.ws.V2.as<double>();
     s.hvalt = s.ws.HV2.as<double>();
     int prec = opt->precision;
+    s.fp32_products = prec == Q3X_MIXED && opt->tolerance_ry >= kMixedProductsMinTol;
 
     // ---- seed (davidson.hpp:1000-1101)
     double t0 = s.now_ms();
@@ -490,6 +509,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
             if (!s.rotate(basis, nb)) return s.status;
             basis = nb;
             prec = Q3X_FP64;
+            s.fp32_products = false;
             if (!s.apply(0, nb, prec)) return s.status;
             if (!s.update(0, nb, nb)) return s.status;
             for (int b = 0; b < nb; ++b) prev[b] = w[b];

@@ -37,9 +37,12 @@ long barrier_redos_total();
 // work: subspace_eig_workspace(n, nvec) bytes; gemm_work: split-K workspace of zgemm.
 // num_cus: compute units of the device (bounds the workgroups of the multi-workgroup reduction).
 // pin (optional): pinned staging for the copies of eigenvalues and status words to the host.
+// lowdin_bound: the first-order factor S^(-1/2) ~ 1.5 I - 0.5 S is used when n max|S - I| is at
+// most this (its error is about (3/8) (n max|S - I|)^2), else Newton-Schulz; 1e-7 keeps the
+// factor to rounding, callers whose S carries complex64 rounding pass a looser bound.
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
                  int num_cus, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof = nullptr,
-                 HostStage* pin = nullptr);
+                 HostStage* pin = nullptr, double lowdin_bound = 1e-7);
 
 }  // namespace q3x

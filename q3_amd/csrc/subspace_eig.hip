@@ -1216,7 +1216,8 @@ size_t subspace_eig_workspace(int n, int nvec) { return layout(std::max(n, 1), s
 
 int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, int nvec, double* w_host, double* X,
                  long ldx, void* work, size_t work_bytes, void* gemm_work, size_t gemm_bytes, size_t max_lds,
-                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof, HostStage* pin) {
+                 int num_cus_hint, hipStream_t st, hipError_t* err, SubspaceEigStats* stats, Profiler* prof, HostStage* pin,
+                 double lowdin_bound) {
     // stage timing on the caller's profiler (events on this stream; nothing when it is off)
     int mark_prev = prof ? prof->mark(st) : -1;
     auto stage_done = [&](int id) {
@@ -1447,7 +1448,7 @@ int subspace_eig(int n, const double* H, long ldh, const double* S, long lds, in
     if (r != 0) return r;
     if (stats) stats->s_deviation = sdev;
     // first-order Loewdin factor error ~ (3/8) |E|_2^2 <= (3/8) (n max|E|)^2
-    if (!(sdev * n <= 1e-7)) {
+    if (!(sdev * n <= lowdin_bound)) {
         if ((r = newton_schulz(sdev)) != 0) return r;
         if ((r = solve_once(&sdev2)) != 0) return r;
     }

@@ -16,6 +16,8 @@ SHAPES = [  # (transa, m, n, k)
     ("C", 320, 160, 3001),     # Gram block
     ("N", 3001, 160, 320),     # rotation
     ("C", 17, 3, 100000),      # very deep split-K
+    ("N", 1100, 100, 64),      # row blocks not a multiple of 8 (the XCD grouping pads them)
+    ("C", 100, 70, 20000),     # split-K over a multiple of 8 chunks on their XCDs
 ]
 
 
