@@ -199,11 +199,42 @@ def gamma_hpsi(args, w, precs):
         for prec, label in precs:
             out[label] = _time_hpsi(args, ctx, h, d_psi, d_hpsi, nb, prec)
             log(f"gamma trick {label}: {out[label]['ms_per_step']:.3f} ms/step, {out[label]['hpsi_per_s']:.0f} Hpsi/s")
+        if args.mode == "full" and not args.no_davidson:
+            out["davidson"] = _gamma_davidson(args, ctx, h, w, psi0, precs)
     finally:
         L.q3x_device_free(ctx.h, d_psi)
         L.q3x_device_free(ctx.h, d_hpsi)
         h.close()
         ctx.close()
+    return out
+
+
+def _gamma_davidson(args, ctx, h, w, psi0, precs):
+    """cfg2 Davidson with the Gamma trick, the basis the reference builds for a Gamma-only run
+    (init.hpp:585-610), from the half-sphere starting block (seed 2), against the full-sphere
+    oracle fixture: the reference's settings, and every band held to 2e-6 Ry (2e-7 Ry if that
+    leaves a band above 1e-6 Ha)."""
+    nb, nocc = w.nbands, w.noccupied
+    fix = np.load(FIXTURE_CFG2)["eigenvalues_ry"]
+    for prec, _ in precs:  # untimed warm-up
+        _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, 2)
+    out = {"oracle_fixture": "tests/golden/cfg2_eigs.npz (full-sphere FP64 oracle Davidson)"}
+    for prec, label in precs:
+        eig, dt, rep = _solve(ctx, h, nb, nocc, psi0, prec, 2e-6, args.davidson_iters)
+        err = np.abs(eig - fix) * HA_PER_RY
+        r = {"reference_settings": {"s_per_solve": dt, "iterations": rep["iterations"],
+                                    "max_abs_err_ha_occupied": float(err[:nocc].max())}}
+        for tol in (2e-6, 2e-7):
+            eig, dt2, rep2 = _solve(ctx, h, nb, nb, psi0, prec, tol, 4 * args.davidson_iters)
+            e2 = float(np.abs(eig - fix).max() * HA_PER_RY)
+            r["all_bands"] = {"tolerance_ry": tol, "s_per_solve": dt2, "iterations": rep2["iterations"],
+                              "max_abs_err_ha_all": e2}
+            if e2 <= 1e-6:
+                break
+        log(f"gamma davidson {label}: reference settings {dt:.3f}s {rep['iterations']} it "
+            f"(occupied err {err[:nocc].max():.1e} Ha), all bands {r['all_bands']['s_per_solve']:.3f}s "
+            f"(max err {r['all_bands']['max_abs_err_ha_all']:.1e} Ha)")
+        out[label] = r
     return out
 
 
