@@ -722,9 +722,22 @@ __global__ __launch_bounds__(kThreads) void tridiag_coop(const z2* __restrict__ 
         // one wavefront: p and the pivot column, f = tn vn^H p / 2, q = p - f vn, the pivot column
         // with reflector k+1, the next reflector; the other wavefronts wait at the barrier below
         if (wv == 0) {
-            for (int i = lane; i < len; i += 64) {
-                p[i] = P[i];
-                colb[i] = Cc[i];
+            // p and the pivot column from the other workgroups: four rounds of loads in flight
+            // before their LDS stores (a loop of load pair, wait, store waited out every round)
+            for (int i0 = lane; i0 < len; i0 += 4 * 64) {
+                z2 pa[4], ca[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {  // unconditional (clamped) loads: all eight in flight
+                    const int i = min(i0 + 64 * u, len - 1);
+                    pa[u] = P[i];
+                    ca[u] = Cc[i];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {  // a clamped index stores entry len - 1's own value again
+                    const int i = min(i0 + 64 * u, len - 1);
+                    p[i] = pa[u];
+                    colb[i] = ca[u];
+                }
             }
             double f = 0.0;
             if (tn != 0.0) {
