@@ -229,8 +229,18 @@ __global__ void splitk_reduce(int m, int n, int ksplit, const TM* __restrict__ w
     if (idx >= total) return;
     const int j = static_cast<int>(idx / (2L * m));
     const int rem = static_cast<int>(idx - static_cast<long>(j) * 2 * m);
+    // partials summed in order z = 0, 1, ... (deterministic); eight loads are issued ahead of
+    // their adds (as one load, wait, add per partial the loop waited out every load)
     double s = 0.0;
-    for (int z = 0; z < ksplit; ++z) s += static_cast<double>(work[static_cast<long>(z) * total + idx]);
+    int z = 0;
+    for (; z + 8 <= ksplit; z += 8) {
+        TM v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = work[static_cast<long>(z + u) * total + idx];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += static_cast<double>(v[u]);
+    }
+    for (; z < ksplit; ++z) s += static_cast<double>(work[static_cast<long>(z) * total + idx]);
     const long o = 2L * (rem >> 1) + 2L * j * ldc + (rem & 1);
     double out = alpha * s;
     if (beta != 0.0) out += beta * static_cast<double>(C[o]);
