@@ -37,8 +37,10 @@ enum {
 /* precision of the H-apply / Davidson products */
 enum {
     Q3X_FP64 = 0,  /* everything complex128 */
-    Q3X_MIXED = 1  /* FFT, V multiply and projector GEMMs in complex64 with fp32 accumulation;
-                      Cholesky, Rayleigh-Ritz and final residual sweep in fp64 */
+    Q3X_MIXED = 1  /* FFT, V multiply and projector GEMMs in complex64 with fp32 accumulation,
+                      and in Davidson at tolerances of 1e-6 Ry and above also the products that
+                      form basis vectors; Gram products, Cholesky, Rayleigh-Ritz and the final
+                      residual sweep in fp64 */
 };
 
 typedef struct q3x_context_s* q3x_context;
