@@ -305,12 +305,15 @@ hipError_t zgemm_shape(int opa, int m, int n, int k, double alpha, const void* A
     // (the algorithmic 343 MB) instead of 977 MB in the same time: the kernel is not bound by
     // HBM, and the traffic saved is left to concurrent work.  The complex64 products ran 4% slower
     // with it and keep the plain order.
+    // The chunk count stays that multiple of 8 after the chunk length is rounded up to KC: chunks
+    // past the end of K are empty and add zero partials.
     const bool want_xmap = sizeof(TM) == 8 && ksplit >= 8;
-    if (want_xmap && force_split <= 0) ksplit -= ksplit % 8;
+    const bool keep8 = want_xmap && force_split <= 0;
+    if (keep8) ksplit -= ksplit % 8;
     int kchunk = (k + ksplit - 1) / ksplit;
     kchunk = ((kchunk + KC - 1) / KC) * KC;
     if (kchunk == 0) kchunk = KC;
-    ksplit = k > 0 ? (k + kchunk - 1) / kchunk : 1;
+    if (!keep8) ksplit = k > 0 ? (k + kchunk - 1) / kchunk : 1;
     int xmap = want_xmap && ksplit % 8 == 0 ? 1 : 0;
     dim3 grid(tm, tn, ksplit);
     // A B without a split (the projector update Hpsi += beta W): the column tiles of a row block
