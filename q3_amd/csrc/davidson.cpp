@@ -172,12 +172,15 @@ struct Solver {
         return info;
     }
 
-    // Orthogonalise candidates V[:, start:start+nc] against V[:, :start] and among themselves
-    // (serial.hpp:41-553).  Returns accepted count (compacted), -1 on device error.
-    int orthogonalize(int start, int nc) {
+    // Orthogonalise candidates against V[:, :start] and among themselves (serial.hpp:41-553); the
+    // result lands in V[:, start:start+nc].  The candidates are V[:, start:start+nc] itself, or,
+    // in_r, the workspace R (the corrections are formed there, so the final product cand U^-1 is
+    // written straight into V instead of into R and copied back).  Returns the accepted count
+    // (compacted), -1 on device error.
+    int orthogonalize(int start, int nc, bool in_r = false) {
         if (nc <= 0) return 0;
         const double t0 = now_ms();
-        double* cand = Vc(start);
+        double* cand = in_r ? R.as<double>() : Vc(start);
         if (start > 0) {
             if (!gram(start, nc, Vc(0), cand, C.get(), start)) return -1;
             if (!gemm(0, npw, nc, start, -1.0, Vc(0), npw, C.get(), start, 1.0, cand, npw)) return -1;
@@ -189,12 +192,24 @@ struct Solver {
         if (chol_upper(nc, Gh.data(), nc) == 0) {
             tri_inv_upper(nc, Gh.data(), nc);  // cand <- cand U^-1 (the ztrsm R,U,N,N of serial.hpp:337-348)
             if (!ok(pin().h2d(G.get(), Gh.data(), Gh.size() * 16, st), "H2D U^-1")) return -1;
-            if (!gemm(0, npw, nc, nc, 1.0, cand, npw, G.get(), nc, 0.0, R.get(), npw)) return -1;
-            if (!ok(hipMemcpyAsync(cand, R.get(), static_cast<size_t>(npw) * nc * 16, hipMemcpyDeviceToDevice, st),
-                    "D2D orthonormal block"))
-                return -1;
+            if (in_r) {
+                if (!gemm(0, npw, nc, nc, 1.0, cand, npw, G.get(), nc, 0.0, Vc(start), npw)) return -1;
+            } else {
+                if (!gemm(0, npw, nc, nc, 1.0, cand, npw, G.get(), nc, 0.0, R.get(), npw)) return -1;
+                if (!ok(hipMemcpyAsync(cand, R.get(), static_cast<size_t>(npw) * nc * 16, hipMemcpyDeviceToDevice,
+                                       st),
+                        "D2D orthonormal block"))
+                    return -1;
+            }
         } else {
-            // modified Gram-Schmidt fallback (serial.hpp:370-552)
+            // modified Gram-Schmidt fallback (serial.hpp:370-552), on the candidates moved into V
+            if (in_r) {
+                if (!ok(hipMemcpyAsync(Vc(start), cand, static_cast<size_t>(npw) * nc * 16, hipMemcpyDeviceToDevice,
+                                       st),
+                        "D2D candidates"))
+                    return -1;
+                cand = Vc(start);
+            }
             ++mgs_fallbacks;
             accepted = 0;
             std::vector<double> n2(1);
@@ -449,7 +464,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
                     ec[c] = w[src[c]];
                     nec[c] = -ec[c];
                 }
-                double* corr = s.Vc(basis);
+                double* corr = s.R.as<double>();  // formed in R, orthonormalised into V[:, basis:]
                 double* xc = s.Xc_d.as<double>();
                 // e, -e and the band indices of the corrections in one upload: [e | -e | idx]
                 std::vector<char> packed(static_cast<size_t>(nc) * 20);
@@ -482,7 +497,7 @@ q3x_status davidson_solve(q3x_hamiltonian h, int nb, const q3x_davidson_options*
             }
             if (!s.stage_sync()) return s.status;
             s.stage[6] += s.now_ms() - t0;
-            const int accepted = s.orthogonalize(basis, nc);
+            const int accepted = s.orthogonalize(basis, nc, true);
             if (accepted < 0) return s.status;
             if (!s.apply(basis, accepted, prec)) return s.status;
             const int old = basis;
