@@ -657,7 +657,7 @@ q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, cons
                                       nvec, w,
                                       static_cast<double*>(X), ldx, ctx->eig_work.get(), ctx->eig_work.size(),
                                       ctx->gemm_work.get(), ws, ctx->max_lds, ctx->num_cus, ctx->stream, &e, nullptr,
-                                      nullptr, &ctx->pin);
+                                      &ctx->prof, &ctx->pin);
         if (info < 0) return hip_ok(e, "q3x_hegv_device") ? Q3X_ERR_INTERNAL : Q3X_ERR_DEVICE;
         if (info > 0) {
             set_error("q3x_hegv_device: S is not positive definite");

@@ -134,3 +134,31 @@ def test_forced_barrier_giveup_is_redone_on_one_workgroup(gpu_context, n, nvec):
     lib().q3x_debug_force_barrier_giveups(0)
     assert lib().q3x_eig_barrier_redos_total() == before + 1
     _check(H, S, w, X, nvec)
+
+
+def test_hegv_device_stage_profile(gpu_context):
+    """q3x_hegv_device times its stages on the context's profiler (q3x_profile_*, ids 6-11:
+    prepare, GEMMs, tridiagonalisation, multisection, inverse iteration, back-transformation),
+    as the Davidson solve does; the solve is unchanged by the timing."""
+    import ctypes
+    from q3_amd._lib import lib
+    rng = np.random.default_rng(17)
+    n, nvec = 320, 160  # the multi-workgroup tridiagonalisation
+    H = _herm(n, rng)
+    B = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (0.3 / n)
+    S = np.eye(n) + B @ B.conj().T
+    tot, cnt = (ctypes.c_double * 12)(), (ctypes.c_long * 12)()
+    h = gpu_context.h
+    assert lib().q3x_profile_enable(h, 1) == 0
+    try:
+        assert lib().q3x_profile_read(h, 12, tot, cnt, 1) == 0
+        w, X = _solve(gpu_context, H, S, nvec)
+        assert lib().q3x_profile_read(h, 12, tot, cnt, 1) == 0
+    finally:
+        lib().q3x_profile_enable(h, 0)
+    _check(H, S, w, X, nvec)
+    names = [lib().q3x_profile_kernel_name(i).decode() for i in range(12)]
+    assert names[8] == "eig_tridiagonal", names
+    for i in range(6, 12):
+        assert cnt[i] >= 1 and tot[i] > 0, (names[i], list(cnt), list(tot))
+    assert all(cnt[i] == 0 for i in range(6)), list(cnt)  # no H|psi> kernel ran
