@@ -230,8 +230,8 @@ q3x_status q3x_hegv(int n, const double* H, const double* S, double* w, double* 
  * math/linear_algebra/gpu/lapack/init.hpp:77-82), solved on the device: H, S complex128 with
  * leading dimensions ldh, lds (upper triangles read); the lowest nvec eigenvalues ascending into
  * host w, their S-orthonormal eigenvectors into device X (n x nvec, ld ldx).  Synchronizes the
- * context stream.  Returns Q3X_OK, Q3X_ERR_NUMERIC (S not positive definite) or
- * Q3X_ERR_DEVICE. */
+ * context stream; with q3x_profile_enable on, its stages are timed under kernel ids 6-11.
+ * Returns Q3X_OK, Q3X_ERR_NUMERIC (S not positive definite) or Q3X_ERR_DEVICE. */
 q3x_status q3x_hegv_device(q3x_context ctx, int n, const void* H, long ldh, const void* S, long lds,
                            int nvec, double* w, void* X, long ldx);
 /* No counterpart in the reference.  Test hook of the device eigensolver: the next `count`
