@@ -59,9 +59,10 @@ def parse():
                    help="1: each stream request is a band path (one straight segment) solved with the "
                         "non-self-consistent k-loop (propagation + backward fix); 0: independent k-points")
     p.add_argument("--davidson-iters", type=int, default=60)
-    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "scf", "copy"], default="full",
+    p.add_argument("--mode", choices=["full", "hpsi", "davidson", "batch", "serve", "scf", "copy", "eig"], default="full",
                    help="full: the JSON line; hpsi: only the H|psi> loops (profiling runs); "
-                        "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; serve: the request server only; copy: a plain 1 GiB device copy, the reference for HBM byte counters")
+                        "davidson: one Davidson solve per precision (profiling runs); batch: the 64-k-point batch only; serve: the request server only; copy: a plain 1 GiB device copy, the reference for HBM byte counters; "
+                        "eig: the device subspace eigensolver at n = 160-880, per stage")
     p.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
     return p.parse_args()
 
@@ -640,6 +641,50 @@ def serving(args):
         return {"error": str(e)}
 
 
+def eig_section(steps):
+    """The device generalized eigensolver (q3x_hegv_device) on Davidson subspace sizes: n = 160,
+    320 (cfg2: nb, 2 nb), 640 and 880 (the largest served cells), H random Hermitian, S = I + a
+    positive perturbation of about 1e-12 (the overlap of the orthonormalised Davidson basis,
+    which takes the first-order Loewdin path: one tridiagonalisation per solve), the lowest n / 2
+    pairs; per-stage times and launches per solve from the context profiler (HIP events), the
+    wall time per solve, and the largest eigenvalue difference from scipy.linalg.eigh."""
+    import scipy.linalg as sl
+    from q3_amd import Context
+    from q3_amd._lib import check, lib
+    from q3_amd.planewave import DeviceArray, hegv_device
+    ctx = Context(0)
+    L = lib()
+    names = [L.q3x_profile_kernel_name(i).decode() for i in range(12)]
+    rng = np.random.default_rng(5)
+    out = {"mode": "eig", "solves_per_size": steps, "sizes": []}
+    for n in (160, 320, 640, 880):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        H = np.asfortranarray(A + A.conj().T)
+        B = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * (1e-6 / n)
+        S = np.asfortranarray(np.eye(n) + B @ B.conj().T)
+        nvec = n // 2
+        dH, dS, dX = DeviceArray.from_host(ctx, H), DeviceArray.from_host(ctx, S), DeviceArray(ctx, n * nvec * 16)
+        for _ in range(3):
+            w = hegv_device(ctx, n, dH, n, dS, n, nvec, dX, n)
+        tot, cnt = (ctypes.c_double * 12)(), (ctypes.c_long * 12)()
+        check(L.q3x_profile_enable(ctx.h, 1))
+        check(L.q3x_profile_read(ctx.h, 12, tot, cnt, 1))
+        t = time.perf_counter()
+        for _ in range(steps):
+            w = hegv_device(ctx, n, dH, n, dS, n, nvec, dX, n)
+        wall = (time.perf_counter() - t) / steps
+        check(L.q3x_profile_read(ctx.h, 12, tot, cnt, 1))
+        check(L.q3x_profile_enable(ctx.h, 0))
+        err = float(np.abs(w - sl.eigh(H, S, eigvals_only=True)[:nvec]).max())
+        row = {"n": n, "nvec": nvec, "wall_ms_per_solve": wall * 1e3, "max_abs_err_vs_scipy": err,
+               "stage_ms_per_solve": {names[i]: tot[i] / steps for i in range(6, 12)},
+               "stage_spans_per_solve": {names[i]: cnt[i] / steps for i in range(6, 12)}}
+        out["sizes"].append(row)
+        log(f"eig n={n}: {wall * 1e3:.2f} ms/solve, tridiagonal {tot[8] / steps:.3f} ms, err {err:.1e}")
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -651,6 +696,8 @@ def main():
     os.environ.setdefault("GPU_MAX_HW_QUEUES", str(max(4, min(16, args.batch_workers))))
     if args.mode == "copy":
         return copy_reference(args.steps)
+    if args.mode == "eig":
+        return eig_section(args.steps)
     if args.mode == "batch":
         out = {"mode": "batch"}
         if args.batch_part in ("both", "kpoints"):
